@@ -1,0 +1,206 @@
+"""Throughput of the MI355X Maxwell step on the 32^3-element N=7 FP64 box.
+
+Workload (one "step" = one 5-stage RK step of cem_maxwell_op_rk over the
+whole mesh): 32x32x32 periodic box on [0,2pi]^3, N=7 (16,777,216 GLL points
+x 6 fields), per-element eps ~ U[1,4] (seed 1), mu = 1, upwind flux, initial
+state a superposition of 8 plane waves with seeded wavevectors |k| <= 4
+(seed 2), dt = 0.25*dxmin.  Synthetic data, generated in-process.
+
+Metric: G(GLL-pt.field.stage)/s = npts*6*5*steps / seconds / 1e9, with the
+state resident in HBM for the whole timed region.
+
+Also reported:
+  roofline      the fused stage kernel (5 launches per step): algorithmic
+                32 B per GLL-pt.field.stage (read u, read k, write k, write
+                u) x 6 x npts per launch, over its average launch duration
+                from HIP events recorded on the library's stream;
+  cpu_baseline  the CPU oracle (a loop-for-loop C restatement of the
+                reference path, OpenMP) on the same workload for a bounded
+                number of steps on this host's cores;
+  parity        relative L2 (bm1-weighted) between the GPU and the CPU
+                oracle after those steps.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+B_ALG = 32.0   # bytes per GLL-pt.field.stage
+
+
+def _hip():
+    for name in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
+        try:
+            return C.CDLL(name)
+        except OSError:
+            continue
+    raise RuntimeError("libamdhip64.so not found")
+
+
+class StreamTimer:
+    """hipEvent pair on a given stream."""
+
+    def __init__(self, stream):
+        self.hip = _hip()
+        self.stream = C.c_void_p(stream)
+        self.a, self.b = C.c_void_p(), C.c_void_p()
+        assert self.hip.hipEventCreate(C.byref(self.a)) == 0
+        assert self.hip.hipEventCreate(C.byref(self.b)) == 0
+
+    def start(self):
+        assert self.hip.hipEventRecord(self.a, self.stream) == 0
+
+    def stop(self):
+        assert self.hip.hipEventRecord(self.b, self.stream) == 0
+
+    def ms(self):
+        assert self.hip.hipEventSynchronize(self.b) == 0
+        t = C.c_float()
+        assert self.hip.hipEventElapsedTime(C.byref(t), self.a, self.b) == 0
+        return float(t.value)
+
+
+def workload(nel=32, N=7):
+    from nekcem_amd.mesh import BoxMesh
+    from nekcem_amd import usr as cases
+    mesh = BoxMesh((nel, nel, nel), N)
+    n3 = mesh.n1 ** 3
+    eps = np.repeat(cases.element_eps(mesh.nelt, seed=1), n3)
+    x, y, z = mesh.coords()
+    H, E = cases.plane_waves(x, y, z, seed=2)
+    del x, y, z
+    return mesh, eps, H, E
+
+
+def load_traffic(npts):
+    """HBM bytes per stage launch from the committed rocprofv3 --pmc summary."""
+    p = os.path.join(ROOT, "profiles", "stage_kernel_box32_N7_traffic.json")
+    if not os.path.exists(p):
+        return None
+    d = json.load(open(p))
+    if d.get("npts") != npts:
+        return None
+    return d.get("hbm_bytes_per_launch")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--nel", type=int, default=32)
+    ap.add_argument("--N", type=int, default=7)
+    ap.add_argument("--cpu-steps", type=int, default=1,
+                    help="steps of the CPU oracle for cpu_baseline (0 = skip)")
+    ap.add_argument("--no-graph", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only: this build has no multi-GPU path "
+              f"(asked for --gpus {args.gpus}); the element sharding across GPUs is "
+              f"described in DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    from nekcem_amd.maxwell import MaxwellSolver
+
+    t0 = time.time()
+    mesh, eps, H, E = workload(args.nel, args.N)
+    npts = mesh.npts
+    solver = MaxwellSolver(mesh, eps=eps, graph=not args.no_graph)
+    dt = mesh.dt(0.25)
+    solver.set_state(H, E)
+    setup_s = time.time() - t0
+    print(f"[bench] setup {setup_s:.1f}s npts={npts} nelt={mesh.nelt} dt={dt:.3e} "
+          f"info={solver.info()}", file=sys.stderr, flush=True)
+
+    solver.step(dt, args.warmup)
+    solver.sync()
+    timer = StreamTimer(solver.stream())
+    solver.sync()
+    w0 = time.perf_counter()
+    timer.start()
+    solver.step(dt, args.steps)
+    timer.stop()
+    solver.sync()
+    wall = time.perf_counter() - w0
+    ev_ms = timer.ms()
+    solver.check_finite()
+    print(f"[bench] timed {args.steps} steps: wall {wall*1e3:.2f} ms, events {ev_ms:.2f} ms",
+          file=sys.stderr, flush=True)
+
+    units = npts * 6 * 5 * args.steps
+    value = units / wall / 1e9
+    launches = 5 * args.steps
+    avg_launch_s = ev_ms * 1e-3 / launches
+    alg_bytes = B_ALG * 6 * npts
+    achieved = alg_bytes / avg_launch_s / 1e9
+    traffic = load_traffic(npts)
+
+    # CPU baseline + parity on the same workload (bounded sample)
+    cpu = None
+    parity = None
+    if args.cpu_steps > 0:
+        from oracle import oracle as O
+        ref = O.BoxCase(O.BoxSpec(nel=mesh.nel, N=args.N), eps=eps)
+        cores = O.num_threads()
+        c0 = time.perf_counter()
+        out = ref.run(H, E, dt, args.cpu_steps)
+        cpu_s = time.perf_counter() - c0
+        cpu = {"value": npts * 30 * args.cpu_steps / cpu_s / 1e9, "unit": "G(GLL-pt*field*stage)/s",
+               "cores": cores, "kind": "port",
+               "sample": f"{args.cpu_steps} RK5 step(s) of the same {args.nel}^3-element N={args.N} "
+                         f"workload, C oracle with OpenMP on {cores} threads, {cpu_s:.1f} s"}
+        g = MaxwellSolver(mesh, eps=eps)
+        g.set_state(H, E)
+        g.step(dt, args.cpu_steps)
+        gH, gE = g.get_state()
+        g.close()
+        bm = mesh.bmn
+        worst = 0.0
+        for a, b in ((gH, out["H"]), (gE, out["E"])):
+            for c in range(3):
+                den = np.sqrt(np.sum(bm * b[c] ** 2))
+                if den > 0:
+                    worst = max(worst, float(np.sqrt(np.sum(bm * (a[c] - b[c]) ** 2)) / den))
+        parity = {"rel_l2_max_over_fields": worst, "steps": args.cpu_steps,
+                  "norm": "per field, sqrt(sum bm1 (gpu-cpu)^2)/sqrt(sum bm1 cpu^2)"}
+
+    line = {
+        "metric": "G(GLL-pt*field*stage)/s FP64, 32^3el N=7",
+        "value": value,
+        "unit": "G(GLL-pt*field*stage)/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"periodic box {args.nel}^3 hexahedra, N={args.N}, "
+                               f"{npts} GLL points x 6 fields, per-element eps U[1,4], "
+                               f"8 plane waves, upwind, CFL 0.25, RK5 (5 stages/step)",
+                   "nelt": mesh.nelt, "N": args.N, "npts": npts,
+                   "graph": not args.no_graph},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "nkc::stage_kernel (fused stage)",
+                     "avg_launch_us": avg_launch_s * 1e6,
+                     "alg_bytes_per_launch": alg_bytes},
+        "cpu_baseline": cpu,
+        "parity": parity,
+    }
+    solver.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,124 @@
+/*
+ * nekcem_hip.h -- C-ABI of libnekcem_hip.so, the MI355X (gfx950) DG-SEM
+ * Maxwell time step that drops in for NekCEM's cem_maxwell_op_rk.
+ *
+ * Conventions (NekCEM's, src/jl/name.h:32-41 and SIZE):
+ *   - double = Fortran REAL with -fdefault-real-8; index arrays are int32 and
+ *     1-based, exactly as the Fortran COMMON arrays hold them;
+ *   - volume index i + n1*j + n1^2*k + n1^3*e (column-major), face index
+ *     a + n1*b + n1^2*(ef-1) + 6*n1^2*e with ef in Ed's (preprocessor) order
+ *     1: s=-1, 2: r=+1, 3: s=+1, 4: r=-1, 5: t=-1, 6: t=+1;
+ *   - sizes are int64 (the reference's CUDA prototype used 32-bit sizes,
+ *     src/cem_mxm_gpu.cu:34-83);
+ *   - every entry point returns an int32 status, NKC_OK = 0; the message of
+ *     the last failure on the calling thread is nkc_last_error().
+ * Fortran binds these through iso_c_binding (nekcem_amd/fortran/
+ * nekcem_hip_mod.f90); Python through ctypes (nekcem_amd/_lib.py).
+ */
+#ifndef NEKCEM_HIP_H
+#define NEKCEM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NKC_OK 0
+#define NKC_ERR_ARG 1          /* bad handle, pointer or size */
+#define NKC_ERR_HIP 2          /* HIP runtime error */
+#define NKC_ERR_UNSUPPORTED 3  /* mesh/material form the kernels do not cover */
+#define NKC_ERR_BLOWUP 4       /* non-finite or > 1e20 state (chk_blowup) */
+#define NKC_ERR_NODEVICE 5     /* no HIP device visible */
+
+typedef struct nkc_handle nkc_handle;
+
+/* Everything cem_maxwell_init leaves in COMMON for the RK path, i.e. the
+ * device working set of acc_copy_all_in (src/cem_drive.F:397-566).
+ * Array sizes: npts = (N+1)^3*nelt, nxzfl = 6*(N+1)^2*nelt. */
+typedef struct nkc_desc {
+    int32_t N;                  /* polynomial order (SIZE lxi); lx1 = N+1 */
+    int32_t ifupwind;           /* 1: upwind flux (C0=1), 0: central (C0=0),
+                                   src/cem_maxwell.F:939-940 */
+    int64_t nelt;               /* elements on this GPU */
+    const double *dxm1;         /* (N+1)^2 derivative matrix, /DXYZ/ */
+    const double *wxm1;         /* (N+1) GLL weights, /WXYZ/ */
+    const double *rxmn, *rymn, *rzmn; /* npts cofactor metrics, /minputv/ */
+    const double *sxmn, *symn, *szmn;
+    const double *txmn, *tymn, *tzmn;
+    const double *jacm;         /* npts Jacobian */
+    const double *bmn;          /* npts mass jac*w3 */
+    const double *unxm, *unym, *unzm; /* nxzfl unit normals, /minputf/ */
+    const double *aream;        /* nxzfl face area weights */
+    const int32_t *cemface;     /* nxzfl, 1-based volume index, /cemfce1/ */
+    const int32_t *face_partner;/* nxzfl, 1-based index of the face node that
+                                   shares this node's gsh_face id, 0 = none.
+                                   Replaces the gs handle gsh_face
+                                   (src/nek5_connect11.F:2179-2233). */
+    const int32_t *cempec;      /* ncempec 1-based PEC face-node list, /bdry1/ */
+    int64_t ncempec;
+    const double *permittivity; /* npts, /materialv/ */
+    const double *permeability; /* npts */
+    const double *Z_0, *Z_1, *Y_0, *Y_1; /* nxzfl, /materialf/ after
+                                   cem_maxwell_materials (:262-325) */
+} nkc_desc;
+
+/* Upload the working set and build the on-device face-exchange tables.
+ * Mirror of acc_copy_all_in (src/cem_drive.F:397) plus the setup half of
+ * cem_maxwell_init (src/cem_maxwell.F:65-191).  Returns NKC_ERR_UNSUPPORTED
+ * for geometry other than axis-aligned affine hexahedra (box meshes). */
+int32_t nkc_create(const nkc_desc *d, nkc_handle **h);
+
+/* Mirror of acc_copy_all_out (src/cem_drive.F:482). */
+int32_t nkc_destroy(nkc_handle *h);
+
+/* !$ACC UPDATE DEVICE / HOST of hn, en, khn, ken (src/EMWAVE:4-23).  Each is
+ * 3 components at stride ld (= lpts1 in the reference) with ld >= npts.
+ * khn/ken may be NULL: set_state then zeroes k (a1 = 0 makes k from a
+ * previous step irrelevant), get_state skips them. */
+int32_t nkc_set_state(nkc_handle *h, const double *hn, const double *en,
+                      const double *khn, const double *ken, int64_t ld);
+int32_t nkc_get_state(nkc_handle *h, double *hn, double *en, double *khn,
+                      double *ken, int64_t ld);
+
+/* nsteps calls of cem_maxwell_op_rk (src/cem_maxwell.F:327-345): 5-stage
+ * low-storage RK per step, each stage one fused kernel, the step replayed
+ * from a captured hipGraph.  *time += nsteps*dt (time_advancing_pde,
+ * src/cem_drive.F:639).  Asynchronous on the handle's stream; the next
+ * get_state or nkc_sync waits for it. */
+int32_t nkc_step(nkc_handle *h, double *time, double dt, int64_t nsteps);
+
+/* One evaluation of cem_maxwell_op (src/cem_maxwell.F:484-508) on the
+ * current state: reshn/resen = inverse-mass-scaled residual, 3 components at
+ * stride ld.  Test entry point for the stage composition. */
+int32_t nkc_residual(nkc_handle *h, double rktime, double *reshn, double *resen,
+                     int64_t ld);
+
+/* chk_blowup (src/cem_common.F:1150-1165): *maxabs = max |u| over the six
+ * fields; returns NKC_ERR_BLOWUP if a value is non-finite or above 1e20. */
+int32_t nkc_check_finite(nkc_handle *h, double *maxabs);
+
+/* Wait for all work queued on the handle's stream. */
+int32_t nkc_sync(nkc_handle *h);
+
+/* The HIP stream (hipStream_t) every kernel of this handle runs on, so a
+ * caller can time it with events or order its own work after it. */
+int32_t nkc_stream(nkc_handle *h, void **stream);
+
+/* Facts about the compiled handle: out[0] = npts, out[1] = material mode
+ * (0 per element, 1 per node), out[2] = elements per workgroup,
+ * out[3] = threads per workgroup, out[4] = LDS bytes per workgroup,
+ * out[5] = graph capture enabled (1/0). */
+int32_t nkc_info(nkc_handle *h, int64_t out[8]);
+
+/* Enable (1, default) or disable (0) hipGraph replay in nkc_step. */
+int32_t nkc_set_graph(nkc_handle *h, int32_t enable);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *nkc_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NEKCEM_HIP_H */

@@ -1,0 +1,194 @@
+"""Host-side mirror of NekCEM's Maxwell RK driver over the HIP step.
+
+Names follow the reference so a NekCEM user finds the same entry points:
+
+* ``cem_maxwell_init(mesh, eps, mu, ifupwind)`` -- cem_maxwell_init +
+  cem_maxwell_materials + acc_copy_all_in (src/cem_maxwell.F:65-191,
+  262-325; src/cem_drive.F:397): builds the nkc_desc and calls nkc_create;
+* ``MaxwellSolver.cem_maxwell_init_fields(userini)`` -- userini hook
+  (src/cem_maxwell.F:193-211);
+* ``MaxwellSolver.cem_maxwell_op_rk()`` -- one RK5 step
+  (src/cem_maxwell.F:327-345);
+* ``MaxwellSolver.time_advancing_pde(nsteps, userchk)`` -- the time loop of
+  src/cem_drive.F:568-657 with the ``userchk`` hook called after each step,
+  which may stop the run like ``exitt(1)`` does;
+* ``cem_error(u, exact, bm1)`` -- mass-weighted L2 and Linf
+  (src/cem_common.F:1335-1355).
+
+Field arrays are (3, npts) float64 numpy arrays, H = (Hx, Hy, Hz) and
+E = (Ex, Ey, Ez), i.e. hn(lpts1,3) / en(lpts1,3) with lpts1 = npts.
+Every step runs in libnekcem_hip.so on the GPU; nothing here computes the
+operator.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .mesh import BoxMesh
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+
+def _d(a):
+    return None if a is None else a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return None if a is None else a.ctypes.data_as(_ip)
+
+
+class RunStopped(RuntimeError):
+    """Raised when userchk asks to stop (the reference calls exitt(1))."""
+
+
+class MaxwellSolver:
+    """One nkc handle: the device-resident Maxwell state of a mesh."""
+
+    def __init__(self, mesh: BoxMesh, eps=None, mu=None, ifupwind=True, graph=True):
+        L = _lib.load()
+        self.mesh = mesh
+        self.npts = mesh.npts
+        eps, mu, Z0, Z1, Y0, Y1 = mesh.materials(eps, mu)
+        self.eps, self.mu = eps, mu
+        keep = dict(eps=eps, mu=mu, Z0=Z0, Z1=Z1, Y0=Y0, Y1=Y1)
+        d = _lib.NkcDesc()
+        d.N = mesh.N
+        d.ifupwind = 1 if ifupwind else 0
+        d.nelt = mesh.nelt
+        for name in ("dxm1", "wxm1", "rxmn", "rymn", "rzmn", "sxmn", "symn", "szmn",
+                     "txmn", "tymn", "tzmn", "jacm", "bmn", "unxm", "unym", "unzm", "aream"):
+            setattr(d, name, _d(getattr(mesh, name)))
+        d.cemface = _i(mesh.cemface)
+        d.face_partner = _i(mesh.face_partner)
+        d.cempec = _i(mesh.cempec) if mesh.ncempec else None
+        d.ncempec = mesh.ncempec
+        d.permittivity, d.permeability = _d(eps), _d(mu)
+        d.Z_0, d.Z_1, d.Y_0, d.Y_1 = _d(Z0), _d(Z1), _d(Y0), _d(Y1)
+        h = C.c_void_p()
+        _lib.check(L.nkc_create(C.byref(d), C.byref(h)))
+        del keep
+        self._h = h
+        self._L = L
+        self.time = 0.0
+        self.istep = 0
+        self.dt = None
+        if not graph:
+            _lib.check(L.nkc_set_graph(h, 0))
+
+    # -- state ------------------------------------------------------------
+    def set_state(self, H, E, kH=None, kE=None):
+        n = self.npts
+        H = np.ascontiguousarray(H, dtype=np.float64).reshape(3, n)
+        E = np.ascontiguousarray(E, dtype=np.float64).reshape(3, n)
+        kH = None if kH is None else np.ascontiguousarray(kH, dtype=np.float64).reshape(3, n)
+        kE = None if kE is None else np.ascontiguousarray(kE, dtype=np.float64).reshape(3, n)
+        _lib.check(self._L.nkc_set_state(self._h, _d(H), _d(E), _d(kH), _d(kE), n))
+
+    def get_state(self, with_k=False):
+        n = self.npts
+        H, E = np.empty((3, n)), np.empty((3, n))
+        kH = np.empty((3, n)) if with_k else None
+        kE = np.empty((3, n)) if with_k else None
+        _lib.check(self._L.nkc_get_state(self._h, _d(H), _d(E), _d(kH), _d(kE), n))
+        return (H, E, kH, kE) if with_k else (H, E)
+
+    # -- operator ---------------------------------------------------------
+    def residual(self, rktime=0.0):
+        n = self.npts
+        rH, rE = np.empty((3, n)), np.empty((3, n))
+        _lib.check(self._L.nkc_residual(self._h, rktime, _d(rH), _d(rE), n))
+        return rH, rE
+
+    def step(self, dt, nsteps=1):
+        t = C.c_double(self.time)
+        _lib.check(self._L.nkc_step(self._h, C.byref(t), float(dt), int(nsteps)))
+        self.time = t.value
+        self.istep += int(nsteps)
+
+    def sync(self):
+        _lib.check(self._L.nkc_sync(self._h))
+
+    def check_finite(self):
+        m = C.c_double()
+        _lib.check(self._L.nkc_check_finite(self._h, C.byref(m)))
+        return m.value
+
+    def stream(self):
+        s = C.c_void_p()
+        _lib.check(self._L.nkc_stream(self._h, C.byref(s)))
+        return s.value
+
+    def info(self):
+        out = (C.c_int64 * 8)()
+        _lib.check(self._L.nkc_info(self._h, out))
+        keys = ("npts", "mat_mode", "elems_per_wg", "threads_per_wg", "lds_bytes", "graph", "fs", "nelt")
+        return dict(zip(keys, list(out)))
+
+    def close(self):
+        if self._h:
+            _lib.check(self._L.nkc_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- reference-named driver ------------------------------------------
+    def cem_maxwell_init_fields(self, userini):
+        """userini(t, xm1, ym1, zm1) -> (H, E); sets the state at self.time."""
+        x, y, z = self.mesh.coords()
+        H, E = userini(self.time, x, y, z)
+        self.set_state(H, E)
+
+    def cem_maxwell_op_rk(self):
+        """One 5-stage low-storage RK step of size self.dt."""
+        if self.dt is None:
+            raise RuntimeError("dt not set (use set_dt)")
+        self.step(self.dt, 1)
+
+    def set_dt(self, p12):
+        """set_dt semantics: p12 < 0 is dt itself, p12 > 0 a CFL number."""
+        self.dt = self.mesh.dt(p12)
+        return self.dt
+
+    def time_advancing_pde(self, nsteps, userchk=None, chk_every=None):
+        """Run nsteps steps; call userchk(self) after each step for which
+        istep <= 10 or istep % chk_every == 0 (the reference's iocomm rule),
+        or only at the end when chk_every is None.  userchk returning False
+        stops the run with RunStopped (exitt(1))."""
+        done = 0
+        while done < nsteps:
+            if userchk is None:
+                self.step(self.dt, nsteps - done)
+                return
+            if chk_every is None:
+                self.step(self.dt, nsteps - done)
+                done = nsteps
+            else:
+                # advance to the next check point
+                nxt = self.istep + 1
+                if nxt > 10:
+                    nxt = ((self.istep // chk_every) + 1) * chk_every
+                n = min(nxt - self.istep, nsteps - done)
+                self.step(self.dt, n)
+                done += n
+            if userchk(self) is False:
+                raise RunStopped(f"userchk stopped the run at istep={self.istep}")
+
+
+def cem_maxwell_init(mesh: BoxMesh, eps=None, mu=None, ifupwind=True, graph=True):
+    return MaxwellSolver(mesh, eps=eps, mu=mu, ifupwind=ifupwind, graph=graph)
+
+
+def cem_error(u, exact, bm1):
+    """(L2, Linf) of exact - u, L2 = sqrt(sum(bm1 err^2) / sum(bm1))."""
+    err = np.asarray(exact, dtype=np.float64) - np.asarray(u, dtype=np.float64)
+    l2 = float(np.sum(err * bm1 * err) / np.sum(bm1))
+    l2 = np.sqrt(l2) if l2 > 0 else l2
+    return float(l2), float(np.max(np.abs(err)))

@@ -1,0 +1,369 @@
+"""CPU oracle for NekCEM's 3-D DG-SEM Maxwell RK step -- TEST INFRASTRUCTURE.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline
+leg may import this module, and only as the checker / the timed CPU baseline.
+The product package ``nekcem_amd`` never imports it.
+
+The arithmetic lives in ``nekcem_oracle.c`` (built by ``oracle/Makefile`` into
+``oracle/_build/libnekcem_oracle.so``).  This module assembles a box case the
+way the reference does for its ``.box``/``.re2`` cases:
+
+* element numbering ``e = ex + nelx*(ey + nely*ez)`` (get_exyz in
+  src/nek5_genbox.F:83-145);
+* straight-sided elements, so ``xm1`` is the tensor-product linear map of the
+  element's vertices (src/nek5_genxyz.F, xyzlin);
+* face-node gs ids from the "extended lattice" of src/nek5_connect11.F:
+  2179-2347 (setup_dgds2 / iface_vert_int8): every element owns an
+  (n1+2)^3 extended grid whose ghost layers coincide with the neighbour's,
+  and face node (a, b) of face f takes the id of the ghost-layer point just
+  outside the face; periodic directions wrap (gen_gtp_vertex,
+  src/nek5_genbox.F:111-117).  Entries sharing an id are the gs partners.
+* CBC letters per face: 'P' (periodic pair) or 'PEC' on the box walls.
+
+Parity status of each C function is recorded in tests/test_oracle.py, which
+pins it against closed forms, invariants and the reference's own analytic
+test cases (tests/3dboxper, tests/3dboxpec) at the reference's tolerances.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "libnekcem_oracle.so")
+_lib = None
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_bp = C.POINTER(C.c_int8)
+
+
+def build():
+    """Compile the oracle (gcc, OpenMP) into oracle/_build/."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        L.orc_zwgll.argtypes = [C.c_int, _dp, _dp]
+        L.orc_dgll.argtypes = [C.c_int, _dp, _dp]
+        L.orc_pnleg.argtypes = [C.c_double, C.c_int]
+        L.orc_pnleg.restype = C.c_double
+        L.orc_geom.argtypes = [C.c_int, C.c_int64, _dp, _dp, _dp, _dp, _dp] + [_dp] * 16
+        L.orc_set_fc_ptr.argtypes = [C.c_int, C.c_int64, _ip]
+        L.orc_get_dxmin.argtypes = [C.c_int, C.c_int64, _dp, _dp, _dp]
+        L.orc_get_dxmin.restype = C.c_double
+        L.orc_gs_add.argtypes = [_ip, _dp, C.c_int64, C.c_int, C.c_int64]
+        L.orc_materials.argtypes = [C.c_int, C.c_int64, _ip, _ip, _bp, _dp, _dp, _dp,
+                                    _dp, _dp, _dp, _dp, _dp, _dp]
+        L.orc_pec_init.argtypes = [C.c_int, C.c_int64, _bp, _ip]
+        L.orc_pec_init.restype = C.c_int64
+        L.orc_maxwell_op.argtypes = [C.c_void_p, C.c_double, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.orc_rk_run.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                                 C.c_double, C.c_int64, _dp]
+        L.orc_rk_coeffs.argtypes = [_dp, _dp, _dp]
+        L.orc_cem_error.argtypes = [_dp, _dp, _dp, C.c_int64, _dp, _dp]
+        L.orc_num_threads.restype = C.c_int
+        L.orc_set_num_threads.argtypes = [C.c_int]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    if a is None:
+        return None
+    if a.dtype == np.float64:
+        return a.ctypes.data_as(_dp)
+    if a.dtype == np.int32:
+        return a.ctypes.data_as(_ip)
+    if a.dtype == np.int8:
+        return a.ctypes.data_as(_bp)
+    raise TypeError(a.dtype)
+
+
+class _Mesh(C.Structure):
+    _fields_ = [
+        ("n1", C.c_int32), ("nelt", C.c_int32), ("npts", C.c_int64), ("nxzfl", C.c_int64),
+        ("D", _dp), ("w3", _dp),
+        ("rx", _dp), ("ry", _dp), ("rz", _dp), ("sx", _dp), ("sy", _dp), ("sz", _dp),
+        ("tx", _dp), ("ty", _dp), ("tz", _dp),
+        ("bm1", _dp),
+        ("unx", _dp), ("uny", _dp), ("unz", _dp), ("area", _dp),
+        ("cemface", _ip), ("partner", _ip), ("cempec", _ip), ("ncempec", C.c_int64),
+        ("Y0", _dp), ("Y1", _dp), ("Z0", _dp), ("Z1", _dp),
+        ("hbm1", _dp), ("ebm1", _dp), ("C0", C.c_double),
+        ("ndrude", C.c_int64), ("dindex", _ip), ("drude_a", _dp), ("drude_b", _dp),
+        ("src_amp", _dp), ("src_dir", C.c_double * 3), ("src_t0", C.c_double),
+        ("src_tw", C.c_double),
+        ("srflx", _dp),
+    ]
+
+
+def zwgll(n1):
+    z = np.zeros(n1)
+    w = np.zeros(n1)
+    lib().orc_zwgll(n1, _p(z), _p(w))
+    return z, w
+
+
+def dgll(z):
+    n1 = len(z)
+    z = np.ascontiguousarray(z, dtype=np.float64)
+    D = np.zeros(n1 * n1)
+    lib().orc_dgll(n1, _p(z), _p(D))
+    return D.reshape(n1, n1, order="F")
+
+
+def rk_coeffs():
+    a, b, c = np.zeros(5), np.zeros(5), np.zeros(6)
+    lib().orc_rk_coeffs(_p(a), _p(b), _p(c))
+    return a, b, c
+
+
+# Ed's face order -> (axis, side): 1 y-, 2 x+, 3 y+, 4 x-, 5 z-, 6 z+
+_ED_FACES = [(1, 0), (0, 1), (1, 1), (0, 0), (2, 0), (2, 1)]
+
+
+@dataclass
+class BoxSpec:
+    nel: tuple          # (nelx, nely, nelz)
+    N: int              # polynomial order
+    lo: tuple = (0.0, 0.0, 0.0)
+    hi: tuple = (2 * np.pi, 2 * np.pi, 2 * np.pi)
+    bc: tuple = ("P", "P", "P")   # per direction: 'P' periodic or 'PEC'
+    upwind: bool = True
+
+
+class BoxCase:
+    """A reference-layout box case: every array the reference's
+    cem_maxwell_init leaves in COMMON, for a box mesh."""
+
+    def __init__(self, spec: BoxSpec, eps=None, mu=None):
+        L = lib()
+        self.spec = spec
+        nelx, nely, nelz = spec.nel
+        n1 = spec.N + 1
+        self.n1 = n1
+        self.nelt = nelt = nelx * nely * nelz
+        n2, n3 = n1 * n1, n1 ** 3
+        self.npts = npts = n3 * nelt
+        self.nxzfl = nxzfl = 6 * n2 * nelt
+        self.z, self.w = zwgll(n1)
+        self.D = dgll(self.z)
+        self._Dflat = np.ascontiguousarray(self.D.reshape(-1, order="F"))
+
+        # coordinates: element (ex,ey,ez) spans vertices lo + (hi-lo)*v/nel
+        e = np.arange(nelt)
+        ex, ey, ez = e % nelx, (e // nelx) % nely, e // (nelx * nely)
+        self.exyz = (ex, ey, ez)
+        coords = []
+        for ax, (ea, nea) in enumerate(((ex, nelx), (ey, nely), (ez, nelz))):
+            lo, hi = spec.lo[ax], spec.hi[ax]
+            v0 = lo + (hi - lo) * ea / nea
+            v1 = lo + (hi - lo) * (ea + 1) / nea
+            h0 = 0.5 * (1.0 - self.z)
+            h1 = 0.5 * (1.0 + self.z)
+            line = v0[:, None] * h0[None, :] + v1[:, None] * h1[None, :]   # (nelt, n1)
+            shape = [nelt, 1, 1, 1]
+            shape[3 - ax] = n1
+            arr = np.broadcast_to(line.reshape(shape), (nelt, n1, n1, n1))
+            coords.append(np.ascontiguousarray(arr).reshape(-1))
+        self.xm1, self.ym1, self.zm1 = coords
+
+        g = {k: np.zeros(npts) for k in ("rx", "ry", "rz", "sx", "sy", "sz",
+                                          "tx", "ty", "tz", "jac", "bm1")}
+        self.w3 = np.zeros(n3)
+        fa = {k: np.zeros(nxzfl) for k in ("unx", "uny", "unz", "area")}
+        L.orc_geom(n1, nelt, _p(self._Dflat), _p(self.w), _p(self.xm1), _p(self.ym1),
+                   _p(self.zm1), *[_p(g[k]) for k in ("rx", "ry", "rz", "sx", "sy", "sz",
+                                                       "tx", "ty", "tz", "jac", "bm1")],
+                   _p(self.w3), _p(fa["unx"]), _p(fa["uny"]), _p(fa["unz"]), _p(fa["area"]))
+        self.geom = g
+        self.face = fa
+        self.bm1 = g["bm1"]
+
+        self.cemface = np.zeros(nxzfl, dtype=np.int32)
+        L.orc_set_fc_ptr(n1, nelt, _p(self.cemface))
+
+        self.gsid = self._face_ids()
+        self.partner = self._partner_from_ids(self.gsid)
+
+        self.pec_face = np.zeros(6 * nelt, dtype=np.int8)
+        for f, (ax, side) in enumerate(_ED_FACES):
+            if spec.bc[ax] == "PEC":
+                ea, nea = self.exyz[ax], spec.nel[ax]
+                on = (ea == 0) if side == 0 else (ea == nea - 1)
+                self.pec_face[6 * e[on] + f] = 1
+        self.cempec = np.zeros(nxzfl, dtype=np.int32)
+        self.ncempec = int(L.orc_pec_init(n1, nelt, _p(self.pec_face), _p(self.cempec)))
+
+        self.set_materials(eps, mu)
+        self.dxmin = float(L.orc_get_dxmin(n1, nelt, _p(self.xm1), _p(self.ym1), _p(self.zm1)))
+        self.srflx = np.zeros(6 * nxzfl)
+        self.src_amp = None
+        self.src_dir = (0.0, 0.0, 1.0)
+        self.src_t0 = 0.0
+        self.src_tw = 1.0
+        self.dindex = None
+        self.drude_a = None
+        self.drude_b = None
+
+    # -- setup pieces ---------------------------------------------------
+    def _face_ids(self):
+        """gs ids of every face node (Ed order, in-face a fast, b slow)."""
+        spec, n1 = self.spec, self.n1
+        nelx, nely, nelz = spec.nel
+        nelt = self.nelt
+        ex, ey, ez = self.exyz
+        Lx, Ly, Lz = nelx * (n1 + 1), nely * (n1 + 1), nelz * (n1 + 1)
+        a = np.arange(n1)
+        A, B = np.meshgrid(a, a, indexing="xy")        # A[b, a] = a, B[b, a] = b
+        A = A.reshape(-1)
+        B = B.reshape(-1)
+        ids = np.zeros((nelt, 6, n1 * n1), dtype=np.int64)
+        for f in range(6):
+            if f == 0:   ix, iy, iz = A + 1, np.zeros_like(A), B + 1
+            elif f == 1: ix, iy, iz = np.full_like(A, n1 + 1), A + 1, B + 1
+            elif f == 2: ix, iy, iz = A + 1, np.full_like(A, n1 + 1), B + 1
+            elif f == 3: ix, iy, iz = np.zeros_like(A), A + 1, B + 1
+            elif f == 4: ix, iy, iz = A + 1, B + 1, np.zeros_like(A)
+            else:        ix, iy, iz = A + 1, B + 1, np.full_like(A, n1 + 1)
+            GX = ex[:, None] * (n1 + 1) + ix[None, :]
+            GY = ey[:, None] * (n1 + 1) + iy[None, :]
+            GZ = ez[:, None] * (n1 + 1) + iz[None, :]
+            if spec.bc[0] == "P": GX = GX % Lx
+            if spec.bc[1] == "P": GY = GY % Ly
+            if spec.bc[2] == "P": GZ = GZ % Lz
+            ids[:, f, :] = GX + (Lx + 1) * (GY + (Ly + 1) * GZ)
+        return ids.reshape(-1)
+
+    @staticmethod
+    def _partner_from_ids(ids):
+        order = np.argsort(ids, kind="stable")
+        s = ids[order]
+        partner = np.full(len(ids), -1, dtype=np.int32)
+        same = np.nonzero(s[1:] == s[:-1])[0]
+        if len(same):
+            # at most two entries per id on a conforming hex mesh
+            assert not np.any(np.diff(same) == 1), "gs id shared by more than two face nodes"
+            i, j = order[same], order[same + 1]
+            partner[i] = j
+            partner[j] = i
+        return partner
+
+    def set_materials(self, eps=None, mu=None):
+        npts, nxzfl = self.npts, self.nxzfl
+        self.eps = np.ones(npts) if eps is None else np.ascontiguousarray(eps, dtype=np.float64)
+        self.mu = np.ones(npts) if mu is None else np.ascontiguousarray(mu, dtype=np.float64)
+        assert self.eps.shape == (npts,) and self.mu.shape == (npts,)
+        self.Z0, self.Z1, self.Y0, self.Y1 = (np.zeros(nxzfl) for _ in range(4))
+        self.ebm1, self.hbm1 = np.zeros(npts), np.zeros(npts)
+        lib().orc_materials(self.n1, self.nelt, _p(self.cemface), _p(self.partner),
+                            _p(self.pec_face), _p(self.eps), _p(self.mu), _p(self.bm1),
+                            _p(self.Z0), _p(self.Z1), _p(self.Y0), _p(self.Y1),
+                            _p(self.ebm1), _p(self.hbm1))
+
+    def set_source(self, amp, direction=(0.0, 0.0, 1.0), t0=0.0, tw=1.0):
+        """usersrc-style source: resE += amp(x) * dir * exp(-((t-t0)/tw)^2),
+        amp already bm1-weighted (as in src/cem_maxwell.F:3130-3132)."""
+        self.src_amp = None if amp is None else np.ascontiguousarray(amp, dtype=np.float64)
+        self.src_dir = tuple(float(x) for x in direction)
+        self.src_t0, self.src_tw = float(t0), float(tw)
+
+    def set_drude(self, dindex, a, b):
+        """Drude ADE over volume nodes dindex with per-node a (gamma) and
+        b (omega_p^2)."""
+        self.dindex = np.ascontiguousarray(dindex, dtype=np.int32)
+        self.drude_a = np.ascontiguousarray(a, dtype=np.float64)
+        self.drude_b = np.ascontiguousarray(b, dtype=np.float64)
+
+    def dt_from_cfl(self, cfl):
+        """set_dt (src/cem_drive.F:351-395): dt = |p12| if p12 < 0 else p12*dxmin."""
+        return abs(cfl) if cfl < 0 else cfl * self.dxmin
+
+    # -- operator ---------------------------------------------------------
+    def _mesh(self):
+        m = _Mesh()
+        m.n1, m.nelt, m.npts, m.nxzfl = self.n1, self.nelt, self.npts, self.nxzfl
+        m.D, m.w3 = _p(self._Dflat), _p(self.w3)
+        for k in ("rx", "ry", "rz", "sx", "sy", "sz", "tx", "ty", "tz", "bm1"):
+            setattr(m, k, _p(self.geom[k]))
+        for k in ("unx", "uny", "unz", "area"):
+            setattr(m, k, _p(self.face[k]))
+        m.cemface, m.partner, m.cempec, m.ncempec = (_p(self.cemface), _p(self.partner),
+                                                      _p(self.cempec), self.ncempec)
+        m.Y0, m.Y1, m.Z0, m.Z1 = _p(self.Y0), _p(self.Y1), _p(self.Z0), _p(self.Z1)
+        m.hbm1, m.ebm1 = _p(self.hbm1), _p(self.ebm1)
+        m.C0 = 1.0 if self.spec.upwind else 0.0
+        if self.dindex is not None and len(self.dindex):
+            m.ndrude = len(self.dindex)
+            m.dindex, m.drude_a, m.drude_b = _p(self.dindex), _p(self.drude_a), _p(self.drude_b)
+        else:
+            m.ndrude = 0
+        if self.src_amp is not None:
+            m.src_amp = _p(self.src_amp)
+        m.src_dir = (C.c_double * 3)(*self.src_dir)
+        m.src_t0, m.src_tw = self.src_t0, self.src_tw
+        m.srflx = _p(self.srflx)
+        self._keep = m
+        return m
+
+    def residual(self, H, E, t=0.0, J=None):
+        """(resH, resE[, resJ]) = cem_maxwell_op(H, E) incl. inverse mass."""
+        H = np.ascontiguousarray(H, dtype=np.float64).reshape(3, self.npts)
+        E = np.ascontiguousarray(E, dtype=np.float64).reshape(3, self.npts)
+        rH, rE = np.zeros_like(H), np.zeros_like(E)
+        m = self._mesh()
+        rJ = None
+        if J is not None:
+            J = np.ascontiguousarray(J, dtype=np.float64)
+            rJ = np.zeros_like(J)
+        lib().orc_maxwell_op(C.byref(m), t, _p(H), _p(E), _p(rH), _p(rE), _p(J), _p(rJ))
+        return (rH, rE) if J is None else (rH, rE, rJ)
+
+    def run(self, H, E, dt, nsteps, t=0.0, kH=None, kE=None, J=None, kJ=None):
+        """nsteps of cem_maxwell_op_rk from (H, E) at time t.  Returns a dict
+        with H, E, kH, kE, time (and J, kJ with Drude)."""
+        H = np.array(H, dtype=np.float64).reshape(3, self.npts)
+        E = np.array(E, dtype=np.float64).reshape(3, self.npts)
+        kH = np.zeros_like(H) if kH is None else np.array(kH, dtype=np.float64)
+        kE = np.zeros_like(E) if kE is None else np.array(kE, dtype=np.float64)
+        rH, rE = np.zeros_like(H), np.zeros_like(E)
+        rJ = None
+        if self.dindex is not None and len(self.dindex):
+            nd = len(self.dindex)
+            J = np.zeros(3 * nd) if J is None else np.array(J, dtype=np.float64)
+            kJ = np.zeros(3 * nd) if kJ is None else np.array(kJ, dtype=np.float64)
+            rJ = np.zeros(3 * nd)
+        tt = np.array([t], dtype=np.float64)
+        m = self._mesh()
+        lib().orc_rk_run(C.byref(m), _p(H), _p(E), _p(kH), _p(kE), _p(rH), _p(rE),
+                         _p(J), _p(kJ), _p(rJ), dt, nsteps, _p(tt))
+        out = dict(H=H, E=E, kH=kH, kE=kE, time=float(tt[0]))
+        if J is not None:
+            out.update(J=J, kJ=kJ)
+        return out
+
+    def cem_error(self, u, exact):
+        l2 = np.zeros(1)
+        li = np.zeros(1)
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        exact = np.ascontiguousarray(exact, dtype=np.float64)
+        lib().orc_cem_error(_p(u), _p(exact), _p(self.bm1), self.npts, _p(l2), _p(li))
+        return float(l2[0]), float(li[0])
+
+
+def set_num_threads(n):
+    lib().orc_set_num_threads(int(n))
+
+
+def num_threads():
+    return int(lib().orc_num_threads())

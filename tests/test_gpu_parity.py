@@ -1,0 +1,212 @@
+"""Parity of the HIP path (libnekcem_hip.so via its C-ABI) with the CPU
+oracle on the same mesh, materials and initial state.
+
+Tolerances: the contract is 1e-11 relative L2 per field (bm1-weighted, the
+cem_error norm) after the stated step count; single residual evaluations
+are held to 1e-11 of the field's max (random input; the two sides build
+their D matrices independently, which differ by ~1e-15 relative).  The HIP kernel uses per-element
+constant metrics and the strong-form arrangement of the same arithmetic, so
+agreement is to round-off, not bitwise.
+"""
+import numpy as np
+import pytest
+
+from nekcem_amd.maxwell import MaxwellSolver, cem_error
+from nekcem_amd.mesh import BoxMesh
+from nekcem_amd import usr as cases
+
+pytestmark = pytest.mark.gpu
+
+
+def _norm(u, bm1):
+    return float(np.sqrt(np.sum(bm1 * u * u)))
+
+
+def parity(gH, gE, oH, oE, bm1):
+    """Largest per-field relative L2 (bm1-weighted) difference.  A field whose
+    own norm is below 1e-6 of the largest field's norm (e.g. a component that
+    is zero in the exact solution and holds only truncation error) is
+    measured against the largest field's norm instead."""
+    g = np.concatenate([gH, gE])
+    o = np.concatenate([oH, oE])
+    norms = [_norm(o[c], bm1) for c in range(6)]
+    big = max(norms)
+    return max(_norm(g[c] - o[c], bm1) / (norms[c] if norms[c] >= 1e-6 * big else big)
+               for c in range(6))
+
+
+def _pair(oracle, nel, N, bc=("P", "P", "P"), lo=(0, 0, 0), hi=(2 * np.pi,) * 3,
+          eps=None, mu=None, upwind=True, graph=True):
+    print(f"[case] nel={nel} N={N} bc={bc} upwind={upwind} graph={graph}", flush=True)
+    mesh = BoxMesh(nel, N, lo, hi, bc)
+    spec = oracle.BoxSpec(nel=nel, N=N, lo=lo, hi=hi, bc=bc, upwind=upwind)
+    ref = oracle.BoxCase(spec, eps=eps, mu=mu)
+    gpu = MaxwellSolver(mesh, eps=eps, mu=mu, ifupwind=upwind, graph=graph)
+    return mesh, ref, gpu
+
+
+RESIDUAL_CASES = [
+    dict(nel=(3, 2, 4), N=5, bc=("P", "P", "P")),
+    dict(nel=(2, 2, 2), N=7, bc=("PEC", "PEC", "PEC"), lo=(-1, -1, -1), hi=(1, 1, 1)),
+    dict(nel=(2, 3, 2), N=3, bc=("P", "PEC", "P")),
+    dict(nel=(2, 2, 3), N=6, bc=("PEC", "P", "P")),
+    dict(nel=(2, 2, 2), N=8, bc=("P", "P", "PEC")),
+    dict(nel=(1, 2, 2), N=9, bc=("P", "P", "P")),
+    dict(nel=(2, 1, 2), N=4, bc=("P", "P", "P"), upwind=False),
+]
+
+
+@pytest.mark.parametrize("case", RESIDUAL_CASES, ids=lambda c: f"N{c['N']}-{'-'.join(c['bc'])}")
+@pytest.mark.parametrize("material", ["vacuum", "element", "node"])
+def test_residual_parity(oracle, case, material):
+    rng = np.random.default_rng(7)
+    n1 = case["N"] + 1
+    nelt = int(np.prod(case["nel"]))
+    npts = nelt * n1 ** 3
+    eps = mu = None
+    if material == "element":
+        eps = np.repeat(rng.uniform(1, 4, nelt), n1 ** 3)
+        mu = np.repeat(rng.uniform(1, 2, nelt), n1 ** 3)
+    elif material == "node":
+        eps = rng.uniform(1, 4, npts)
+        mu = rng.uniform(1, 2, npts)
+    mesh, ref, gpu = _pair(oracle, eps=eps, mu=mu, **case)
+    assert gpu.info()["mat_mode"] == (1 if material == "node" else 0)
+    H = rng.standard_normal((3, npts))
+    E = rng.standard_normal((3, npts))
+    gpu.set_state(H, E)
+    rH, rE = gpu.residual()
+    oH, oE = ref.residual(H, E)
+    for a, b in ((rH, oH), (rE, oE)):
+        for c in range(3):
+            scale = np.max(np.abs(b[c]))
+            assert np.max(np.abs(a[c] - b[c])) <= 1e-11 * scale, (c, np.max(np.abs(a[c] - b[c])), scale)
+    gpu.close()
+
+
+def test_config1_100_steps_parity(oracle):
+    """4^3 periodic box on [0,2pi]^3, N=7, vacuum, upwind, (1,1,1) mode,
+    CFL 0.25, 100 steps: GPU vs oracle within 1e-11 relative L2 per field,
+    and both at the analytic solution to the same error."""
+    mesh, ref, gpu = _pair(oracle, (4, 4, 4), 7)
+    x, y, z = mesh.coords()
+    H, E = cases.boxper_solution(0.0, x, y, z)
+    dt = mesh.dt(0.25)
+    assert abs(dt - ref.dt_from_cfl(0.25)) < 1e-15
+    gpu.set_state(H, E)
+    gpu.step(dt, 100)
+    gH, gE = gpu.get_state()
+    out = ref.run(H, E, dt, 100)
+    Hs, Es = cases.boxper_solution(out["time"], x, y, z)
+    assert parity(gH, gE, out["H"], out["E"], mesh.bmn) <= 1e-11
+    for g, o, s in ((gH, out["H"], Hs), (gE, out["E"], Es)):
+        for c in range(3):
+            eg, eo = cem_error(g[c], s[c], mesh.bmn)[0], ref.cem_error(o[c], s[c])[0]
+            assert abs(eg - eo) <= 1e-3 * eo + 1e-15
+    assert abs(gpu.time - out["time"]) < 1e-13
+    gpu.close()
+
+
+def test_3dboxper_analytic_on_gpu(oracle):
+    """tests/3dboxper on the HIP path: 4x4x8, N=8, dt=2e-4, 50 steps, checked
+    like its userchk (istep <= 10 and every iocomm=50) at L2 <= 5e-10,
+    Linf <= 5e-9."""
+    mesh = BoxMesh((4, 4, 8), 8)
+    gpu = MaxwellSolver(mesh)
+    gpu.cem_maxwell_init_fields(cases.boxper_solution)
+    gpu.set_dt(-0.0002)
+    x, y, z = mesh.coords()
+    seen = []
+
+    def userchk(s):
+        H, E = s.get_state()
+        Hs, Es = cases.boxper_solution(s.time, x, y, z)
+        for u, ex in ((H, Hs), (E, Es)):
+            for c in range(3):
+                l2, li = cem_error(u[c], ex[c], mesh.bmn)
+                if l2 > 5e-10 or li > 5e-9:
+                    return False
+        seen.append(s.istep)
+        return True
+
+    gpu.time_advancing_pde(50, userchk, chk_every=50)
+    assert seen == list(range(1, 11)) + [50]
+    gpu.close()
+
+
+def test_3dboxpec_analytic_on_gpu(oracle):
+    """tests/3dboxpec on the HIP path: 3x3x3 PEC box, N=8, dt=0.005, 1000
+    steps, L2 <= 5e-8, Linf <= 5e-7 (checked every iocomm=10 steps)."""
+    mesh = BoxMesh((3, 3, 3), 8, (-1, -1, -1), (1, 1, 1), ("PEC",) * 3)
+    gpu = MaxwellSolver(mesh)
+    gpu.cem_maxwell_init_fields(cases.boxpec_solution)
+    gpu.set_dt(-0.005)
+    x, y, z = mesh.coords()
+    worst = [0.0]
+
+    def userchk(s):
+        H, E = s.get_state()
+        Hs, Es = cases.boxpec_solution(s.time, x, y, z)
+        for u, ex in ((H, Hs), (E, Es)):
+            for c in range(3):
+                l2, li = cem_error(u[c], ex[c], mesh.bmn)
+                worst[0] = max(worst[0], l2)
+                if l2 > 5e-8 or li > 5e-7:
+                    return False
+        return True
+
+    gpu.time_advancing_pde(1000, userchk, chk_every=100)
+    assert 0 < worst[0] <= 5e-8
+    gpu.close()
+
+
+def test_graph_replay_matches_direct_launches(oracle):
+    mesh = BoxMesh((3, 3, 3), 7)
+    rng = np.random.default_rng(11)
+    eps = np.repeat(rng.uniform(1, 4, mesh.nelt), mesh.n1 ** 3)
+    H = rng.standard_normal((3, mesh.npts))
+    E = rng.standard_normal((3, mesh.npts))
+    outs = []
+    for graph in (True, False):
+        g = MaxwellSolver(mesh, eps=eps, graph=graph)
+        g.set_state(H, E)
+        for _ in range(3):          # replays with host work in between
+            g.step(1e-3, 2)
+            g.get_state()
+        outs.append(g.get_state(with_k=True))
+        g.close()
+    for a, b in zip(outs[0], outs[1]):
+        assert np.array_equal(a, b)
+
+
+def test_blowup_detected():
+    mesh = BoxMesh((2, 2, 2), 5)
+    g = MaxwellSolver(mesh)
+    H = np.zeros((3, mesh.npts))
+    E = np.ones((3, mesh.npts))
+    g.set_state(H, E)
+    assert abs(g.check_finite() - 1.0) < 1e-15
+    E[1, 17] = np.nan
+    g.set_state(H, E)
+    from nekcem_amd._lib import NkcError, NKC_ERR_BLOWUP
+    with pytest.raises(NkcError) as ei:
+        g.check_finite()
+    assert ei.value.code == NKC_ERR_BLOWUP
+    g.close()
+
+
+def test_config2_shape_reduced_parity(oracle):
+    """The 32^3 workload's physics on an 8^3 box: per-element eps ~ U[1,4]
+    (seed 1), 8 seeded plane waves, upwind, CFL 0.25, 20 steps."""
+    nel = (8, 8, 8)
+    eps = np.repeat(cases.element_eps(512, seed=1), 512)
+    mesh, ref, gpu = _pair(oracle, nel, 7, eps=eps)
+    x, y, z = mesh.coords()
+    H, E = cases.plane_waves(x, y, z, seed=2)
+    dt = mesh.dt(0.25)
+    gpu.set_state(H, E)
+    gpu.step(dt, 20)
+    gH, gE = gpu.get_state()
+    out = ref.run(H, E, dt, 20)
+    assert parity(gH, gE, out["H"], out["E"], mesh.bmn) <= 1e-11
+    gpu.close()

@@ -1,0 +1,299 @@
+"""Pins of the CPU oracle (oracle/nekcem_oracle.c) -- no GPU needed.
+
+The reference ships no golden vectors (its tests are analytic checks inside
+each case's userchk), and its Fortran build needs MPI and LAPACK, so every
+oracle function is pinned here by closed forms, invariants, independent
+brute force, or the reference's own analytic test cases at the reference's
+own tolerances:
+
+  orc_zwgll       roots of (1-x^2) P_N', quadrature exact to degree 2N-1
+  orc_dgll        exact on polynomials of degree <= N; matches numpy
+                  Legendre differentiation of the Lagrange basis
+  orc_geom        affine box: constant metrics = scaled identity, bm1 sums to
+                  the box volume, normals/areas of a box; plus the 3dboxper /
+                  3dboxpec runs below (which consume every array)
+  orc_set_fc_ptr  every face node lies on its face plane, in skpdat order
+  gs partner ids  partners sit at the same physical point (mod period)
+  orc_materials   closed form for two-material interfaces and PEC walls
+  orc_maxwell_op  curl(grad phi) = 0 in the interior; central flux is
+                  skew-adjoint and upwind dissipative in the bm1-eps/mu inner
+                  product; spectral convergence to the exact curl
+  orc_rk_run      RK45 stability polynomial = exp to 4th order (constants of
+                  Carpenter & Kennedy); 3dboxper and 3dboxpec analytic
+                  errors within the reference's userchk tolerances
+  Drude / source  uniform-field ODEs with closed-form solutions
+"""
+import math
+
+import numpy as np
+import pytest
+from numpy.polynomial import legendre as npleg
+
+from nekcem_amd import usr as cases
+
+
+# ------------------------------------------------------------------ basis --
+@pytest.mark.parametrize("n1", [2, 3, 5, 6, 8, 9, 10, 12])
+def test_zwgll_closed_form(oracle, n1):
+    z, w = oracle.zwgll(n1)
+    N = n1 - 1
+    assert z[0] == -1.0 and z[-1] == 1.0
+    assert np.all(np.diff(z) > 0)
+    c = np.zeros(N + 1)
+    c[N] = 1
+    if N > 1:
+        dP = npleg.legval(z[1:-1], npleg.legder(c))
+        assert np.max(np.abs(dP)) < 1e-12 * N * N
+    PN = npleg.legval(z, c)
+    np.testing.assert_allclose(w, 2.0 / (N * (N + 1) * PN ** 2), rtol=1e-14)
+    # exact for degree <= 2N-1
+    for deg in range(2 * N):
+        exact = 0.0 if deg % 2 else 2.0 / (deg + 1)
+        assert abs(np.dot(w, z ** deg) - exact) < 1e-13
+
+
+@pytest.mark.parametrize("n1", [3, 5, 8, 10])
+def test_dgll_exact_on_polynomials(oracle, n1):
+    z, _ = oracle.zwgll(n1)
+    D = oracle.dgll(z)
+    N = n1 - 1
+    for deg in range(N + 1):
+        f = z ** deg
+        df = deg * z ** (deg - 1) if deg > 0 else 0 * z
+        assert np.max(np.abs(D @ f - df)) < 1e-12 * max(1, deg) * N * N
+    # degree N+1 is NOT differentiated exactly (guards a wrong-size D)
+    assert np.max(np.abs(D @ z ** (N + 1) - (N + 1) * z ** N)) > 1e-6
+    # corner entries of DGLL
+    assert D[0, 0] == -N * (N + 1) / 4 and D[N, N] == N * (N + 1) / 4
+
+
+def test_rk45_stability_polynomial(oracle):
+    a, b, c = oracle.rk_coeffs()
+    # scalar 2N-storage recursion on y' = lam y, one step, as polynomial in z
+    P = np.polynomial.Polynomial
+    u, k = P([1.0]), P([0.0])
+    zpoly = P([0.0, 1.0])
+    for i in range(5):
+        k = a[i] * k + zpoly * u
+        u = u + b[i] * k
+    coef = u.coef
+    for n in range(5):
+        assert abs(coef[n] - 1.0 / math.factorial(n)) < 1e-14
+    assert len(coef) == 6 and 0 < coef[5] < 1.0 / 120   # 5-stage, 4th order
+    # stage times c_i = sum of the increments reaching stage i
+    assert c[0] == 0.0 and c[5] == 1.0 and np.all(np.diff(c) > 0)
+
+
+# --------------------------------------------------------------- geometry --
+def test_box_geometry(oracle):
+    spec = oracle.BoxSpec(nel=(2, 3, 4), N=5, lo=(0, 0, 0), hi=(1.0, 3.0, 2.0))
+    c = oracle.BoxCase(spec)
+    hx, hy, hz = 0.5, 1.0, 0.5
+    J = hx * hy * hz / 8
+    g = c.geom
+    np.testing.assert_allclose(g["jac"], J, rtol=1e-12)
+    np.testing.assert_allclose(g["rx"], hy * hz / 4, rtol=1e-12)
+    np.testing.assert_allclose(g["sy"], hx * hz / 4, rtol=1e-12)
+    np.testing.assert_allclose(g["tz"], hx * hy / 4, rtol=1e-12)
+    for k in ("ry", "rz", "sx", "sz", "tx", "ty"):
+        assert np.max(np.abs(g[k])) < 1e-12
+    assert abs(c.bm1.sum() - 6.0) < 1e-12          # box volume
+    # areas of each face sum to the face area
+    area = c.face["area"].reshape(c.nelt, 6, -1).sum(axis=2)
+    fa = {0: hx * hz, 1: hy * hz, 2: hx * hz, 3: hy * hz, 4: hx * hy, 5: hx * hy}
+    for f in range(6):
+        np.testing.assert_allclose(area[:, f], fa[f], rtol=1e-12)
+    n = np.stack([c.face[k].reshape(c.nelt, 6, -1) for k in ("unx", "uny", "unz")])
+    expect = {0: (0, -1, 0), 1: (1, 0, 0), 2: (0, 1, 0), 3: (-1, 0, 0), 4: (0, 0, -1), 5: (0, 0, 1)}
+    for f, v in expect.items():
+        for ax in range(3):
+            np.testing.assert_allclose(n[ax, :, f], v[ax], atol=1e-12)
+
+
+def test_cemface_and_partners_physical(oracle):
+    spec = oracle.BoxSpec(nel=(3, 2, 2), N=4, bc=("P", "PEC", "P"))
+    c = oracle.BoxCase(spec)
+    n1, n2 = c.n1, c.n1 ** 2
+    v = c.cemface.reshape(c.nelt, 6, n2) % (n1 ** 3)
+    i, j, k = v % n1, (v // n1) % n1, v // n2
+    # Ed order: 1 y-, 2 x+, 3 y+, 4 x-, 5 z-, 6 z+
+    assert np.all(j[:, 0] == 0) and np.all(i[:, 1] == n1 - 1) and np.all(j[:, 2] == n1 - 1)
+    assert np.all(i[:, 3] == 0) and np.all(k[:, 4] == 0) and np.all(k[:, 5] == n1 - 1)
+    # in-face order a fast, b slow (skpdat)
+    assert np.all(j[:, 1, :n1] == np.arange(n1)) and np.all(i[:, 0, :n1] == np.arange(n1))
+    # partners are at the same physical point modulo the period, on the
+    # facing side; PEC walls have none
+    X = np.stack([c.xm1, c.ym1, c.zm1])[:, c.cemface]
+    p = c.partner
+    has = p >= 0
+    d = X[:, has] - X[:, p[has]]
+    L = 2 * np.pi
+    d = d - L * np.round(d / L)
+    assert np.max(np.abs(d)) < 1e-12
+    assert np.all(p[p >= 0][p[p[p >= 0]] >= 0] >= 0)
+    assert np.array_equal(p[p[has]], np.nonzero(has)[0])     # symmetric
+    assert (~has).sum() == c.ncempec == 2 * 3 * 2 * n2        # the two y walls
+
+
+def test_materials_closed_form(oracle):
+    spec = oracle.BoxSpec(nel=(2, 1, 1), N=3, bc=("P", "PEC", "PEC"))
+    c = oracle.BoxCase(spec)
+    n3 = c.n1 ** 3
+    eps = np.concatenate([np.full(n3, 1.0), np.full(n3, 4.0)])
+    mu = np.concatenate([np.full(n3, 1.0), np.full(n3, 2.0)])
+    c.set_materials(eps, mu)
+    n2 = c.n1 ** 2
+    Y0 = c.Y0.reshape(2, 6, n2)
+    Y1 = c.Y1.reshape(2, 6, n2)
+    Z0 = c.Z0.reshape(2, 6, n2)
+    Z1 = c.Z1.reshape(2, 6, n2)
+    ya, yb = 1.0, math.sqrt(2.0)      # sqrt(eps/mu)
+    za, zb = 1.0, math.sqrt(0.5)
+    # element 0 x+ face (Ed 2) faces element 1: Y0 = (Y- + Y+)/2, Y1 = Y+
+    np.testing.assert_allclose(Y0[0, 1], 0.5 * (ya + yb), rtol=1e-15)
+    np.testing.assert_allclose(Y1[0, 1], yb, rtol=1e-15)
+    np.testing.assert_allclose(Z0[1, 3], 0.5 * (za + zb), rtol=1e-15)
+    np.testing.assert_allclose(Z1[1, 3], za, rtol=1e-15)
+    # PEC walls: Y0 = Y1 = Y-
+    np.testing.assert_allclose(Y0[1, 0], yb, rtol=1e-15)
+    np.testing.assert_allclose(Y1[1, 0], yb, rtol=1e-15)
+    np.testing.assert_allclose(Z0[0, 4], za, rtol=1e-15)
+    np.testing.assert_allclose(Z1[0, 4], za, rtol=1e-15)
+    np.testing.assert_allclose(c.ebm1, 1.0 / (eps * c.bm1), rtol=1e-15)
+
+
+def test_dxmin_box(oracle):
+    spec = oracle.BoxSpec(nel=(4, 4, 4), N=7)
+    c = oracle.BoxCase(spec)
+    z, _ = oracle.zwgll(8)
+    h = 2 * np.pi / 4
+    expect = 0.5 * (h / 2) * np.min(z[2:] - z[:-2])
+    assert abs(c.dxmin - expect) < 1e-14
+    assert c.dt_from_cfl(-0.01) == 0.01 and abs(c.dt_from_cfl(0.25) - 0.25 * expect) < 1e-15
+
+
+# --------------------------------------------------------------- operator --
+def _rand_state(c, seed):
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal((3, c.npts)), rng.standard_normal((3, c.npts))
+
+
+def test_curl_grad_zero_interior(oracle):
+    """E = grad(phi), phi a degree-<=N polynomial: exact discrete gradient,
+    continuous across elements, so resH = -curl E / mu vanishes away from
+    the PEC walls (where the mirror flux acts)."""
+    spec = oracle.BoxSpec(nel=(2, 2, 2), N=5, lo=(-1, -1, -1), hi=(1, 1, 1), bc=("PEC",) * 3)
+    c = oracle.BoxCase(spec)
+    x, y, z = c.xm1, c.ym1, c.zm1
+    E = np.stack([2 * x * y + z ** 2, x ** 2 + 3 * y ** 2 * z, 2 * x * z + y ** 3])  # grad(x^2 y + x z^2 + y^3 z)
+    H = np.zeros_like(E)
+    rH, rE = c.residual(H, E)
+    inner = (np.abs(x) < 1 - 1e-9) & (np.abs(y) < 1 - 1e-9) & (np.abs(z) < 1 - 1e-9)
+    assert np.max(np.abs(rH[:, inner])) < 1e-11
+    assert np.max(np.abs(rE[:, inner])) < 1e-11
+    # and the wall flux is really there (guards a dropped PEC branch)
+    assert np.max(np.abs(rH[:, ~inner])) > 1e-3
+
+
+@pytest.mark.parametrize("bc", [("P", "P", "P"), ("PEC", "P", "PEC")])
+def test_skew_adjoint_central_dissipative_upwind(oracle, bc):
+    rng = np.random.default_rng(3)
+    for upwind in (False, True):
+        spec = oracle.BoxSpec(nel=(2, 3, 2), N=4, bc=bc, upwind=upwind)
+        c = oracle.BoxCase(spec)
+        eps = np.repeat(rng.uniform(1, 4, c.nelt), c.n1 ** 3)
+        mu = np.repeat(rng.uniform(1, 2, c.nelt), c.n1 ** 3)
+        c.set_materials(eps, mu)
+        Mh, Me = mu * c.bm1, eps * c.bm1
+        u = _rand_state(c, 1)
+        v = _rand_state(c, 2)
+        Ru = c.residual(*u)
+        Rv = c.residual(*v)
+        ip = lambda a, Rb: float(np.sum(a[0] * Mh * Rb[0]) + np.sum(a[1] * Me * Rb[1]))
+        uKv, vKu, uKu = ip(u, Rv), ip(v, Ru), ip(u, Ru)
+        scale = abs(uKv) + abs(vKu) + 1.0
+        if not upwind:
+            assert abs(uKv + vKu) < 1e-11 * scale
+            assert abs(uKu) < 1e-11 * scale
+        else:
+            assert uKu < -1e-3 * scale
+
+
+def test_spectral_convergence_to_exact_curl(oracle):
+    errs = []
+    for N in (4, 6, 8, 10):
+        spec = oracle.BoxSpec(nel=(2, 2, 2), N=N)
+        c = oracle.BoxCase(spec)
+        H, E = cases.boxper_solution(0.3, c.xm1, c.ym1, c.zm1)
+        rH, rE = c.residual(H, E)
+        # exact time derivative of the mode
+        dt = 1e-6
+        Hp, Ep = cases.boxper_solution(0.3 + dt, c.xm1, c.ym1, c.zm1)
+        Hm, Em = cases.boxper_solution(0.3 - dt, c.xm1, c.ym1, c.zm1)
+        err = max(np.max(np.abs(rH - (Hp - Hm) / (2 * dt))), np.max(np.abs(rE - (Ep - Em) / (2 * dt))))
+        errs.append(err)
+    # each +2 in N gains well over an order of magnitude (spectral)
+    for a, b in zip(errs, errs[1:]):
+        assert b < a / 20
+    assert errs[-1] < 1e-6
+
+
+# ------------------------------------------- the reference's own test cases --
+def test_3dboxper_reference_tolerance(oracle):
+    """tests/3dboxper: 4x4x8 periodic box, N=8 (SIZE lxi=8), dt=2e-4, 50
+    steps, upwind; userchk tolerances L2 <= 5e-10, Linf <= 5e-9 per field."""
+    spec = oracle.BoxSpec(nel=(4, 4, 8), N=8)
+    c = oracle.BoxCase(spec)
+    H, E = cases.boxper_solution(0.0, c.xm1, c.ym1, c.zm1)
+    out = c.run(H, E, c.dt_from_cfl(-0.0002), 50)
+    Hs, Es = cases.boxper_solution(out["time"], c.xm1, c.ym1, c.zm1)
+    for u, s in ((out["H"], Hs), (out["E"], Es)):
+        for comp in range(3):
+            l2, li = c.cem_error(u[comp], s[comp])
+            assert l2 <= 5e-10 and li <= 5e-9
+
+
+def test_3dboxpec_reference_tolerance(oracle):
+    """tests/3dboxpec: 3x3x3 PEC box on [-1,1]^3, N=8, dt=0.005, 1000 steps;
+    userchk tolerances L2 <= 5e-8, Linf <= 5e-7 per field."""
+    spec = oracle.BoxSpec(nel=(3, 3, 3), N=8, lo=(-1, -1, -1), hi=(1, 1, 1), bc=("PEC",) * 3)
+    c = oracle.BoxCase(spec)
+    H, E = cases.boxpec_solution(0.0, c.xm1, c.ym1, c.zm1)
+    out = c.run(H, E, 0.005, 1000)
+    Hs, Es = cases.boxpec_solution(out["time"], c.xm1, c.ym1, c.zm1)
+    for u, s in ((out["H"], Hs), (out["E"], Es)):
+        for comp in range(3):
+            l2, li = c.cem_error(u[comp], s[comp])
+            assert l2 <= 5e-8 and li <= 5e-7
+
+
+# ------------------------------------------------------ Drude and sources --
+def test_drude_uniform_field_ode(oracle):
+    """Uniform E in a periodic Drude box: curl and jumps vanish, so
+    eps dE/dt = -J, dJ/dt = -gamma J + wp^2 E (cem_maxwell_drude)."""
+    spec = oracle.BoxSpec(nel=(2, 2, 2), N=3)
+    c = oracle.BoxCase(spec)
+    eps = 2.0
+    c.set_materials(np.full(c.npts, eps), None)
+    gam, wp2 = 0.1, 1.5
+    c.set_drude(np.arange(c.npts), np.full(c.npts, gam), np.full(c.npts, wp2))
+    E = np.zeros((3, c.npts))
+    E[1] = 1.0
+    out = c.run(np.zeros_like(E), E, 0.01, 300)
+    # exact: y'' + gam y' + (wp2/eps) y = 0, y(0)=1, y'(0)=0
+    w0 = math.sqrt(wp2 / eps - gam ** 2 / 4)
+    t = out["time"]
+    y = math.exp(-gam * t / 2) * (math.cos(w0 * t) + gam / (2 * w0) * math.sin(w0 * t))
+    assert np.max(np.abs(out["E"][1] - y)) < 1e-9
+    assert np.max(np.abs(out["E"][0])) < 1e-12
+
+
+def test_gaussian_source_uniform(oracle):
+    spec = oracle.BoxSpec(nel=(2, 2, 2), N=3)
+    c = oracle.BoxCase(spec)
+    a = 0.7
+    c.set_source(a * c.bm1, direction=(0, 0, 1), t0=0.5, tw=0.2)
+    out = c.run(np.zeros((3, c.npts)), np.zeros((3, c.npts)), 0.005, 200)
+    t = out["time"]
+    exact = a * 0.2 * math.sqrt(math.pi) / 2 * (math.erf((t - 0.5) / 0.2) - math.erf(-0.5 / 0.2))
+    assert np.max(np.abs(out["E"][2] - exact)) < 1e-10
