@@ -68,6 +68,24 @@ class StreamTimer:
         return float(t.value)
 
 
+def copy_calibration(nbytes):
+    """One device-to-device hipMemcpy of nbytes (the runtime's plain copy
+    kernel): under rocprofv3 --pmc its FETCH_SIZE / WRITE_SIZE per dispatch
+    give the counter's scale against a known byte count."""
+    hip = _hip()
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0   # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_calibration_bytes": nbytes, "copies": 3}), flush=True)
+
+
 def workload(nel=32, N=7):
     from nekcem_amd.mesh import BoxMesh
     from nekcem_amd import usr as cases
@@ -101,7 +119,12 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=1,
                     help="steps of the CPU oracle for cpu_baseline (0 = skip)")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run a device-to-device copy of BYTES (counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: this build has no multi-GPU path "
               f"(asked for --gpus {args.gpus}); the element sharding across GPUs is "

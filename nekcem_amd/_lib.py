@@ -11,7 +11,9 @@ import ctypes as C
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "lib", "libnekcem_hip.so")
+# NKC_LIBRARY may point at an alternative build of the same sources (e.g. a
+# tuning variant); by default the in-tree build is used.
+LIB_PATH = os.environ.get("NKC_LIBRARY") or os.path.join(_HERE, "lib", "libnekcem_hip.so")
 HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "nekcem_hip.h")
 
 NKC_OK = 0

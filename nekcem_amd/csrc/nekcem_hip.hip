@@ -98,9 +98,10 @@ bool close(double x, double y, double scale, double tol) { return std::fabs(x - 
 
 template <int N1, int MAT, int MODE>
 void launch_t(const StageArgs &a, int grid, hipStream_t s) {
-    constexpr int BS = ((N1 * N1 + 63) / 64) * 64;
-    hipLaunchKernelGGL((stage_kernel<N1, MAT, MODE>), dim3(grid), dim3(BS), 0, s, a);
+    hipLaunchKernelGGL((stage_kernel<N1, MAT, MODE>), dim3(grid), dim3(Shape<N1>::BS), 0, s, a);
 }
+
+int block_threads(int n1) { return ((n1 * n1 + 63) / 64) * 64 * 4; }
 
 template <int N1>
 void launch_n(const StageArgs &a, int mat, int mode, int grid, hipStream_t s) {
@@ -539,8 +540,8 @@ int32_t nkc_info(nkc_handle *h, int64_t out[8]) {
     out[0] = h->npts;
     out[1] = h->mat;
     out[2] = 1;
-    out[3] = ((n1 * n1 + 63) / 64) * 64;
-    out[4] = (int64_t)sizeof(double) * (6 * n1 * n1 * n1 + n1 * n1);
+    out[3] = block_threads(n1);
+    out[4] = (int64_t)sizeof(double) * (6 * n1 * n1 * n1 + 25 * n1 * n1);
     out[5] = h->use_graph ? 1 : 0;
     out[6] = h->fs;
     out[7] = h->nelt;

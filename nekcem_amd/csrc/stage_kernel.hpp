@@ -9,13 +9,14 @@
 // src/cem_common.F:18-76), fused so every field value crosses HBM once per
 // stage in each direction.
 //
-// Mapping: one workgroup per element; lane (i,j) of the n1 x n1 lanes owns
-// the column (i,j,0..n1-1) of all six fields in registers.  The element is
-// also staged in LDS so lanes can read rows (r-derivative) and columns
-// (s-derivative); the t-derivative runs down the lane's own registers.  For
-// the four x/y faces the lanes are re-mapped onto face nodes, the fluxes go
-// through LDS back to the owning lanes; the two z faces are computed by the
-// owners directly.  Geometry is per element (axis-aligned affine hexahedra):
+// Mapping: one workgroup per element, four roles of n1 x n1 lanes each.
+// Every global load of the element (the six fields, this role's RK register,
+// the neighbour traces) is issued up front, so an element costs one HBM
+// round trip.  The element is staged in LDS; lane (i,j) of a role reads rows
+// (r-derivative), columns (s-derivative) and z-columns (t-derivative).  In
+// the face phase the four roles take one x/y face each with the lanes mapped
+// onto face nodes; the fluxes go through LDS to the owning lanes.  The two z
+// faces are evaluated by the roles whose level half holds them.  Geometry is per element (axis-aligned affine hexahedra):
 // g = (rx, sy, tz)/J and a lift factor per face (area / (J w3) at the face
 // node, constant on an affine face).
 //
@@ -26,7 +27,36 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+// Build-time tuning knobs (the defaults are the measured best on MI355X).
+#ifndef NKC_NT_STORE
+#define NKC_NT_STORE 1      // non-temporal stores for the streamed outputs
+#endif
+#ifndef NKC_NT_LOAD_K
+#define NKC_NT_LOAD_K 1     // non-temporal loads of the RK register
+#endif
+#ifndef NKC_WAVES_PER_EU
+#define NKC_WAVES_PER_EU 0  // >0: ask the register allocator for this occupancy
+#endif
+
 namespace nkc {
+
+template <class T>
+__device__ __forceinline__ void st_stream(T *p, T v) {
+#if NKC_NT_STORE
+    __builtin_nontemporal_store(v, p);
+#else
+    *p = v;
+#endif
+}
+
+template <class T>
+__device__ __forceinline__ T ld_stream(const T *p) {
+#if NKC_NT_LOAD_K
+    return __builtin_nontemporal_load(p);
+#else
+    return *p;
+#endif
+}
 
 // Kernel face order (symmetric): 0 x-, 1 x+, 2 y-, 3 y+, 4 z-, 5 z+.
 enum FaceType : int32_t { FACE_PEC = -1, FACE_NONE = -2 };
@@ -111,158 +141,209 @@ __device__ __forceinline__ void load_coef(const StageArgs &a, int e, int f, int 
     c4[0] = p[0]; c4[1] = p[1]; c4[2] = p[2]; c4[3] = p[3];
 }
 
-// LDS footprint (doubles) of one element: s_u (6 n1^3), s_f (4 faces x 6
-// components x n1^2), s_D (n1^2).
+// Launch shape and LDS layout for polynomial order N1-1.
+//   ROLES = 4 waves-groups per element: role & 1 selects the update group
+//   (0: H from -curl E, 1: E from curl H), role >> 1 the half of the levels.
+//   In phase 1 role r computes the fluxes of x/y face r.
+//   LDS element layout: u(i,j,k) of field f at f*N3 + k*N2 + j*N1 + (i ^ j)
+//   for power-of-two N1 (XOR swizzle: rows, columns and z-columns are all
+//   bank-conflict free), plain i otherwise.
 template <int N1>
-struct StageLds {
+struct Shape {
     static constexpr int N2 = N1 * N1, N3 = N2 * N1;
-    static constexpr int U = 0, F = 6 * N3, Dm = F + 24 * N2, TOTAL = Dm + N2;
+    static constexpr int LANES = ((N2 + 63) / 64) * 64;
+    static constexpr int ROLES = 4;
+    static constexpr int BS = LANES * ROLES;
+    static constexpr int KH = (N1 + 1) / 2;           // levels in the lower half
+    static constexpr bool SWZ = (N1 & (N1 - 1)) == 0;
+    static constexpr int LDS_DOUBLES = 6 * N3 + 24 * N2 + N2;
+    __device__ static __forceinline__ int idx(int i, int j, int k) {
+        return (SWZ ? (i ^ j) : i) + N1 * j + N2 * k;
+    }
 };
 
+#if NKC_WAVES_PER_EU > 0
+#define NKC_OCC __attribute__((amdgpu_waves_per_eu(NKC_WAVES_PER_EU)))
+#else
+#define NKC_OCC
+#endif
+
 template <int N1, int MAT, int MODE>
-__global__ void __launch_bounds__(((N1 * N1 + 63) / 64) * 64)
+__global__ void __launch_bounds__(Shape<N1>::BS) NKC_OCC
 stage_kernel(StageArgs a) {
-    constexpr int N2 = N1 * N1;
-    constexpr int N3 = N2 * N1;
-    using L = StageLds<N1>;
-    __shared__ double lds[L::TOTAL];
-    double *s_u = lds + L::U;    // [field][k][j][i]
-    double *s_f = lds + L::F;    // [face 0..3][comp 0..5][a + n1 b]
-    double *s_D = lds + L::Dm;   // D(i,l) at [i + n1 l]
+    using S = Shape<N1>;
+    constexpr int N2 = S::N2, N3 = S::N3, KH = S::KH, BS = S::BS, LANES = S::LANES;
+    __shared__ double s_u[6 * N3];      // element, swizzled
+    __shared__ double s_f[24 * N2];     // [face 0..3][comp 0..5][a + n1 b]
+    __shared__ double s_D[N2];          // D(i,l) at [i + n1 l]
 
     const int e = remap_element(blockIdx.x, a.nelt, a.per_xcd);
     if (e >= a.nelt) return;   // whole workgroup leaves together
-    const int tid = threadIdx.x;
-    const bool act = tid < N2;
-    const int i = tid % N1, j = tid / N1;
+    const int t = threadIdx.x;
+    const int role = t / LANES, q = t % LANES;
+    const bool act = q < N2;
+    const int i = q % N1, j = q / N1;
+    const int grp = role & 1;
+    const int half = role >> 1;
+    const int k0 = half ? KH : 0;
     const int64_t eo = (int64_t)e * N3;
     const int64_t fs = a.fs;
 
-    // ---- phase 0: element -> LDS (coalesced: lanes cover one k-slice)
-    if (act) {
-        s_D[tid] = a.D[tid];
+    int nb[6];
 #pragma unroll
-        for (int f = 0; f < 6; ++f)
+    for (int f = 0; f < 6; ++f) nb[f] = a.nbr[e * 6 + f];
+
+    // ---- phase 0: every global load of the element, issued up front ----
+    constexpr int NLOAD = (6 * N3 + BS - 1) / BS;
+    double ld[NLOAD];
 #pragma unroll
-            for (int k = 0; k < N1; ++k) s_u[f * N3 + k * N2 + tid] = a.uin[f * fs + eo + k * N2 + tid];
+    for (int n = 0; n < NLOAD; ++n) {
+        const int v = t + BS * n;
+        ld[n] = (v < 6 * N3) ? a.uin[(v / N3) * fs + eo + (v % N3)] : 0.0;
+    }
+    // RK register of this role's fields and levels
+    double kp[3][KH];
+    if (MODE == MODE_RK && act) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int kk = 0; kk < KH; ++kk) {
+                const int k = k0 + kk;
+                kp[c][kk] = (k < N1) ? ld_stream(a.kbuf + (3 * grp + c) * fs + eo + k * N2 + q) : 0.0;
+            }
+    }
+    // neighbour trace for this role's x/y face (node (a,b) = (i,j))
+    double nxy[6];
+    int64_t nbn_xy = 0;
+    if (role == 0) nbn_xy = (N1 - 1) + N1 * i + N2 * j;   // x-: nbr x+ face
+    if (role == 1) nbn_xy = 0 + N1 * i + N2 * j;          // x+: nbr x- face
+    if (role == 2) nbn_xy = i + N1 * (N1 - 1) + N2 * j;   // y-: nbr y+ face
+    if (role == 3) nbn_xy = i + N2 * j;                   // y+: nbr y- face
+    const int nbr_xy = nb[role];
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
+        nxy[c] = (act && nbr_xy >= 0) ? a.uin[c * fs + (int64_t)nbr_xy * N3 + nbn_xy] : 0.0;
+    // neighbour trace of the z face inside this role's level half
+    double nz[6];
+    const int zface = half ? 5 : 4;
+    const int nbr_z = nb[zface];
+    const int64_t nbn_z = half ? (int64_t)q : (int64_t)(N1 - 1) * N2 + q;
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
+        nz[c] = (act && nbr_z >= 0) ? a.uin[c * fs + (int64_t)nbr_z * N3 + nbn_z] : 0.0;
+
+    if (t < N2) s_D[t] = a.D[t];
+#pragma unroll
+    for (int n = 0; n < NLOAD; ++n) {
+        const int v = t + BS * n;
+        if (v < 6 * N3) {
+            const int f = v / N3, r = v % N3;
+            s_u[f * N3 + S::idx(r % N1, (r / N1) % N1, r / N2)] = ld[n];
+        }
     }
     const double *geo = a.geo + (int64_t)e * 9;
     __syncthreads();
 
-    // ---- phase 1: x/y faces with lanes re-mapped onto face node (a,b)=(i,j)
+    // ---- phase 1: role r computes the flux of x/y face r at node (i,j) ----
     if (act) {
-        double cf[4], um[6], up[6], FH[3], FE[3];
+        int vi, vj, vk;
+        if (role == 0) { vi = 0;      vj = i;      vk = j; }
+        if (role == 1) { vi = N1 - 1; vj = i;      vk = j; }
+        if (role == 2) { vi = i;      vj = 0;      vk = j; }
+        if (role == 3) { vi = i;      vj = N1 - 1; vk = j; }
+        double um[6], cf[4], FH[3], FE[3];
+        const int own = S::idx(vi, vj, vk);
 #pragma unroll
-        for (int f = 0; f < 4; ++f) {
-            const int nb = a.nbr[e * 6 + f];
-            int64_t own, nbn;
-            if (f == 0) { own = 0 + N1 * i + N2 * j;          nbn = (N1 - 1) + N1 * i + N2 * j; }
-            if (f == 1) { own = (N1 - 1) + N1 * i + N2 * j;   nbn = 0 + N1 * i + N2 * j; }
-            if (f == 2) { own = i + N2 * j;                   nbn = i + N1 * (N1 - 1) + N2 * j; }
-            if (f == 3) { own = i + N1 * (N1 - 1) + N2 * j;   nbn = i + N2 * j; }
+        for (int c = 0; c < 6; ++c) um[c] = s_u[c * N3 + own];
+        load_coef<N1, MAT>(a, e, role, q, cf);
+        const double sg = (role & 1) ? 1.0 : -1.0;
+        if (role < 2) face_flux<0>(sg, nbr_xy, um, nxy, cf, geo[3 + role], FH, FE);
+        else          face_flux<1>(sg, nbr_xy, um, nxy, cf, geo[3 + role], FH, FE);
 #pragma unroll
-            for (int c = 0; c < 6; ++c) {
-                um[c] = s_u[c * N3 + own];
-                up[c] = (nb >= 0) ? a.uin[c * fs + (int64_t)nb * N3 + nbn] : 0.0;
-            }
-            load_coef<N1, MAT>(a, e, f, tid, cf);
-            if (f < 2) face_flux<0>(f == 0 ? -1.0 : 1.0, nb, um, up, cf, geo[3 + f], FH, FE);
-            else       face_flux<1>(f == 2 ? -1.0 : 1.0, nb, um, up, cf, geo[3 + f], FH, FE);
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                s_f[(f * 6 + c) * N2 + tid] = FH[c];
-                s_f[(f * 6 + 3 + c) * N2 + tid] = FE[c];
-            }
+        for (int c = 0; c < 3; ++c) {
+            s_f[(role * 6 + c) * N2 + q] = FH[c];
+            s_f[(role * 6 + 3 + c) * N2 + q] = FE[c];
         }
     }
     __syncthreads();
     if (!act) return;
 
-    // ---- phase 2: one level at a time: volume curl, z faces, lift of the
-    // x/y fluxes, inverse mass, RK update
+    // ---- phase 2: this role's levels; group 0 updates H from -curl E,
+    // group 1 updates E from curl H ----
     const double gx = geo[0], gy = geo[1], gz = geo[2];
-    const int nbzm = a.nbr[e * 6 + 4], nbzp = a.nbr[e * 6 + 5];
-    double ie = 0.0, im = 0.0;
-    if (MAT == MAT_ELEM) { ie = a.ieps[e]; im = a.imu[e]; }
+    const int src = grp ? 0 : 3;             // fields differentiated
+    const double sgn = grp ? 1.0 : -1.0;
+    const double *S0 = s_u + (src + 0) * N3;
+    const double *S1 = s_u + (src + 1) * N3;
+    const double *S2 = s_u + (src + 2) * N3;
+    double Dr[N1], Ds[N1];
 #pragma unroll
-    for (int k = 0; k < N1; ++k) {
-        double Hy_r = 0, Hz_r = 0, Ey_r = 0, Ez_r = 0;
-        double Hx_s = 0, Hz_s = 0, Ex_s = 0, Ez_s = 0;
-        double Hx_t = 0, Hy_t = 0, Ex_t = 0, Ey_t = 0;
-        const double *row = s_u + k * N2 + N1 * j;   // u(l, j, k)
-        const double *col = s_u + k * N2 + i;        // u(i, l, k)
-        const double *clm = s_u + N1 * j + i;        // u(i, j, l)
+    for (int l = 0; l < N1; ++l) { Dr[l] = s_D[i + N1 * l]; Ds[l] = s_D[j + N1 * l]; }
+    double minv = 0.0;
+    if (MAT == MAT_ELEM) minv = grp ? a.ieps[e] : a.imu[e];
+    double cfz[4];
+    load_coef<N1, MAT>(a, e, zface, q, cfz);
+#pragma unroll
+    for (int kk = 0; kk < KH; ++kk) {
+        const int k = k0 + kk;
+        if (k >= N1) break;
+        double S1_r = 0, S2_r = 0, S0_s = 0, S2_s = 0, S0_t = 0, S1_t = 0;
 #pragma unroll
         for (int l = 0; l < N1; ++l) {
-            const double dil = s_D[i + N1 * l], djl = s_D[j + N1 * l], dkl = s_D[k + N1 * l];
-            Hy_r += dil * row[1 * N3 + l];
-            Hz_r += dil * row[2 * N3 + l];
-            Ey_r += dil * row[4 * N3 + l];
-            Ez_r += dil * row[5 * N3 + l];
-            Hx_s += djl * col[0 * N3 + N1 * l];
-            Hz_s += djl * col[2 * N3 + N1 * l];
-            Ex_s += djl * col[3 * N3 + N1 * l];
-            Ez_s += djl * col[5 * N3 + N1 * l];
-            Hx_t += dkl * clm[0 * N3 + N2 * l];
-            Hy_t += dkl * clm[1 * N3 + N2 * l];
-            Ex_t += dkl * clm[3 * N3 + N2 * l];
-            Ey_t += dkl * clm[4 * N3 + N2 * l];
+            const int pr = S::idx(l, j, k), ps = S::idx(i, l, k), pt = S::idx(i, j, l);
+            const double dkl = s_D[k + N1 * l];
+            S1_r += Dr[l] * S1[pr];
+            S2_r += Dr[l] * S2[pr];
+            S0_s += Ds[l] * S0[ps];
+            S2_s += Ds[l] * S2[ps];
+            S0_t += dkl * S0[pt];
+            S1_t += dkl * S1[pt];
         }
-        // resH = -W curl E, resE = W curl H (diagonal metric), per unit J w3
-        double r[6];
-        r[0] = -(gy * Ez_s - gz * Ey_t);
-        r[1] = -(gz * Ex_t - gx * Ez_r);
-        r[2] = -(gx * Ey_r - gy * Ex_s);
-        r[3] = gy * Hz_s - gz * Hy_t;
-        r[4] = gz * Hx_t - gx * Hz_r;
-        r[5] = gx * Hy_r - gy * Hx_s;
-        const int64_t vnode = k * N2 + tid;
-        if (k == 0 || k == N1 - 1) {
-            const int zf = (k == 0) ? 4 : 5;
-            const int nb = (k == 0) ? nbzm : nbzp;
-            const int64_t nbn = (k == 0) ? (N1 - 1) * N2 + tid : tid;
-            double cf[4], um[6], up[6], FH[3], FE[3];
+        double r[3];
+        r[0] = sgn * (gy * S2_s - gz * S1_t);
+        r[1] = sgn * (gz * S0_t - gx * S2_r);
+        r[2] = sgn * (gx * S1_r - gy * S0_s);
+        const int own = S::idx(i, j, k);
+        if (k == (half ? N1 - 1 : 0)) {
+            double um[6], FH[3], FE[3];
 #pragma unroll
-            for (int c = 0; c < 6; ++c) {
-                um[c] = s_u[c * N3 + vnode];
-                up[c] = (nb >= 0) ? a.uin[c * fs + (int64_t)nb * N3 + nbn] : 0.0;
-            }
-            load_coef<N1, MAT>(a, e, zf, tid, cf);
-            face_flux<2>(k == 0 ? -1.0 : 1.0, nb, um, up, cf, geo[3 + zf], FH, FE);
+            for (int c = 0; c < 6; ++c) um[c] = s_u[c * N3 + own];
+            face_flux<2>(half ? 1.0 : -1.0, nbr_z, um, nz, cfz, geo[3 + zface], FH, FE);
 #pragma unroll
-            for (int c = 0; c < 3; ++c) { r[c] += FH[c]; r[3 + c] += FE[c]; }
+            for (int c = 0; c < 3; ++c) r[c] += grp ? FE[c] : FH[c];
         }
+        const int co = 3 * grp;
         if (i == 0) {
 #pragma unroll
-            for (int c = 0; c < 6; ++c) r[c] += s_f[(0 * 6 + c) * N2 + j + N1 * k];
+            for (int c = 0; c < 3; ++c) r[c] += s_f[(0 * 6 + co + c) * N2 + j + N1 * k];
         }
         if (i == N1 - 1) {
 #pragma unroll
-            for (int c = 0; c < 6; ++c) r[c] += s_f[(1 * 6 + c) * N2 + j + N1 * k];
+            for (int c = 0; c < 3; ++c) r[c] += s_f[(1 * 6 + co + c) * N2 + j + N1 * k];
         }
         if (j == 0) {
 #pragma unroll
-            for (int c = 0; c < 6; ++c) r[c] += s_f[(2 * 6 + c) * N2 + i + N1 * k];
+            for (int c = 0; c < 3; ++c) r[c] += s_f[(2 * 6 + co + c) * N2 + i + N1 * k];
         }
         if (j == N1 - 1) {
 #pragma unroll
-            for (int c = 0; c < 6; ++c) r[c] += s_f[(3 * 6 + c) * N2 + i + N1 * k];
+            for (int c = 0; c < 3; ++c) r[c] += s_f[(3 * 6 + co + c) * N2 + i + N1 * k];
         }
-        // inverse mass: hbm1 = 1/(mu J w3), ebm1 = 1/(eps J w3); J w3 is
-        // already divided out of the volume and lift terms
-        if (MAT == MAT_NODE) { ie = a.ieps[eo + vnode]; im = a.imu[eo + vnode]; }
+        // inverse mass (the J w3 of hbm1/ebm1 is already divided out)
+        const int64_t node = eo + k * N2 + q;
+        if (MAT == MAT_NODE) minv = grp ? a.ieps[node] : a.imu[node];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) { r[c] *= im; r[3 + c] *= ie; }
+        for (int c = 0; c < 3; ++c) r[c] *= minv;
         if (MODE == MODE_RESIDUAL) {
 #pragma unroll
-            for (int c = 0; c < 6; ++c) a.res[c * fs + eo + vnode] = r[c];
+            for (int c = 0; c < 3; ++c) st_stream(a.res + (co + c) * fs + node, r[c]);
         } else {
 #pragma unroll
-            for (int c = 0; c < 6; ++c) {
-                const int64_t p = c * fs + eo + vnode;
-                const double kk = a.rka * a.kbuf[p] + a.dt * r[c];
-                a.kbuf[p] = kk;
-                a.uout[p] = s_u[c * N3 + vnode] + a.rkb * kk;
+            for (int c = 0; c < 3; ++c) {
+                const int64_t p = (co + c) * fs + node;
+                const double kk2 = a.rka * kp[c][kk] + a.dt * r[c];
+                st_stream(a.kbuf + p, kk2);
+                st_stream(a.uout + p, s_u[(co + c) * N3 + own] + a.rkb * kk2);
             }
         }
     }
