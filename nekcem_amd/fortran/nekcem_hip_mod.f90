@@ -1,0 +1,151 @@
+! nekcem_hip_mod.f90 -- iso_c_binding interface of libnekcem_hip.so for the
+! Fortran host (NekCEM's driver).  Mirrors include/nekcem_hip.h one to one.
+! A NekCEM build calls these in place of cem_maxwell_op_rk
+! (src/cem_maxwell.F:327-345), with the COMMON arrays of src/EMWAVE, src/GEOM,
+! src/INPUT passed as they are (REAL*8, 1-based INTEGER index arrays).
+module nekcem_hip
+  use iso_c_binding
+  implicit none
+  private
+
+  integer(c_int32_t), parameter, public :: NKC_OK = 0, NKC_ERR_ARG = 1, &
+       NKC_ERR_HIP = 2, NKC_ERR_UNSUPPORTED = 3, NKC_ERR_BLOWUP = 4, &
+       NKC_ERR_NODEVICE = 5
+
+  ! struct nkc_desc
+  type, bind(C), public :: nkc_desc
+     integer(c_int32_t) :: N = 0
+     integer(c_int32_t) :: ifupwind = 1
+     integer(c_int64_t) :: nelt = 0
+     type(c_ptr) :: dxm1 = c_null_ptr, wxm1 = c_null_ptr
+     type(c_ptr) :: rxmn = c_null_ptr, rymn = c_null_ptr, rzmn = c_null_ptr
+     type(c_ptr) :: sxmn = c_null_ptr, symn = c_null_ptr, szmn = c_null_ptr
+     type(c_ptr) :: txmn = c_null_ptr, tymn = c_null_ptr, tzmn = c_null_ptr
+     type(c_ptr) :: jacm = c_null_ptr, bmn = c_null_ptr
+     type(c_ptr) :: unxm = c_null_ptr, unym = c_null_ptr, unzm = c_null_ptr
+     type(c_ptr) :: aream = c_null_ptr
+     type(c_ptr) :: cemface = c_null_ptr, face_partner = c_null_ptr
+     type(c_ptr) :: cempec = c_null_ptr
+     integer(c_int64_t) :: ncempec = 0
+     type(c_ptr) :: permittivity = c_null_ptr, permeability = c_null_ptr
+     type(c_ptr) :: Z_0 = c_null_ptr, Z_1 = c_null_ptr
+     type(c_ptr) :: Y_0 = c_null_ptr, Y_1 = c_null_ptr
+  end type nkc_desc
+
+  public :: nkc_create, nkc_destroy, nkc_set_state, nkc_get_state, nkc_step, &
+       nkc_residual, nkc_check_finite, nkc_sync, nkc_stream, nkc_info, &
+       nkc_set_graph, nkc_last_error, nkc_check, nkc_error_message
+
+  interface
+     integer(c_int32_t) function nkc_create(d, h) bind(C, name='nkc_create')
+       import :: c_int32_t, c_ptr, nkc_desc
+       type(nkc_desc), intent(in) :: d
+       type(c_ptr), intent(out) :: h
+     end function nkc_create
+
+     integer(c_int32_t) function nkc_destroy(h) bind(C, name='nkc_destroy')
+       import :: c_int32_t, c_ptr
+       type(c_ptr), value :: h
+     end function nkc_destroy
+
+     ! hn, en: (ld,3); khn, ken: c_loc of (ld,3) arrays or c_null_ptr
+     integer(c_int32_t) function nkc_set_state(h, hn, en, khn, ken, ld) &
+          bind(C, name='nkc_set_state')
+       import :: c_int32_t, c_int64_t, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(in) :: hn(*), en(*)
+       type(c_ptr), value :: khn, ken
+       integer(c_int64_t), value :: ld
+     end function nkc_set_state
+
+     integer(c_int32_t) function nkc_get_state(h, hn, en, khn, ken, ld) &
+          bind(C, name='nkc_get_state')
+       import :: c_int32_t, c_int64_t, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(out) :: hn(*), en(*)
+       type(c_ptr), value :: khn, ken
+       integer(c_int64_t), value :: ld
+     end function nkc_get_state
+
+     integer(c_int32_t) function nkc_step(h, time, dt, nsteps) bind(C, name='nkc_step')
+       import :: c_int32_t, c_int64_t, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(inout) :: time
+       real(c_double), value :: dt
+       integer(c_int64_t), value :: nsteps
+     end function nkc_step
+
+     integer(c_int32_t) function nkc_residual(h, rktime, reshn, resen, ld) &
+          bind(C, name='nkc_residual')
+       import :: c_int32_t, c_int64_t, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), value :: rktime
+       real(c_double), intent(out) :: reshn(*), resen(*)
+       integer(c_int64_t), value :: ld
+     end function nkc_residual
+
+     integer(c_int32_t) function nkc_check_finite(h, maxabs) bind(C, name='nkc_check_finite')
+       import :: c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(out) :: maxabs
+     end function nkc_check_finite
+
+     integer(c_int32_t) function nkc_sync(h) bind(C, name='nkc_sync')
+       import :: c_int32_t, c_ptr
+       type(c_ptr), value :: h
+     end function nkc_sync
+
+     integer(c_int32_t) function nkc_stream(h, stream) bind(C, name='nkc_stream')
+       import :: c_int32_t, c_ptr
+       type(c_ptr), value :: h
+       type(c_ptr), intent(out) :: stream
+     end function nkc_stream
+
+     integer(c_int32_t) function nkc_info(h, out) bind(C, name='nkc_info')
+       import :: c_int32_t, c_int64_t, c_ptr
+       type(c_ptr), value :: h
+       integer(c_int64_t), intent(out) :: out(8)
+     end function nkc_info
+
+     integer(c_int32_t) function nkc_set_graph(h, enable) bind(C, name='nkc_set_graph')
+       import :: c_int32_t, c_ptr
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: enable
+     end function nkc_set_graph
+
+     type(c_ptr) function nkc_last_error() bind(C, name='nkc_last_error')
+       import :: c_ptr
+     end function nkc_last_error
+  end interface
+
+contains
+
+  ! Message of the last failed call as a Fortran string.
+  function nkc_error_message() result(msg)
+    character(len=512) :: msg
+    character(kind=c_char), pointer :: s(:)
+    type(c_ptr) :: p
+    integer :: i
+    msg = ' '
+    p = nkc_last_error()
+    if (.not. c_associated(p)) return
+    call c_f_pointer(p, s, [512])
+    do i = 1, 512
+       if (s(i) == c_null_char) exit
+       msg(i:i) = s(i)
+    end do
+  end function nkc_error_message
+
+  ! Any nonzero status ends the run, as exitt(rc) does in the reference
+  ! (src/nek5_comm_mpi.F:650-690).
+  subroutine nkc_check(rc, where)
+    integer(c_int32_t), intent(in) :: rc
+    character(len=*), intent(in) :: where
+    if (rc /= NKC_OK) then
+       write(6, '(a,a,a,i0,a,a)') 'nekcem_hip: ', where, ' failed, status ', rc, ': ', &
+            trim(nkc_error_message())
+       error stop 1
+    end if
+  end subroutine nkc_check
+
+end module nekcem_hip
