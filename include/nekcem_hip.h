@@ -94,6 +94,26 @@ int32_t nkc_step(nkc_handle *h, double *time, double dt, int64_t nsteps);
 int32_t nkc_residual(nkc_handle *h, double rktime, double *reshn, double *resen,
                      int64_t ld);
 
+/* usersrc-style source on E (the Gaussian pulse of the configs): every
+ * stage adds amp(x) * dir * exp(-((t - t0)/tw)^2) to resE before the inverse
+ * mass, with amp (npts) bm1-weighted as a usersrc hook would add it
+ * (src/cem_maxwell.F:500-504) and t the stage time rktime.  amp = NULL
+ * removes the source. */
+int32_t nkc_set_source(nkc_handle *h, const double *amp, const double dir[3],
+                       double t0, double tw);
+
+/* Drude ADE of cem_maxwell_drude (src/cem_maxwell.F:3095-3147) over the
+ * ndrude volume nodes dindex (1-based): resE -= J*bmn, resJ = -a J + b E,
+ * J advanced by the same low-storage RK.  params is (a, b) per listed node
+ * (a = gamma, b = omega_p^2), interleaved [2*l + 0/1]; J and kJ are
+ * [3][ndrude] (component-major) or NULL for zero.  ndrude = 0 removes it. */
+int32_t nkc_set_drude(nkc_handle *h, int64_t ndrude, const int32_t *dindex,
+                      const double *params, const double *J, const double *kJ);
+
+/* Copy the Drude current, its RK register and the last residual (from
+ * nkc_residual) back, [3][ndrude] each; any pointer may be NULL. */
+int32_t nkc_get_drude(nkc_handle *h, double *J, double *kJ, double *resJ);
+
 /* chk_blowup (src/cem_common.F:1150-1165): *maxabs = max |u| over the six
  * fields; returns NKC_ERR_BLOWUP if a value is non-finite or above 1e20. */
 int32_t nkc_check_finite(nkc_handle *h, double *maxabs);

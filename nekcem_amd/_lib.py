@@ -59,6 +59,9 @@ SIGNATURES = [
     ("nkc_get_state", C.c_int32, [C.c_void_p, _dp, _dp, _dp, _dp, C.c_int64]),
     ("nkc_step", C.c_int32, [C.c_void_p, _dp, C.c_double, C.c_int64]),
     ("nkc_residual", C.c_int32, [C.c_void_p, C.c_double, _dp, _dp, C.c_int64]),
+    ("nkc_set_source", C.c_int32, [C.c_void_p, _dp, _dp, C.c_double, C.c_double]),
+    ("nkc_set_drude", C.c_int32, [C.c_void_p, C.c_int64, _ip, _dp, _dp, _dp]),
+    ("nkc_get_drude", C.c_int32, [C.c_void_p, _dp, _dp, _dp]),
     ("nkc_check_finite", C.c_int32, [C.c_void_p, _dp]),
     ("nkc_sync", C.c_int32, [C.c_void_p]),
     ("nkc_stream", C.c_int32, [C.c_void_p, C.POINTER(C.c_void_p)]),

@@ -53,6 +53,11 @@ const double RK4B[5] = {1432997174477.0 / 9575080441755.0,
                         1720146321549.0 / 2090206949498.0,
                         3134564353537.0 / 4481467310338.0,
                         2277821191437.0 / 14882151754819.0};
+const double RK4C[5] = {0.0,
+                        1432997174477.0 / 9575080441755.0,
+                        2526269341429.0 / 6820363962896.0,
+                        2006345519317.0 / 3224310063776.0,
+                        2802321613138.0 / 2924317926251.0};
 
 }  // namespace
 
@@ -72,6 +77,22 @@ struct nkc_handle {
     bool use_graph = true;
     double graph_dt = NAN;
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    // device time: tdev[0] = time at the start of the next step, tdev[1] =
+    // stage time for nkc_residual
+    double *tdev = nullptr;
+    double t_host = 0.0;
+    // per-element geometry kept on the host (source weights)
+    std::vector<double> jac_e;
+    std::vector<double> wgll;
+    // source
+    int32_t *src_off = nullptr;
+    double *src_amp = nullptr;
+    double src_dir[3] = {0, 0, 0}, src_t0 = 0, src_tw = 1;
+    // Drude
+    int64_t ndrude = 0, ndelem = 0;
+    int32_t *drude_off = nullptr;
+    double *dJ = nullptr, *dkJ = nullptr, *dpar = nullptr, *dresJ = nullptr;
+    std::vector<int64_t> drude_slot;      // listed node -> compact (elem*3*n3 + vloc)
 };
 
 namespace {
@@ -137,6 +158,17 @@ int32_t launch_stage(nkc_handle *h, const StageArgs &a, int mode) {
 
 StageArgs base_args(nkc_handle *h) {
     StageArgs a{};
+    a.tdev = h->tdev;
+    a.src_off = h->src_off;
+    a.src_amp = h->src_amp;
+    for (int c = 0; c < 3; ++c) a.src_dir[c] = h->src_dir[c];
+    a.src_t0 = h->src_t0;
+    a.src_tw = h->src_tw;
+    a.drude_off = h->drude_off;
+    a.dJ = h->dJ;
+    a.dkJ = h->dkJ;
+    a.dpar = h->dpar;
+    a.dresJ = h->dresJ;
     a.D = h->D;
     a.geo = h->geo;
     a.nbr = h->nbr;
@@ -148,6 +180,10 @@ StageArgs base_args(nkc_handle *h) {
     return a;
 }
 
+// time = time + dt at the end of a step (src/cem_drive.F:639), on the device
+// so a replayed graph sees the right stage times.
+__global__ void advance_time(double *t, double dt) { t[0] = t[0] + dt; }
+
 // Queue one RK step starting from buffer `from` (ends in 1-from).
 int32_t queue_step(nkc_handle *h, int from, double dt) {
     for (int s = 0; s < 5; ++s) {
@@ -158,10 +194,13 @@ int32_t queue_step(nkc_handle *h, int from, double dt) {
         a.kbuf = h->k;
         a.rka = RK4A[s];
         a.rkb = RK4B[s];
+        a.rkc = RK4C[s];
         a.dt = dt;
         int32_t rc = launch_stage(h, a, MODE_RK);
         if (rc) return rc;
     }
+    hipLaunchKernelGGL(advance_time, dim3(1), dim3(1), 0, h->stream, h->tdev, dt);
+    HIPCHK(hipGetLastError());
     return NKC_OK;
 }
 
@@ -225,7 +264,10 @@ void free_all(nkc_handle *h) {
     drop_graphs(h);
     (void)hipFree(h->u[0]); (void)hipFree(h->u[1]); (void)hipFree(h->k); (void)hipFree(h->res);
     (void)hipFree(h->D); (void)hipFree(h->geo); (void)hipFree(h->fco); (void)hipFree(h->ieps); (void)hipFree(h->imu);
-    (void)hipFree(h->nbr); (void)hipFree(h->red);
+    (void)hipFree(h->nbr); (void)hipFree(h->red); (void)hipFree(h->tdev);
+    (void)hipFree(h->src_off); (void)hipFree(h->src_amp);
+    (void)hipFree(h->drude_off); (void)hipFree(h->dJ); (void)hipFree(h->dkJ);
+    (void)hipFree(h->dpar); (void)hipFree(h->dresJ);
     if (h->stream) (void)hipStreamDestroy(h->stream);
 }
 
@@ -388,6 +430,9 @@ int32_t nkc_create(const nkc_desc *d, nkc_handle **out) {
         return fail(NKC_ERR_NODEVICE, "no HIP device visible");
 
     nkc_handle *h = new nkc_handle();
+    h->jac_e.resize(nelt);
+    for (int64_t e = 0; e < nelt; ++e) h->jac_e[e] = d->jacm[e * n3];
+    h->wgll.assign(d->wxm1, d->wxm1 + n1);
     h->N1 = n1;
     h->nelt = nelt;
     h->npts = npts;
@@ -407,6 +452,8 @@ int32_t nkc_create(const nkc_desc *d, nkc_handle **out) {
         }
         if (hipMemset(h->u[0], 0, ub) != hipSuccess || hipMemset(h->u[1], 0, ub) != hipSuccess ||
             hipMemset(h->k, 0, ub) != hipSuccess) { rc = fail(NKC_ERR_HIP, "hipMemset"); break; }
+        if (hipMalloc((void **)&h->tdev, 2 * sizeof(double)) != hipSuccess ||
+            hipMemset(h->tdev, 0, 2 * sizeof(double)) != hipSuccess) { rc = fail(NKC_ERR_HIP, "hipMalloc time"); break; }
         std::vector<double> Dv(d->dxm1, d->dxm1 + n2);
         if (chk(upload(&h->D, Dv))) break;
         if (chk(upload(&h->geo, geo))) break;
@@ -468,6 +515,12 @@ int32_t nkc_step(nkc_handle *h, double *time, double dt, int64_t nsteps) {
     if (nsteps < 0) return fail(NKC_ERR_ARG, "nsteps < 0");
     if (!std::isfinite(dt)) return fail(NKC_ERR_ARG, "dt not finite");
     if (nsteps == 0) return NKC_OK;
+    const double t0 = time ? *time : h->t_host;
+    if (!(t0 == h->t_host)) {
+        HIPCHK(hipStreamSynchronize(h->stream));
+        HIPCHK(hipMemcpy(h->tdev, &t0, sizeof(double), hipMemcpyHostToDevice));
+        h->t_host = t0;
+    }
     if (h->use_graph) {
         if (!(h->graph_dt == dt)) {
             int32_t rc = capture(h, dt);
@@ -484,16 +537,20 @@ int32_t nkc_step(nkc_handle *h, double *time, double dt, int64_t nsteps) {
             h->cur ^= 1;
         }
     }
-    if (time) *time += (double)nsteps * dt;
+    for (int64_t s = 0; s < nsteps; ++s) h->t_host = h->t_host + dt;   // same sums as the device
+    if (time) *time = h->t_host;
     return NKC_OK;
 }
 
 int32_t nkc_residual(nkc_handle *h, double rktime, double *reshn, double *resen, int64_t ld) {
-    (void)rktime;
     if (!h || !reshn || !resen) return fail(NKC_ERR_ARG, "null handle or output");
     if (ld < h->npts) return fail(NKC_ERR_ARG, "ld < npts");
     if (!h->res) HIPCHK(hipMalloc((void **)&h->res, sizeof(double) * 6 * h->fs));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipMemcpy(h->tdev + 1, &rktime, sizeof(double), hipMemcpyHostToDevice));
     StageArgs a = base_args(h);
+    a.tdev = h->tdev + 1;
+    a.rkc = 0.0;
     a.uin = h->u[h->cur];
     a.res = h->res;
     int32_t rc = launch_stage(h, a, MODE_RESIDUAL);
@@ -503,6 +560,105 @@ int32_t nkc_residual(nkc_handle *h, double rktime, double *reshn, double *resen,
     for (int c = 0; c < 3; ++c) {
         HIPCHK(hipMemcpy(reshn + c * ld, h->res + c * h->fs, nb, hipMemcpyDeviceToHost));
         HIPCHK(hipMemcpy(resen + c * ld, h->res + (3 + c) * h->fs, nb, hipMemcpyDeviceToHost));
+    }
+    return NKC_OK;
+}
+
+int32_t nkc_set_source(nkc_handle *h, const double *amp, const double dir[3], double t0, double tw) {
+    if (!h) return fail(NKC_ERR_ARG, "null handle");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    (void)hipFree(h->src_off); h->src_off = nullptr;
+    (void)hipFree(h->src_amp); h->src_amp = nullptr;
+    int32_t rc = drop_graphs(h);
+    if (rc || !amp) return rc;
+    if (!dir || !(tw != 0.0) || !std::isfinite(t0) || !std::isfinite(tw)) return fail(NKC_ERR_ARG, "bad source direction/time");
+    const int n1 = h->N1, n2 = n1 * n1, n3 = n2 * n1;
+    std::vector<int32_t> off(h->nelt, -1);
+    std::vector<double> vals;
+    for (int64_t e = 0; e < h->nelt; ++e) {
+        bool any = false;
+        for (int q = 0; q < n3; ++q) any |= amp[e * n3 + q] != 0.0;
+        if (!any) continue;
+        off[e] = (int32_t)(vals.size() / n3);
+        for (int k = 0; k < n1; ++k)
+            for (int j = 0; j < n1; ++j)
+                for (int i = 0; i < n1; ++i) {
+                    // usersrc adds amp (bm1-weighted) before ebm1; the kernel
+                    // works per unit J w3
+                    const double bm = h->jac_e[e] * (h->wgll[i] * h->wgll[j] * h->wgll[k]);
+                    vals.push_back(amp[e * n3 + i + n1 * j + n2 * k] / bm);
+                }
+    }
+    if (vals.empty()) return NKC_OK;
+    if ((rc = upload(&h->src_off, off))) return rc;
+    if ((rc = upload(&h->src_amp, vals))) return rc;
+    for (int c = 0; c < 3; ++c) h->src_dir[c] = dir[c];
+    h->src_t0 = t0;
+    h->src_tw = tw;
+    return NKC_OK;
+}
+
+int32_t nkc_set_drude(nkc_handle *h, int64_t ndrude, const int32_t *dindex, const double *params,
+                      const double *J, const double *kJ) {
+    if (!h) return fail(NKC_ERR_ARG, "null handle");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    (void)hipFree(h->drude_off); h->drude_off = nullptr;
+    (void)hipFree(h->dJ); h->dJ = nullptr;
+    (void)hipFree(h->dkJ); h->dkJ = nullptr;
+    (void)hipFree(h->dpar); h->dpar = nullptr;
+    (void)hipFree(h->dresJ); h->dresJ = nullptr;
+    h->ndrude = 0;
+    h->ndelem = 0;
+    h->drude_slot.clear();
+    int32_t rc = drop_graphs(h);
+    if (rc || ndrude == 0) return rc;
+    if (ndrude < 0 || !dindex || !params) return fail(NKC_ERR_ARG, "bad Drude node list");
+    const int n3 = h->N1 * h->N1 * h->N1;
+    std::vector<int32_t> off(h->nelt, -1);
+    int32_t nde = 0;
+    for (int64_t l = 0; l < ndrude; ++l) {
+        const int64_t v = (int64_t)dindex[l] - 1;
+        if (v < 0 || v >= h->npts) return fail(NKC_ERR_ARG, "dindex[%lld]=%d out of range", (long long)l, dindex[l]);
+        const int64_t e = v / n3;
+        if (off[e] < 0) off[e] = nde++;
+    }
+    std::vector<double> par((size_t)nde * n3 * 2, 0.0), Jv((size_t)nde * 3 * n3, 0.0), kv = Jv;
+    h->drude_slot.resize(ndrude);
+    for (int64_t l = 0; l < ndrude; ++l) {
+        const int64_t v = (int64_t)dindex[l] - 1, e = v / n3, vl = v % n3;
+        const int64_t b = (int64_t)off[e] * 3 * n3 + vl;
+        h->drude_slot[l] = b;
+        par[((int64_t)off[e] * n3 + vl) * 2 + 0] = params[2 * l + 0];
+        par[((int64_t)off[e] * n3 + vl) * 2 + 1] = params[2 * l + 1];
+        for (int c = 0; c < 3; ++c) {
+            if (J) Jv[b + c * n3] = J[c * ndrude + l];
+            if (kJ) kv[b + c * n3] = kJ[c * ndrude + l];
+        }
+    }
+    if ((rc = upload(&h->drude_off, off))) return rc;
+    if ((rc = upload(&h->dpar, par))) return rc;
+    if ((rc = upload(&h->dJ, Jv))) return rc;
+    if ((rc = upload(&h->dkJ, kv))) return rc;
+    if ((rc = upload(&h->dresJ, kv))) return rc;
+    h->ndrude = ndrude;
+    h->ndelem = nde;
+    return NKC_OK;
+}
+
+int32_t nkc_get_drude(nkc_handle *h, double *J, double *kJ, double *resJ) {
+    if (!h) return fail(NKC_ERR_ARG, "null handle");
+    if (h->ndrude == 0) return fail(NKC_ERR_ARG, "no Drude region set");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    const int n3 = h->N1 * h->N1 * h->N1;
+    const size_t nb = sizeof(double) * h->ndelem * 3 * n3;
+    std::vector<double> buf(h->ndelem * 3 * n3);
+    double *dst[3] = {J, kJ, resJ};
+    double *srcp[3] = {h->dJ, h->dkJ, h->dresJ};
+    for (int w = 0; w < 3; ++w) {
+        if (!dst[w]) continue;
+        HIPCHK(hipMemcpy(buf.data(), srcp[w], nb, hipMemcpyDeviceToHost));
+        for (int64_t l = 0; l < h->ndrude; ++l)
+            for (int c = 0; c < 3; ++c) dst[w][c * h->ndrude + l] = buf[h->drude_slot[l] + c * n3];
     }
     return NKC_OK;
 }

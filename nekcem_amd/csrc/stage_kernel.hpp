@@ -79,6 +79,21 @@ struct StageArgs {
     int32_t nelt;
     int32_t per_xcd;                  // elements per XCD slice (grid remap)
     double rka, rkb, dt;
+    // stage time = *tdev + rkc*dt (tdev advanced on the device after stage 5)
+    const double *__restrict__ tdev;
+    double rkc;
+    // usersrc-style source on E: src_amp (un-weighted, per node of the
+    // elements with src_off[e] >= 0) * dir * exp(-((t-t0)/tw)^2)
+    const int32_t *__restrict__ src_off;
+    const double *__restrict__ src_amp;
+    double src_dir[3], src_t0, src_tw;
+    // Drude ADE (cem_maxwell_drude): elements with drude_off[e] >= 0 carry
+    // J, kJ ([3][n1^3] each) and (a, b) per node
+    const int32_t *__restrict__ drude_off;
+    double *__restrict__ dJ;
+    double *__restrict__ dkJ;
+    const double *__restrict__ dpar;
+    double *__restrict__ dresJ;       // MODE_RESIDUAL: resJ out (same layout as J)
 };
 
 // n x v for n = sg * e_AX.
@@ -328,6 +343,41 @@ stage_kernel(StageArgs a) {
         if (j == N1 - 1) {
 #pragma unroll
             for (int c = 0; c < 3; ++c) r[c] += s_f[(3 * 6 + co + c) * N2 + i + N1 * k];
+        }
+        const int vloc = k * N2 + q;
+        if (grp == 1 && a.src_off) {
+            // usersrc: resE += amp * bm1 * dir * g(t) before the inverse mass
+            const int so = a.src_off[e];
+            if (so >= 0) {
+                const double tt = (*a.tdev + a.rkc * a.dt - a.src_t0) / a.src_tw;
+                const double g = exp(-tt * tt);
+                const double am = a.src_amp[(int64_t)so * N3 + vloc] * g;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) r[c] += am * a.src_dir[c];
+            }
+        }
+        if (grp == 1 && a.drude_off) {
+            // cem_maxwell_drude: resE -= J bm1; resJ = -a J + b E; J's RK
+            // update uses the stage-start E and J
+            const int dd = a.drude_off[e];
+            if (dd >= 0) {
+                const int64_t jb = (int64_t)dd * 3 * N3 + vloc;
+                const double pa = a.dpar[((int64_t)dd * N3 + vloc) * 2 + 0];
+                const double pb = a.dpar[((int64_t)dd * N3 + vloc) * 2 + 1];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const double Jc = a.dJ[jb + c * N3];
+                    const double rJ = -pa * Jc + pb * s_u[(3 + c) * N3 + own];
+                    r[c] -= Jc;
+                    if (MODE == MODE_RESIDUAL) {
+                        a.dresJ[jb + c * N3] = rJ;
+                    } else {
+                        const double kj = a.rka * a.dkJ[jb + c * N3] + a.dt * rJ;
+                        a.dkJ[jb + c * N3] = kj;
+                        a.dJ[jb + c * N3] = Jc + a.rkb * kj;
+                    }
+                }
+            }
         }
         // inverse mass (the J w3 of hbm1/ebm1 is already divided out)
         const int64_t node = eo + k * N2 + q;

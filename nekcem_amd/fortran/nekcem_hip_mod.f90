@@ -33,7 +33,8 @@ module nekcem_hip
   end type nkc_desc
 
   public :: nkc_create, nkc_destroy, nkc_set_state, nkc_get_state, nkc_step, &
-       nkc_residual, nkc_check_finite, nkc_sync, nkc_stream, nkc_info, &
+       nkc_residual, nkc_set_source, nkc_set_drude, nkc_get_drude, &
+       nkc_check_finite, nkc_sync, nkc_stream, nkc_info, &
        nkc_set_graph, nkc_last_error, nkc_check, nkc_error_message
 
   interface
@@ -83,6 +84,31 @@ module nekcem_hip
        real(c_double), intent(out) :: reshn(*), resen(*)
        integer(c_int64_t), value :: ld
      end function nkc_residual
+
+     ! amp: c_loc of npts bm1-weighted amplitudes, or c_null_ptr to remove
+     integer(c_int32_t) function nkc_set_source(h, amp, dir, t0, tw) bind(C, name='nkc_set_source')
+       import :: c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: h, amp
+       real(c_double), intent(in) :: dir(3)
+       real(c_double), value :: t0, tw
+     end function nkc_set_source
+
+     ! dindex(ndrude) 1-based volume nodes; params(2,ndrude) = (gamma,
+     ! omega_p^2); J, kJ: c_loc of (ndrude,3) arrays or c_null_ptr
+     integer(c_int32_t) function nkc_set_drude(h, ndrude, dindex, params, J, kJ) &
+          bind(C, name='nkc_set_drude')
+       import :: c_int32_t, c_int64_t, c_ptr, c_double
+       type(c_ptr), value :: h
+       integer(c_int64_t), value :: ndrude
+       integer(c_int32_t), intent(in) :: dindex(*)
+       real(c_double), intent(in) :: params(*)
+       type(c_ptr), value :: J, kJ
+     end function nkc_set_drude
+
+     integer(c_int32_t) function nkc_get_drude(h, J, kJ, resJ) bind(C, name='nkc_get_drude')
+       import :: c_int32_t, c_ptr
+       type(c_ptr), value :: h, J, kJ, resJ
+     end function nkc_get_drude
 
      integer(c_int32_t) function nkc_check_finite(h, maxabs) bind(C, name='nkc_check_finite')
        import :: c_int32_t, c_ptr, c_double

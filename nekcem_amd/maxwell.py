@@ -76,6 +76,7 @@ class MaxwellSolver:
         self.time = 0.0
         self.istep = 0
         self.dt = None
+        self.ndrude = 0
         if not graph:
             _lib.check(L.nkc_set_graph(h, 0))
 
@@ -108,6 +109,40 @@ class MaxwellSolver:
         _lib.check(self._L.nkc_step(self._h, C.byref(t), float(dt), int(nsteps)))
         self.time = t.value
         self.istep += int(nsteps)
+
+    def set_source(self, amp, direction=(0.0, 0.0, 1.0), t0=0.0, tw=1.0):
+        """usersrc-style E source: resE += amp * dir * exp(-((t-t0)/tw)^2)
+        with amp (npts) bm1-weighted; amp=None removes it."""
+        if amp is None:
+            _lib.check(self._L.nkc_set_source(self._h, None, None, 0.0, 1.0))
+            return
+        amp = np.ascontiguousarray(amp, dtype=np.float64).reshape(self.npts)
+        d = np.ascontiguousarray(direction, dtype=np.float64)
+        self._src_keep = (amp, d)
+        _lib.check(self._L.nkc_set_source(self._h, _d(amp), _d(d), float(t0), float(tw)))
+
+    def set_drude(self, dindex, a, b, J=None, kJ=None):
+        """Drude ADE on volume nodes dindex (0-based): a = gamma, b =
+        omega_p^2 per listed node; J, kJ (3, ndrude) or None (zero)."""
+        dindex = np.asarray(dindex, dtype=np.int64)
+        nd = len(dindex)
+        if nd == 0:
+            _lib.check(self._L.nkc_set_drude(self._h, 0, None, None, None, None))
+            self.ndrude = 0
+            return
+        di = np.ascontiguousarray(dindex + 1, dtype=np.int32)
+        par = np.ascontiguousarray(np.stack([np.broadcast_to(a, (nd,)), np.broadcast_to(b, (nd,))], axis=1),
+                                   dtype=np.float64)
+        J = None if J is None else np.ascontiguousarray(J, dtype=np.float64).reshape(3, nd)
+        kJ = None if kJ is None else np.ascontiguousarray(kJ, dtype=np.float64).reshape(3, nd)
+        _lib.check(self._L.nkc_set_drude(self._h, nd, _i(di), _d(par), _d(J), _d(kJ)))
+        self.ndrude = nd
+
+    def get_drude(self):
+        nd = self.ndrude
+        J, kJ, rJ = np.empty((3, nd)), np.empty((3, nd)), np.empty((3, nd))
+        _lib.check(self._L.nkc_get_drude(self._h, _d(J), _d(kJ), _d(rJ)))
+        return J, kJ, rJ
 
     def sync(self):
         _lib.check(self._L.nkc_sync(self._h))

@@ -210,3 +210,67 @@ def test_config2_shape_reduced_parity(oracle):
     out = ref.run(H, E, dt, 20)
     assert parity(gH, gE, out["H"], out["E"], mesh.bmn) <= 1e-11
     gpu.close()
+
+
+def test_source_and_drude_residual_parity(oracle):
+    """Gaussian source + Drude region in one residual evaluation (random
+    state and random J)."""
+    rng = np.random.default_rng(21)
+    nel, N = (3, 2, 2), 5
+    mesh, ref, gpu = _pair(oracle, nel, N, bc=("PEC", "P", "PEC"),
+                           eps=np.repeat(rng.uniform(1, 3, 12), 216))
+    npts = mesh.npts
+    x, y, z = mesh.coords()
+    amp = np.exp(-((x - 3) ** 2 + (y - 2) ** 2 + (z - 3) ** 2) / 0.5) * mesh.bmn
+    amp[np.abs(x - 3) > 2] = 0.0          # some elements carry no source
+    dindex = np.nonzero((x > 1.5) & (x < 5) & (z < 4))[0]
+    a = rng.uniform(0.05, 0.2, len(dindex))
+    b = rng.uniform(0.5, 2.0, len(dindex))
+    J = rng.standard_normal((3, len(dindex)))
+    H = rng.standard_normal((3, npts))
+    E = rng.standard_normal((3, npts))
+    ref.set_source(amp, direction=(0.3, -0.2, 1.0), t0=0.4, tw=0.15)
+    ref.set_drude(dindex, a, b)
+    gpu.set_source(amp, direction=(0.3, -0.2, 1.0), t0=0.4, tw=0.15)
+    gpu.set_drude(dindex, a, b, J=J)
+    gpu.set_state(H, E)
+    for t in (0.35, 0.5):
+        rH, rE = gpu.residual(t)
+        oH, oE, oJ = ref.residual(H, E, t=t, J=J.reshape(-1))
+        gJ = gpu.get_drude()[2]
+        for a_, b_ in ((rH, oH), (rE, oE), (gJ, oJ.reshape(3, -1))):
+            for c in range(3):
+                assert np.max(np.abs(a_[c] - b_[c])) <= 1e-11 * np.max(np.abs(b_[c]))
+    gpu.close()
+
+
+def test_source_and_drude_steps_parity(oracle):
+    """40 RK steps with the source switching on and a Drude block: fields and
+    J within 1e-11 relative L2 of the oracle; device time matches."""
+    rng = np.random.default_rng(5)
+    nel, N = (3, 3, 3), 6
+    lo, hi = (0, 0, 0), (1, 1, 1)
+    mesh, ref, gpu = _pair(oracle, nel, N, bc=("PEC",) * 3, lo=lo, hi=hi)
+    x, y, z = mesh.coords()
+    amp = np.exp(-((x - 0.5) ** 2 + (y - 0.5) ** 2 + (z - 0.5) ** 2) / (2 * 0.1 ** 2)) * mesh.bmn
+    inner = (np.abs(x - 0.5) < 0.17) & (np.abs(y - 0.5) < 0.17) & (np.abs(z - 0.5) < 0.17)
+    dindex = np.nonzero(inner)[0]
+    a = np.full(len(dindex), 0.1)
+    b = np.full(len(dindex), 4.0)
+    for side in (ref, gpu):
+        side.set_source(amp, direction=(0, 0, 1), t0=0.05, tw=0.02)
+        side.set_drude(dindex, a, b)
+    H = np.zeros((3, mesh.npts))
+    E = np.zeros((3, mesh.npts))
+    dt = mesh.dt(0.25)
+    gpu.set_state(H, E)
+    gpu.step(dt, 40)
+    gH, gE = gpu.get_state()
+    gJ = gpu.get_drude()[0]
+    out = ref.run(H, E, dt, 40)
+    assert np.max(np.abs(out["E"])) > 1e-3       # the pulse did fire
+    assert parity(gH, gE, out["H"], out["E"], mesh.bmn) <= 1e-11
+    oJ = out["J"].reshape(3, -1)
+    assert np.sqrt(np.sum((gJ - oJ) ** 2)) <= 1e-11 * np.sqrt(np.sum(oJ ** 2))
+    assert gpu.time == out["time"]
+    gpu.close()
