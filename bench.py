@@ -98,6 +98,62 @@ def workload(nel=32, N=7):
     return mesh, eps, H, E
 
 
+def batched(ncav=256, steps=50, warmup=3, seed=3, log=sys.stderr):
+    """Batched-cavity speedup: ncav independent PEC cavities (n^3 elements,
+    n ~ U{2..16}, N ~ U{5..9}, per-element eps ~ U[1,4], Gaussian-pulse Ez,
+    each with its own CFL-0.25 dt) advanced together by one batch vs. one
+    plain handle per cavity run one after another; same steps, same kernels,
+    bit-identical states (checked on every cavity)."""
+    from nekcem_amd import usr
+    from nekcem_amd.maxwell import BatchSolver, MaxwellSolver
+    from nekcem_amd.mesh import box_coords, box_dxmin
+    cavs = usr.cavity_batch(ncav, seed)
+    dts = np.array([0.25 * box_dxmin(c["nel"], c["N"], c["lo"], c["hi"]) for c in cavs])
+    npts = sum(int(np.prod(c["nel"])) * (c["N"] + 1) ** 3 for c in cavs)
+
+    def init(c):
+        cv = cavs[c]
+        x, y, z = box_coords(cv["nel"], cv["N"], cv["lo"], cv["hi"])
+        return usr.gaussian_ez(x, y, z)
+
+    b = BatchSolver(cavs)
+    for c in range(ncav):
+        b.set_state(c, *init(c))
+    b.step(dts, warmup)
+    b.sync()
+    t0 = time.perf_counter()
+    b.step(dts, steps)
+    b.sync()
+    t_batch = time.perf_counter() - t0
+    print(f"[bench] batched: {ncav} cavities, {npts} points, {b.info()['norders']} orders, "
+          f"{steps} steps in {t_batch:.3f} s", file=log, flush=True)
+    t_seq = 0.0
+    identical = True
+    for c in range(ncav):
+        cv = cavs[c]
+        s = MaxwellSolver.from_box(cv["nel"], cv["N"], cv["lo"], cv["hi"], cv["bc"], eps_elem=cv["eps_elem"])
+        s.set_state(*init(c))
+        s.step(dts[c], warmup)
+        s.sync()
+        t0 = time.perf_counter()
+        s.step(dts[c], steps)
+        s.sync()
+        t_seq += time.perf_counter() - t0
+        gb = b.get_state(c)
+        gs = s.get_state()
+        identical &= all(np.array_equal(x, y) for x, y in zip(gb, gs))
+        s.close()
+    b.close()
+    print(f"[bench] sequential: {t_seq:.3f} s, speedup {t_seq / t_batch:.2f}", file=log, flush=True)
+    return {"metric": "batched speedup", "value": t_seq / t_batch, "unit": "x",
+            "t_batched_s": t_batch, "t_sequential_s": t_seq, "cavities": ncav, "npts": npts,
+            "steps": steps, "batched_G_per_s": npts * 30 * steps / t_batch / 1e9,
+            "sequential_G_per_s": npts * 30 * steps / t_seq / 1e9,
+            "bitwise_identical_to_sequential": bool(identical),
+            "workload": f"{ncav} PEC cavities on [0,1]^3, n^3 elements n~U{{2..16}}, N~U{{5..9}}, "
+                        f"per-element eps U[1,4], Gaussian Ez (width 0.2), own CFL-0.25 dt, seed {seed}"}
+
+
 def load_traffic(npts):
     """HBM bytes per stage launch from the committed rocprofv3 --pmc summary."""
     p = os.path.join(ROOT, "profiles", "stage_kernel_box32_N7_traffic.json")
@@ -119,11 +175,17 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=1,
                     help="steps of the CPU oracle for cpu_baseline (0 = skip)")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--batched-steps", type=int, default=50,
+                    help="steps of the batched-cavity speedup measurement (0 = skip)")
+    ap.add_argument("--batched-only", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (counter calibration)")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.batched_only:
+        print(json.dumps(batched(steps=args.batched_steps)), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -195,6 +257,8 @@ def main():
         parity = {"rel_l2_max_over_fields": worst, "steps": args.cpu_steps,
                   "norm": "per field, sqrt(sum bm1 (gpu-cpu)^2)/sqrt(sum bm1 cpu^2)"}
 
+    bat = batched(steps=args.batched_steps) if args.batched_steps > 0 else None
+
     line = {
         "metric": "G(GLL-pt*field*stage)/s FP64, 32^3el N=7",
         "value": value,
@@ -220,6 +284,7 @@ def main():
                      "alg_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
         "parity": parity,
+        "batched": bat,
     }
     solver.close()
     print(json.dumps(line), flush=True)

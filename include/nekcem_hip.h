@@ -63,11 +63,30 @@ typedef struct nkc_desc {
                                    cem_maxwell_materials (:262-325) */
 } nkc_desc;
 
+/* A box of nel[0] x nel[1] x nel[2] straight-sided hexahedra on
+ * [lo, hi] (element e = ex + nel0*(ey + nel1*ez)), periodic or PEC per axis,
+ * with per-element permittivity/permeability: the .box description genbox2
+ * reads (src/nek5_genbox.F:2-145), for callers that have no reference arrays.
+ * It produces the same tables nkc_create derives from the arrays of the same
+ * mesh. */
+typedef struct nkc_box_desc {
+    int32_t N;                  /* polynomial order */
+    int32_t ifupwind;           /* 1 upwind, 0 central */
+    int32_t nel[3];             /* elements per axis */
+    int32_t pec[3];             /* per axis: 0 periodic, 1 PEC walls */
+    double lo[3], hi[3];        /* box extent */
+    const double *eps_elem;     /* nelt permittivities, NULL = 1 */
+    const double *mu_elem;      /* nelt permeabilities, NULL = 1 */
+} nkc_box_desc;
+
 /* Upload the working set and build the on-device face-exchange tables.
  * Mirror of acc_copy_all_in (src/cem_drive.F:397) plus the setup half of
  * cem_maxwell_init (src/cem_maxwell.F:65-191).  Returns NKC_ERR_UNSUPPORTED
  * for geometry other than axis-aligned affine hexahedra (box meshes). */
 int32_t nkc_create(const nkc_desc *d, nkc_handle **h);
+
+/* Same handle from a box description (no reference arrays needed). */
+int32_t nkc_create_box(const nkc_box_desc *d, nkc_handle **h);
 
 /* Mirror of acc_copy_all_out (src/cem_drive.F:482). */
 int32_t nkc_destroy(nkc_handle *h);
@@ -133,6 +152,30 @@ int32_t nkc_info(nkc_handle *h, int64_t out[8]);
 
 /* Enable (1, default) or disable (0) hipGraph replay in nkc_step. */
 int32_t nkc_set_graph(nkc_handle *h, int32_t enable);
+
+/* ---- batches of independent cavities --------------------------------
+ * Many independent box cavities of different sizes and polynomial orders
+ * advance together: cavities of one order share one element table, each
+ * order's RK stage is one launch, the orders run side by side, and a whole
+ * step is one graph replay.  The reference's only way to run independent
+ * cases at once is one MPI session per case (neknek, src/nek5_comm_mpi.F:
+ * 45-119); no entry point there corresponds to these. */
+typedef struct nkc_batch nkc_batch;
+int32_t nkc_batch_create(int32_t ncav, const nkc_box_desc *d, nkc_batch **b);
+int32_t nkc_batch_destroy(nkc_batch *b);
+/* State of cavity cav (0-based): hn/en 3 components at stride ld; k := 0. */
+int32_t nkc_batch_set_state(nkc_batch *b, int32_t cav, const double *hn,
+                            const double *en, int64_t ld);
+int32_t nkc_batch_get_state(nkc_batch *b, int32_t cav, double *hn, double *en,
+                            int64_t ld);
+/* nsteps RK steps of every cavity, cavity c with its own dt[c]; times[c]
+ * (may be NULL) receives each cavity's time afterwards. */
+int32_t nkc_batch_step(nkc_batch *b, const double *dt, double *times, int64_t nsteps);
+int32_t nkc_batch_sync(nkc_batch *b);
+int32_t nkc_batch_stream(nkc_batch *b, void **stream);
+/* out[0] = total points, out[1] = cavities, out[2] = polynomial orders
+ * (launches per stage), out[3] = total elements. */
+int32_t nkc_batch_info(nkc_batch *b, int64_t out[8]);
 
 /* Message of the last failed call on this thread ("" if none). */
 const char *nkc_last_error(void);

@@ -43,6 +43,16 @@ class NkcDesc(C.Structure):
     ]
 
 
+class NkcBoxDesc(C.Structure):
+    """Mirror of struct nkc_box_desc."""
+    _fields_ = [
+        ("N", C.c_int32), ("ifupwind", C.c_int32),
+        ("nel", C.c_int32 * 3), ("pec", C.c_int32 * 3),
+        ("lo", C.c_double * 3), ("hi", C.c_double * 3),
+        ("eps_elem", _dp), ("mu_elem", _dp),
+    ]
+
+
 class NkcError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"nekcem_hip error {code}: {msg}")
@@ -54,6 +64,7 @@ _lib = None
 # (name, restype, argtypes) for every function in include/nekcem_hip.h
 SIGNATURES = [
     ("nkc_create", C.c_int32, [C.POINTER(NkcDesc), C.POINTER(C.c_void_p)]),
+    ("nkc_create_box", C.c_int32, [C.POINTER(NkcBoxDesc), C.POINTER(C.c_void_p)]),
     ("nkc_destroy", C.c_int32, [C.c_void_p]),
     ("nkc_set_state", C.c_int32, [C.c_void_p, _dp, _dp, _dp, _dp, C.c_int64]),
     ("nkc_get_state", C.c_int32, [C.c_void_p, _dp, _dp, _dp, _dp, C.c_int64]),
@@ -67,6 +78,14 @@ SIGNATURES = [
     ("nkc_stream", C.c_int32, [C.c_void_p, C.POINTER(C.c_void_p)]),
     ("nkc_info", C.c_int32, [C.c_void_p, C.POINTER(C.c_int64)]),
     ("nkc_set_graph", C.c_int32, [C.c_void_p, C.c_int32]),
+    ("nkc_batch_create", C.c_int32, [C.c_int32, C.POINTER(NkcBoxDesc), C.POINTER(C.c_void_p)]),
+    ("nkc_batch_destroy", C.c_int32, [C.c_void_p]),
+    ("nkc_batch_set_state", C.c_int32, [C.c_void_p, C.c_int32, _dp, _dp, C.c_int64]),
+    ("nkc_batch_get_state", C.c_int32, [C.c_void_p, C.c_int32, _dp, _dp, C.c_int64]),
+    ("nkc_batch_step", C.c_int32, [C.c_void_p, _dp, _dp, C.c_int64]),
+    ("nkc_batch_sync", C.c_int32, [C.c_void_p]),
+    ("nkc_batch_stream", C.c_int32, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    ("nkc_batch_info", C.c_int32, [C.c_void_p, C.POINTER(C.c_int64)]),
     ("nkc_last_error", C.c_char_p, []),
 ]
 

@@ -1,0 +1,362 @@
+// mesh_tables.hpp -- host-side compact tables the stage kernel reads.
+//
+// Two producers fill the same HostMesh:
+//   compact_reference()  from the reference-layout arrays of nkc_desc (what
+//                        NekCEM's cem_maxwell_init leaves in COMMON), checking
+//                        that the mesh is made of axis-aligned affine
+//                        hexahedra paired face to face;
+//   build_box()          directly from a box description (nkc_box_desc), the
+//                        genbox2 semantics of src/nek5_genbox.F:2-145 without
+//                        materialising per-node arrays.
+// Both give, per element: g = (rx, sy, tz)/J, the six lift factors
+// area/(J w3), the six neighbours (or PEC / unpaired), the upwind/central
+// flux coefficients (Y02, C02Y, Z02, C02Z) of cem_maxwell_flux3d
+// (src/cem_maxwell.F:970-975) from the impedance averages of
+// cem_maxwell_materials (src/cem_maxwell.F:262-325), and 1/eps, 1/mu.
+#pragma once
+#include <cmath>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/nekcem_hip.h"
+
+namespace nkc {
+
+// FaceType codes live in stage_kernel.hpp: -1 PEC, -2 no partner.
+constexpr int32_t HOST_FACE_PEC = -1;
+constexpr int32_t HOST_FACE_NONE = -2;
+
+struct HostMesh {
+    int n1 = 0;
+    int64_t nelt = 0;
+    int mat = 0;                     // 0 per element, 1 per node
+    std::vector<double> D;           // n1*n1, D(i,l) at [i + n1 l]
+    std::vector<double> w;           // GLL weights
+    std::vector<double> geo;         // nelt*9
+    std::vector<int32_t> nbr;        // nelt*6 (kernel face order)
+    std::vector<double> fco;         // nelt*6*4 or nelt*6*n2*4
+    std::vector<double> ieps, imu;   // nelt or npts
+    std::vector<double> jac;         // nelt (element Jacobian)
+    std::string err;
+};
+
+// Ed's face (1..6) of kernel face kf (x-, x+, y-, y+, z-, z+).
+inline const int *ed_of_kf() { static const int t[6] = {4, 2, 1, 3, 5, 6}; return t; }
+inline const int *kf_of_ed() { static const int t[7] = {-1, 2, 1, 3, 0, 4, 5}; return t; }
+
+// Volume node (in-element index) of in-face node q = a + n1*b on kernel face
+// kf, in the skpdat order of src/nek5_connect11.F:1483-1528.
+inline int64_t face_node(int n1, int kf, int q) {
+    int a = q % n1, b = q / n1, m = n1 - 1, n2 = n1 * n1;
+    switch (kf) {
+    case 0: return 0 + (int64_t)n1 * a + (int64_t)n2 * b;
+    case 1: return m + (int64_t)n1 * a + (int64_t)n2 * b;
+    case 2: return a + 0 + (int64_t)n2 * b;
+    case 3: return a + (int64_t)n1 * m + (int64_t)n2 * b;
+    case 4: return a + (int64_t)n1 * b + 0;
+    default: return a + (int64_t)n1 * b + (int64_t)n2 * m;
+    }
+}
+
+inline bool near(double x, double y, double scale, double tol) { return std::fabs(x - y) <= tol * scale; }
+
+inline void flux_coef(double Y0, double Y1, double Z0, double Z1, double C0, double *c) {
+    c[0] = -0.5 / Y0 * Y1;
+    c[1] = 0.5 / Y0 * C0;
+    c[2] = 0.5 / Z0 * Z1;
+    c[3] = 0.5 / Z0 * C0;
+}
+
+// ---------------------------------------------------------------------------
+// from the reference arrays
+// ---------------------------------------------------------------------------
+inline int32_t compact_reference(const nkc_desc *d, HostMesh &m) {
+    const int n1 = d->N + 1, n2 = n1 * n1, n3 = n2 * n1;
+    const int64_t nelt = d->nelt, npts = (int64_t)n3 * nelt, nxzfl = (int64_t)6 * n2 * nelt;
+    const double C0 = d->ifupwind ? 1.0 : 0.0;
+    const double tol = 1e-8;
+    char buf[320];
+    auto bad = [&](int32_t code, const char *msg) { m.err = msg; return code; };
+
+    std::vector<int8_t> pecslot(nxzfl, 0);
+    for (int64_t l = 0; l < d->ncempec; ++l) {
+        int64_t s = (int64_t)d->cempec[l] - 1;
+        if (s < 0 || s >= nxzfl) {
+            snprintf(buf, sizeof buf, "cempec[%lld]=%d out of range", (long long)l, d->cempec[l]);
+            return bad(NKC_ERR_ARG, buf);
+        }
+        pecslot[s] = 1;
+    }
+    bool elem_mat = true;
+    for (int64_t e = 0; e < nelt && elem_mat; ++e) {
+        const int64_t b = e * n3;
+        for (int q = 1; q < n3; ++q)
+            if (d->permittivity[b + q] != d->permittivity[b] || d->permeability[b + q] != d->permeability[b]) {
+                elem_mat = false;
+                break;
+            }
+        for (int f = 0; f < 6 && elem_mat; ++f) {
+            const int64_t s0 = e * 6 * n2 + (int64_t)f * n2;
+            for (int q = 1; q < n2; ++q)
+                if (d->Y_0[s0 + q] != d->Y_0[s0] || d->Y_1[s0 + q] != d->Y_1[s0] ||
+                    d->Z_0[s0 + q] != d->Z_0[s0] || d->Z_1[s0 + q] != d->Z_1[s0]) {
+                    elem_mat = false;
+                    break;
+                }
+        }
+    }
+    m.n1 = n1;
+    m.nelt = nelt;
+    m.mat = elem_mat ? 0 : 1;
+    m.D.assign(d->dxm1, d->dxm1 + n2);
+    m.w.assign(d->wxm1, d->wxm1 + n1);
+    m.geo.assign(nelt * 9, 0.0);
+    m.nbr.assign(nelt * 6, 0);
+    m.fco.assign(elem_mat ? nelt * 6 * 4 : nelt * 6 * n2 * 4, 0.0);
+    m.ieps.assign(elem_mat ? nelt : npts, 0.0);
+    m.imu.assign(elem_mat ? nelt : npts, 0.0);
+    m.jac.assign(nelt, 0.0);
+
+    for (int64_t e = 0; e < nelt; ++e) {
+        const int64_t b = e * n3;
+        const double J0 = d->jacm[b], rx0 = d->rxmn[b], sy0 = d->symn[b], tz0 = d->tzmn[b];
+        const double sc = std::fmax(std::fabs(rx0), std::fmax(std::fabs(sy0), std::fabs(tz0)));
+        if (!(J0 > 0.0)) {
+            snprintf(buf, sizeof buf, "element %lld: Jacobian %g not positive", (long long)e, J0);
+            return bad(NKC_ERR_ARG, buf);
+        }
+        for (int q = 0; q < n3; ++q) {
+            const int64_t p = b + q;
+            if (!near(d->jacm[p], J0, std::fabs(J0), tol) || !near(d->rxmn[p], rx0, sc, tol) ||
+                !near(d->symn[p], sy0, sc, tol) || !near(d->tzmn[p], tz0, sc, tol) ||
+                std::fabs(d->rymn[p]) > tol * sc || std::fabs(d->rzmn[p]) > tol * sc ||
+                std::fabs(d->sxmn[p]) > tol * sc || std::fabs(d->szmn[p]) > tol * sc ||
+                std::fabs(d->txmn[p]) > tol * sc || std::fabs(d->tymn[p]) > tol * sc) {
+                snprintf(buf, sizeof buf,
+                         "element %lld is not an axis-aligned affine hexahedron; only box meshes are built",
+                         (long long)e);
+                return bad(NKC_ERR_UNSUPPORTED, buf);
+            }
+        }
+        m.jac[e] = J0;
+        m.geo[e * 9 + 0] = rx0 / J0;
+        m.geo[e * 9 + 1] = sy0 / J0;
+        m.geo[e * 9 + 2] = tz0 / J0;
+        for (int kf = 0; kf < 6; ++kf) {
+            const int ef = ed_of_kf()[kf];
+            const int ax = kf / 2;
+            const double sg = (kf & 1) ? 1.0 : -1.0;
+            const int64_t s0 = e * 6 * n2 + (int64_t)(ef - 1) * n2;
+            double L0 = 0.0;
+            for (int q = 0; q < n2; ++q) {
+                const int64_t s = s0 + q;
+                const int64_t v = (int64_t)d->cemface[s] - 1;
+                if (v != b + face_node(n1, kf, q)) {
+                    snprintf(buf, sizeof buf, "cemface[%lld]=%d does not follow the skpdat face order",
+                             (long long)s, d->cemface[s]);
+                    return bad(NKC_ERR_ARG, buf);
+                }
+                const double nv[3] = {d->unxm[s], d->unym[s], d->unzm[s]};
+                for (int c = 0; c < 3; ++c)
+                    if (std::fabs(nv[c] - (c == ax ? sg : 0.0)) > tol) {
+                        snprintf(buf, sizeof buf, "element %lld face %d: normal is not an axis direction",
+                                 (long long)e, ef);
+                        return bad(NKC_ERR_UNSUPPORTED, buf);
+                    }
+                const double L = d->aream[s] / d->bmn[v];
+                if (q == 0) L0 = L;
+                else if (!near(L, L0, std::fabs(L0), tol)) {
+                    snprintf(buf, sizeof buf, "element %lld face %d: lift factor varies over the face",
+                             (long long)e, ef);
+                    return bad(NKC_ERR_UNSUPPORTED, buf);
+                }
+            }
+            m.geo[e * 9 + 3 + kf] = L0;
+            const int64_t p0 = (int64_t)d->face_partner[s0] - 1;
+            int32_t nb;
+            if (p0 < 0) {
+                nb = pecslot[s0] ? HOST_FACE_PEC : HOST_FACE_NONE;
+                for (int q = 0; q < n2; ++q)
+                    if (d->face_partner[s0 + q] != 0 || pecslot[s0 + q] != pecslot[s0]) {
+                        snprintf(buf, sizeof buf, "element %lld face %d: mixed boundary face", (long long)e, ef);
+                        return bad(NKC_ERR_UNSUPPORTED, buf);
+                    }
+            } else {
+                if (p0 >= nxzfl) return bad(NKC_ERR_ARG, "face_partner out of range");
+                const int64_t ne = p0 / (6 * n2);
+                const int nef = (int)((p0 / n2) % 6) + 1;
+                if (kf_of_ed()[nef] != (kf ^ 1)) {
+                    snprintf(buf, sizeof buf,
+                             "element %lld face %d pairs with face %d of element %lld; only opposite-face "
+                             "box pairing is built", (long long)e, ef, nef, (long long)ne);
+                    return bad(NKC_ERR_UNSUPPORTED, buf);
+                }
+                const int64_t ns0 = ne * 6 * n2 + (int64_t)(nef - 1) * n2;
+                for (int q = 0; q < n2; ++q)
+                    if ((int64_t)d->face_partner[s0 + q] - 1 != ns0 + q || pecslot[s0 + q]) {
+                        snprintf(buf, sizeof buf, "element %lld face %d: rotated face pairing not built",
+                                 (long long)e, ef);
+                        return bad(NKC_ERR_UNSUPPORTED, buf);
+                    }
+                nb = (int32_t)ne;
+            }
+            m.nbr[e * 6 + kf] = nb;
+            const int nq = elem_mat ? 1 : n2;
+            for (int q = 0; q < nq; ++q) {
+                const int64_t s = s0 + q;
+                double *c = elem_mat ? &m.fco[(e * 6 + kf) * 4] : &m.fco[((e * 6 + kf) * n2 + q) * 4];
+                flux_coef(d->Y_0[s], d->Y_1[s], d->Z_0[s], d->Z_1[s], C0, c);
+            }
+        }
+        if (elem_mat) {
+            m.ieps[e] = 1.0 / d->permittivity[b];
+            m.imu[e] = 1.0 / d->permeability[b];
+        }
+    }
+    if (!elem_mat)
+        for (int64_t p = 0; p < npts; ++p) {
+            m.ieps[p] = 1.0 / d->permittivity[p];
+            m.imu[p] = 1.0 / d->permeability[p];
+        }
+    return NKC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// GLL basis (ZWGLL / DGLL, src/nek5_speclib.F:107-121, 807-838): interior
+// nodes are the roots of P_N' (Newton with deflation), w = 2/(N(N+1)P_N^2),
+// D_ij = P_N(z_i)/(P_N(z_j)(z_i - z_j)), D_00 = -N(N+1)/4, D_NN = N(N+1)/4.
+// ---------------------------------------------------------------------------
+inline void legendre(double x, int n, double &p, double &dp) {
+    double p0 = 1.0, p1 = x, d0 = 0.0, d1 = 1.0;
+    if (n == 0) { p = 1.0; dp = 0.0; return; }
+    for (int k = 1; k < n; ++k) {
+        const double p2 = ((2.0 * k + 1.0) * x * p1 - k * p0) / (k + 1.0);
+        const double d2 = d0 + (2.0 * k + 1.0) * p1;
+        p0 = p1; p1 = p2; d0 = d1; d1 = d2;
+    }
+    p = p1;
+    dp = d1;
+}
+
+inline void gll_basis(int n1, std::vector<double> &z, std::vector<double> &w, std::vector<double> &D) {
+    const int N = n1 - 1;
+    z.assign(n1, 0.0);
+    w.assign(n1, 0.0);
+    D.assign(n1 * n1, 0.0);
+    z[0] = -1.0;
+    z[N] = 1.0;
+    for (int i = 1; i < N; ++i) {
+        double x = -std::cos(M_PI * i / N);
+        for (int it = 0; it < 100; ++it) {
+            double p, dp;
+            legendre(x, N, p, dp);
+            const double d2p = (2.0 * x * dp - N * (N + 1.0) * p) / (1.0 - x * x);
+            double defl = 0.0;
+            for (int j = 1; j < i; ++j) defl += 1.0 / (x - z[j]);
+            const double dx = dp / (d2p - dp * defl);
+            x -= dx;
+            if (std::fabs(dx) < 1e-17) break;
+        }
+        z[i] = x;
+    }
+    for (int i = 0; i < n1 / 2; ++i) {
+        const double a = 0.5 * (z[N - i] - z[i]);
+        z[i] = -a;
+        z[N - i] = a;
+    }
+    if (n1 % 2) z[N / 2] = 0.0;
+    std::vector<double> P(n1);
+    for (int i = 0; i < n1; ++i) {
+        double dp;
+        legendre(z[i], N, P[i], dp);
+        w[i] = 2.0 / (N * (N + 1.0) * P[i] * P[i]);
+    }
+    for (int i = 0; i < n1; ++i)
+        for (int j = 0; j < n1; ++j)
+            if (i != j) D[i + n1 * j] = P[i] / (P[j] * (z[i] - z[j]));
+    D[0] = -N * (N + 1.0) / 4.0;
+    D[N + n1 * N] = N * (N + 1.0) / 4.0;
+}
+
+// ---------------------------------------------------------------------------
+// from a box description
+// ---------------------------------------------------------------------------
+inline int32_t build_box(const nkc_box_desc *d, HostMesh &m) {
+    const int n1 = d->N + 1;
+    const int64_t nel[3] = {d->nel[0], d->nel[1], d->nel[2]};
+    if (nel[0] <= 0 || nel[1] <= 0 || nel[2] <= 0) { m.err = "box element counts must be positive"; return NKC_ERR_ARG; }
+    for (int a = 0; a < 3; ++a)
+        if (!(d->hi[a] > d->lo[a])) { m.err = "box extent must be positive"; return NKC_ERR_ARG; }
+    const int64_t nelt = nel[0] * nel[1] * nel[2];
+    const double C0 = d->ifupwind ? 1.0 : 0.0;
+    std::vector<double> z;
+    gll_basis(n1, z, m.w, m.D);
+    m.n1 = n1;
+    m.nelt = nelt;
+    m.mat = 0;
+    m.geo.assign(nelt * 9, 0.0);
+    m.nbr.assign(nelt * 6, 0);
+    m.fco.assign(nelt * 6 * 4, 0.0);
+    m.ieps.assign(nelt, 0.0);
+    m.imu.assign(nelt, 0.0);
+    m.jac.assign(nelt, 0.0);
+    const double h[3] = {(d->hi[0] - d->lo[0]) / nel[0], (d->hi[1] - d->lo[1]) / nel[1],
+                         (d->hi[2] - d->lo[2]) / nel[2]};
+    // cofactors of the affine map (GLMAPM1): rx = y_s z_t, sy = x_r z_t, tz = x_r y_s
+    const double xr = 0.5 * h[0], ys = 0.5 * h[1], zt = 0.5 * h[2];
+    const double J = xr * ys * zt;
+    const double rx = ys * zt, sy = xr * zt, tz = xr * ys;
+    const double w0 = m.w[0];
+    // area/(J w3) at a face node: face Jacobian / (J w_end)  (AREA3 + GEODAT1)
+    const double Lx = rx / (J * w0), Ly = sy / (J * w0), Lz = tz / (J * w0);
+    auto eps_of = [&](int64_t e) { return d->eps_elem ? d->eps_elem[e] : 1.0; };
+    auto mu_of = [&](int64_t e) { return d->mu_elem ? d->mu_elem[e] : 1.0; };
+    for (int64_t e = 0; e < nelt; ++e) {
+        const double ee = eps_of(e), mm = mu_of(e);
+        if (!(ee > 0.0) || !(mm > 0.0)) { m.err = "permittivity and permeability must be positive"; return NKC_ERR_ARG; }
+    }
+    for (int64_t e = 0; e < nelt; ++e) {
+        const int64_t ex[3] = {e % nel[0], (e / nel[0]) % nel[1], e / (nel[0] * nel[1])};
+        m.jac[e] = J;
+        double *g = &m.geo[e * 9];
+        g[0] = rx / J;
+        g[1] = sy / J;
+        g[2] = tz / J;
+        g[3] = g[4] = Lx;
+        g[5] = g[6] = Ly;
+        g[7] = g[8] = Lz;
+        const double Zm = std::sqrt(mu_of(e) / eps_of(e)), Ym = std::sqrt(eps_of(e) / mu_of(e));
+        m.ieps[e] = 1.0 / eps_of(e);
+        m.imu[e] = 1.0 / mu_of(e);
+        for (int kf = 0; kf < 6; ++kf) {
+            const int ax = kf / 2, side = kf & 1;
+            int64_t c[3] = {ex[0], ex[1], ex[2]};
+            c[ax] += side ? 1 : -1;
+            int32_t nb;
+            double Y0, Z0;   // gs sums, as cem_maxwell_materials forms them
+            if (c[ax] < 0 || c[ax] >= nel[ax]) {
+                if (d->pec[ax]) {
+                    nb = HOST_FACE_PEC;
+                    Y0 = 2.0 * Ym;
+                    Z0 = 2.0 * Zm;
+                } else {
+                    c[ax] = (c[ax] + nel[ax]) % nel[ax];
+                    nb = (int32_t)(c[0] + nel[0] * (c[1] + nel[1] * c[2]));
+                    Y0 = Ym + std::sqrt(eps_of(nb) / mu_of(nb));
+                    Z0 = Zm + std::sqrt(mu_of(nb) / eps_of(nb));
+                }
+            } else {
+                nb = (int32_t)(c[0] + nel[0] * (c[1] + nel[1] * c[2]));
+                Y0 = Ym + std::sqrt(eps_of(nb) / mu_of(nb));
+                Z0 = Zm + std::sqrt(mu_of(nb) / eps_of(nb));
+            }
+            m.nbr[e * 6 + kf] = nb;
+            const double Y1 = Y0 - Ym, Z1 = Z0 - Zm;
+            flux_coef(0.5 * Y0, Y1, 0.5 * Z0, Z1, C0, &m.fco[(e * 6 + kf) * 4]);
+        }
+    }
+    return NKC_OK;
+}
+
+}  // namespace nkc

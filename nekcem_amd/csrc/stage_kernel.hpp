@@ -79,6 +79,7 @@ struct StageArgs {
     int32_t nelt;
     int32_t per_xcd;                  // elements per XCD slice (grid remap)
     double rka, rkb, dt;
+    const double *__restrict__ dt_e;  // per-element dt (batched cavities) or null
     // stage time = *tdev + rkc*dt (tdev advanced on the device after stage 5)
     const double *__restrict__ tdev;
     double rkc;
@@ -296,6 +297,7 @@ stage_kernel(StageArgs a) {
     for (int l = 0; l < N1; ++l) { Dr[l] = s_D[i + N1 * l]; Ds[l] = s_D[j + N1 * l]; }
     double minv = 0.0;
     if (MAT == MAT_ELEM) minv = grp ? a.ieps[e] : a.imu[e];
+    const double dt = a.dt_e ? a.dt_e[e] : a.dt;
     double cfz[4];
     load_coef<N1, MAT>(a, e, zface, q, cfz);
 #pragma unroll
@@ -372,7 +374,7 @@ stage_kernel(StageArgs a) {
                     if (MODE == MODE_RESIDUAL) {
                         a.dresJ[jb + c * N3] = rJ;
                     } else {
-                        const double kj = a.rka * a.dkJ[jb + c * N3] + a.dt * rJ;
+                        const double kj = a.rka * a.dkJ[jb + c * N3] + dt * rJ;
                         a.dkJ[jb + c * N3] = kj;
                         a.dJ[jb + c * N3] = Jc + a.rkb * kj;
                     }
@@ -391,7 +393,7 @@ stage_kernel(StageArgs a) {
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 const int64_t p = (co + c) * fs + node;
-                const double kk2 = a.rka * kp[c][kk] + a.dt * r[c];
+                const double kk2 = a.rka * kp[c][kk] + dt * r[c];
                 st_stream(a.kbuf + p, kk2);
                 st_stream(a.uout + p, s_u[(co + c) * N3 + own] + a.rkb * kk2);
             }

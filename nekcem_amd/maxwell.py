@@ -45,8 +45,67 @@ class RunStopped(RuntimeError):
     """Raised when userchk asks to stop (the reference calls exitt(1))."""
 
 
+def box_desc(nel, N, lo=(0.0, 0.0, 0.0), hi=(2 * np.pi,) * 3, bc=("P", "P", "P"),
+             eps_elem=None, mu_elem=None, ifupwind=True):
+    """nkc_box_desc for a box; the per-element arrays are kept alive on the
+    returned structure."""
+    d = _lib.NkcBoxDesc()
+    d.N, d.ifupwind = int(N), 1 if ifupwind else 0
+    d.nel = (C.c_int32 * 3)(*[int(x) for x in nel])
+    d.pec = (C.c_int32 * 3)(*[1 if b == "PEC" else 0 for b in bc])
+    d.lo = (C.c_double * 3)(*[float(x) for x in lo])
+    d.hi = (C.c_double * 3)(*[float(x) for x in hi])
+    keep = []
+    for name, arr in (("eps_elem", eps_elem), ("mu_elem", mu_elem)):
+        if arr is not None:
+            arr = np.ascontiguousarray(arr, dtype=np.float64).reshape(-1)
+            assert len(arr) == int(np.prod(nel))
+            keep.append(arr)
+            setattr(d, name, _d(arr))
+    d._keep = keep
+    return d
+
+
+class _BoxMeshInfo:
+    """What a box-built solver knows of its mesh (no per-node arrays)."""
+
+    def __init__(self, nel, N, lo, hi, bc):
+        from .mesh import box_coords, box_dxmin, gll
+        self.nel, self.N, self.lo, self.hi, self.bc = tuple(nel), int(N), tuple(lo), tuple(hi), tuple(bc)
+        self.n1 = self.N + 1
+        self.nelt = int(np.prod(nel))
+        self.npts = self.nelt * self.n1 ** 3
+        self._coords, self._dxmin = box_coords, box_dxmin
+
+    def coords(self):
+        return self._coords(self.nel, self.N, self.lo, self.hi)
+
+    def dxmin(self):
+        return self._dxmin(self.nel, self.N, self.lo, self.hi)
+
+    def dt(self, p12):
+        return abs(p12) if p12 < 0 else p12 * self.dxmin()
+
+
 class MaxwellSolver:
     """One nkc handle: the device-resident Maxwell state of a mesh."""
+
+    @classmethod
+    def from_box(cls, nel, N, lo=(0.0, 0.0, 0.0), hi=(2 * np.pi,) * 3, bc=("P", "P", "P"),
+                 eps_elem=None, mu_elem=None, ifupwind=True, graph=True):
+        """Handle built by nkc_create_box (no per-node reference arrays)."""
+        L = _lib.load()
+        d = box_desc(nel, N, lo, hi, bc, eps_elem, mu_elem, ifupwind)
+        h = C.c_void_p()
+        _lib.check(L.nkc_create_box(C.byref(d), C.byref(h)))
+        self = cls.__new__(cls)
+        self.mesh = _BoxMeshInfo(nel, N, lo, hi, bc)
+        self.npts = self.mesh.npts
+        self._h, self._L = h, L
+        self.time, self.istep, self.dt, self.ndrude = 0.0, 0, None, 0
+        if not graph:
+            _lib.check(L.nkc_set_graph(h, 0))
+        return self
 
     def __init__(self, mesh: BoxMesh, eps=None, mu=None, ifupwind=True, graph=True):
         L = _lib.load()
@@ -215,6 +274,75 @@ class MaxwellSolver:
                 done += n
             if userchk(self) is False:
                 raise RunStopped(f"userchk stopped the run at istep={self.istep}")
+
+
+class BatchSolver:
+    """Many independent box cavities advanced together (nkc_batch_*).
+
+    cavities: list of dicts with keys nel, N, lo, hi, bc, eps_elem, mu_elem.
+    """
+
+    def __init__(self, cavities, ifupwind=True):
+        L = _lib.load()
+        self._L = L
+        n = len(cavities)
+        arr = (_lib.NkcBoxDesc * n)()
+        self._keep = []
+        self.info_cav = []
+        for c, cv in enumerate(cavities):
+            d = box_desc(cv["nel"], cv["N"], cv.get("lo", (0, 0, 0)), cv.get("hi", (1, 1, 1)),
+                         cv.get("bc", ("PEC",) * 3), cv.get("eps_elem"), cv.get("mu_elem"), ifupwind)
+            self._keep.append(d._keep)
+            arr[c] = d
+            self.info_cav.append(_BoxMeshInfo(cv["nel"], cv["N"], cv.get("lo", (0, 0, 0)),
+                                              cv.get("hi", (1, 1, 1)), cv.get("bc", ("PEC",) * 3)))
+        b = C.c_void_p()
+        _lib.check(L.nkc_batch_create(n, arr, C.byref(b)))
+        self._b = b
+        self.ncav = n
+        self.times = np.zeros(n)
+
+    def set_state(self, cav, H, E):
+        n = self.info_cav[cav].npts
+        H = np.ascontiguousarray(H, dtype=np.float64).reshape(3, n)
+        E = np.ascontiguousarray(E, dtype=np.float64).reshape(3, n)
+        _lib.check(self._L.nkc_batch_set_state(self._b, int(cav), _d(H), _d(E), n))
+
+    def get_state(self, cav):
+        n = self.info_cav[cav].npts
+        H, E = np.empty((3, n)), np.empty((3, n))
+        _lib.check(self._L.nkc_batch_get_state(self._b, int(cav), _d(H), _d(E), n))
+        return H, E
+
+    def step(self, dt, nsteps=1):
+        dt = np.ascontiguousarray(np.broadcast_to(dt, (self.ncav,)), dtype=np.float64)
+        t = np.empty(self.ncav)
+        _lib.check(self._L.nkc_batch_step(self._b, _d(dt), _d(t), int(nsteps)))
+        self.times = t
+
+    def sync(self):
+        _lib.check(self._L.nkc_batch_sync(self._b))
+
+    def stream(self):
+        s = C.c_void_p()
+        _lib.check(self._L.nkc_batch_stream(self._b, C.byref(s)))
+        return s.value
+
+    def info(self):
+        out = (C.c_int64 * 8)()
+        _lib.check(self._L.nkc_batch_info(self._b, out))
+        return dict(npts=out[0], ncav=out[1], norders=out[2], nelt=out[3])
+
+    def close(self):
+        if self._b:
+            _lib.check(self._L.nkc_batch_destroy(self._b))
+            self._b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def cem_maxwell_init(mesh: BoxMesh, eps=None, mu=None, ifupwind=True, graph=True):

@@ -85,6 +85,38 @@ def skp_face_nodes(n1):
     return out
 
 
+def box_lines(nel, N, lo, hi):
+    """Per-axis GLL coordinates of a box: three (nel_ax, n1) arrays."""
+    z, _ = gll(N + 1)
+    out = []
+    for ax in range(3):
+        ea = np.arange(nel[ax])
+        v0 = lo[ax] + (hi[ax] - lo[ax]) * ea / nel[ax]
+        v1 = lo[ax] + (hi[ax] - lo[ax]) * (ea + 1) / nel[ax]
+        out.append(v0[:, None] * (0.5 * (1 - z))[None, :] + v1[:, None] * (0.5 * (1 + z))[None, :])
+    return out
+
+
+def box_coords(nel, N, lo=(0.0, 0.0, 0.0), hi=(2 * np.pi,) * 3):
+    """xm1, ym1, zm1 (flat, reference volume order) of a box without
+    building the other reference arrays."""
+    n1 = N + 1
+    nelx, nely, nelz = nel
+    lx, ly, lz = box_lines(nel, N, lo, hi)
+    # (ez, ey, ex, k, j, i) -> element-major, i fastest
+    X = np.broadcast_to(lx[None, None, :, None, None, :], (nelz, nely, nelx, n1, n1, n1))
+    Y = np.broadcast_to(ly[None, :, None, None, :, None], (nelz, nely, nelx, n1, n1, n1))
+    Z = np.broadcast_to(lz[:, None, None, :, None, None], (nelz, nely, nelx, n1, n1, n1))
+    return [np.ascontiguousarray(a).reshape(-1) for a in (X, Y, Z)]
+
+
+def box_dxmin(nel, N, lo=(0.0, 0.0, 0.0), hi=(2 * np.pi,) * 3):
+    """get_dxmin of a uniform box (interior-node half spacings)."""
+    lines = box_lines(nel, N, lo, hi)
+    d2 = min(float(np.min((l[:, 2:] - l[:, :-2]) ** 2)) for l in lines)
+    return float(np.sqrt(d2) / 2.0)
+
+
 class BoxMesh:
     """Reference-layout arrays for a box mesh.
 

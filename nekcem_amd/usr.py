@@ -62,3 +62,25 @@ def plane_waves(x, y, z, seed=2, nwaves=8, kmax=4):
 
 def element_eps(nelt, seed=1, lo=1.0, hi=4.0):
     return np.random.default_rng(seed).uniform(lo, hi, nelt)
+
+
+def cavity_batch(ncav=256, seed=3, nmin=2, nmax=16, Nmin=5, Nmax=9):
+    """The batched workload: ncav independent PEC box cavities on [0,1]^3,
+    each n^3 elements with n ~ U{nmin..nmax} and polynomial order
+    N ~ U{Nmin..Nmax}, per-element eps ~ U[1,4], mu = 1 (one seed)."""
+    rng = np.random.default_rng(seed)
+    cavs = []
+    for _ in range(ncav):
+        n = int(rng.integers(nmin, nmax + 1))
+        N = int(rng.integers(Nmin, Nmax + 1))
+        cavs.append(dict(nel=(n, n, n), N=N, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0),
+                         bc=("PEC", "PEC", "PEC"), eps_elem=rng.uniform(1.0, 4.0, n ** 3)))
+    return cavs
+
+
+def gaussian_ez(x, y, z, center=(0.5, 0.5, 0.5), width=0.2):
+    """Initial state with Ez = exp(-|x-c|^2 / (2 w^2)), all other fields 0."""
+    r2 = (x - center[0]) ** 2 + (y - center[1]) ** 2 + (z - center[2]) ** 2
+    E = np.zeros((3,) + x.shape)
+    E[2] = np.exp(-r2 / (2.0 * width * width))
+    return np.zeros_like(E), E
