@@ -154,6 +154,83 @@ def batched(ncav=256, steps=50, warmup=3, seed=3, log=sys.stderr):
                         f"per-element eps U[1,4], Gaussian Ez (width 0.2), own CFL-0.25 dt, seed {seed}"}
 
 
+def adjoint(nel=16, N=7, nsteps=500, every=50, ndir=32, nprobe=64, seed=4, log=sys.stderr):
+    """Checkpointed adjoint of the probe loss on the dielectric box: time
+    relative to the forward, and agreement with central differences
+    (h = 1e-6) of the HIP forward along ndir seeded random directions."""
+    from nekcem_amd import usr
+    from nekcem_amd.maxwell import MaxwellSolver
+    from nekcem_amd.mesh import BoxMesh
+    rng = np.random.default_rng(seed)
+    lo, hi = (0.0, 0.0, 0.0), (1.0, 1.0, 1.0)
+    mesh = BoxMesh((nel, nel, nel), N, lo, hi, ("PEC",) * 3)
+    n = mesh.npts
+    x, y, z = mesh.coords()
+    eps = 2.0 + 0.3 * usr.smooth_field(x, y, z, seed)
+    eps_t = 2.0 + 0.3 * usr.smooth_field(x, y, z, seed + 1)
+    amp = usr.gaussian_source(x, y, z, mesh.bmn)
+    probes = rng.choice(n, nprobe, replace=False)
+    dt = mesh.dt(0.25)
+    T = nsteps * dt
+    src = dict(direction=(0.0, 0.0, 1.0), t0=0.15 * T, tw=0.05 * T)
+    H0, E0 = np.zeros((3, n)), np.zeros((3, n))
+    s = MaxwellSolver(mesh, eps=eps)
+    s.set_source(amp, **src)
+
+    def forward(e):
+        s.set_materials(e)
+        s.time = 0.0
+        s.set_state(H0, E0)
+        s.step(dt, nsteps)
+        return s.get_state()[1][2, probes]
+
+    target = forward(eps_t)
+    # forward timing (plain steps) and the checkpointed adjoint
+    s.set_materials(eps)
+    s.time = 0.0
+    s.set_state(H0, E0)
+    s.sync()
+    t0 = time.perf_counter()
+    s.step(dt, nsteps)
+    s.sync()
+    t_fwd = time.perf_counter() - t0
+    s.time = 0.0
+    s.set_state(H0, E0)
+    s.sync()
+    t0 = time.perf_counter()
+    s.adjoint_forward(dt, nsteps, every)
+    s.sync()
+    t_fwd_ckpt = time.perf_counter() - t0
+    Ez = s.get_state()[1][2, probes]
+    diff = Ez - target
+    loss = float(np.sum(diff * diff))
+    lamE = np.zeros((3, n))
+    np.add.at(lamE[2], probes, 2.0 * diff)
+    t0 = time.perf_counter()
+    ge, _ = s.adjoint_backward(np.zeros((3, n)), lamE)
+    t_adj = time.perf_counter() - t0
+    print(f"[bench] adjoint: forward {t_fwd:.3f} s, forward+checkpoints {t_fwd_ckpt:.3f} s, "
+          f"backward {t_adj:.3f} s, loss {loss:.4e}", file=log, flush=True)
+    h = 1e-6
+    errs = []
+    for _ in range(ndir):
+        d = rng.standard_normal(n)
+        lp = float(np.sum((forward(eps + h * d) - target) ** 2))
+        lm = float(np.sum((forward(eps - h * d) - target) ** 2))
+        fd = (lp - lm) / (2 * h)
+        errs.append(abs(fd - float(np.dot(ge, d))) / abs(fd))
+    s.close()
+    err = max(errs) if errs else None
+    if errs:
+        print(f"[bench] adjoint vs FD over {ndir} directions: max rel err {err:.2e}", file=log, flush=True)
+    return {"metric": "adjoint/fwd time", "value": t_adj / t_fwd, "unit": "x",
+            "t_forward_s": t_fwd, "t_forward_checkpointed_s": t_fwd_ckpt, "t_adjoint_s": t_adj,
+            "fd_directions": ndir, "fd_h": h, "fd_max_rel_err": err, "loss": loss,
+            "workload": f"PEC box {nel}^3 elements N={N} ({n} points), per-node eps = 2 + 0.3 s(x), "
+                        f"Gaussian-pulse Ez source, {nprobe} probe nodes, {nsteps} steps, "
+                        f"checkpoint every {every}"}
+
+
 def load_traffic(npts):
     """HBM bytes per stage launch from the committed rocprofv3 --pmc summary."""
     p = os.path.join(ROOT, "profiles", "stage_kernel_box32_N7_traffic.json")
@@ -178,6 +255,9 @@ def main():
     ap.add_argument("--batched-steps", type=int, default=50,
                     help="steps of the batched-cavity speedup measurement (0 = skip)")
     ap.add_argument("--batched-only", action="store_true")
+    ap.add_argument("--adjoint-dirs", type=int, default=32,
+                    help="FD directions of the adjoint check (0 = skip the adjoint measurement)")
+    ap.add_argument("--adjoint-only", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (counter calibration)")
     args = ap.parse_args()
@@ -186,6 +266,9 @@ def main():
         return
     if args.batched_only:
         print(json.dumps(batched(steps=args.batched_steps)), flush=True)
+        return
+    if args.adjoint_only:
+        print(json.dumps(adjoint(ndir=args.adjoint_dirs)), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -258,6 +341,7 @@ def main():
                   "norm": "per field, sqrt(sum bm1 (gpu-cpu)^2)/sqrt(sum bm1 cpu^2)"}
 
     bat = batched(steps=args.batched_steps) if args.batched_steps > 0 else None
+    adj = adjoint(ndir=args.adjoint_dirs) if args.adjoint_dirs > 0 else None
 
     line = {
         "metric": "G(GLL-pt*field*stage)/s FP64, 32^3el N=7",
@@ -285,6 +369,7 @@ def main():
         "cpu_baseline": cpu,
         "parity": parity,
         "batched": bat,
+        "adjoint": adj,
     }
     solver.close()
     print(json.dumps(line), flush=True)

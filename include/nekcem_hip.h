@@ -133,6 +133,28 @@ int32_t nkc_set_drude(nkc_handle *h, int64_t ndrude, const int32_t *dindex,
  * nkc_residual) back, [3][ndrude] each; any pointer may be NULL. */
 int32_t nkc_get_drude(nkc_handle *h, double *J, double *kJ, double *resJ);
 
+/* Replace the materials by per-node permittivity and permeability (npts
+ * each): 1/eps, 1/mu and every face node's flux coefficients are rebuilt as
+ * cem_maxwell_materials does (src/cem_maxwell.F:262-325). */
+int32_t nkc_set_materials(nkc_handle *h, const double *eps, const double *mu);
+
+/* ---- discrete adjoint (no counterpart in the reference) ----------------
+ * nkc_adjoint_forward advances nsteps RK steps from the current state like
+ * nkc_step, keeping a checkpoint of the state every `every` steps.
+ * nkc_adjoint_backward takes dL/dH, dL/dE at the final state (3 components
+ * at stride ld) and returns dL/deps and dL/dmu per node (npts, may be NULL):
+ * segments are recomputed from their checkpoints keeping every stage, then
+ * the RK stages are reversed (transposed curl, transposed flux/lift,
+ * reversed low-storage RK).  The forward's final state is restored. */
+int32_t nkc_adjoint_forward(nkc_handle *h, double *time, double dt, int64_t nsteps,
+                            int64_t every);
+int32_t nkc_adjoint_backward(nkc_handle *h, const double *lamH, const double *lamE,
+                             int64_t ld, double *geps, double *gmu);
+/* aH, aE = B^T P (kH, kE) on the current state: the transpose of the
+ * operator whose residual is nkc_residual (without sources).  Test entry. */
+int32_t nkc_adjoint_apply(nkc_handle *h, const double *kH, const double *kE, double *aH,
+                          double *aE, int64_t ld);
+
 /* chk_blowup (src/cem_common.F:1150-1165): *maxabs = max |u| over the six
  * fields; returns NKC_ERR_BLOWUP if a value is non-finite or above 1e20. */
 int32_t nkc_check_finite(nkc_handle *h, double *maxabs);

@@ -223,6 +223,43 @@ inline int32_t compact_reference(const nkc_desc *d, HostMesh &m) {
 }
 
 // ---------------------------------------------------------------------------
+// per-node materials on an existing element table (box pairing): the flux
+// coefficients of every face node from eps, mu at the node and at its
+// partner, in the order cem_maxwell_materials forms them: gs sum, PEC
+// doubling, Z_1 = Z_0' - Z-, Z_0 = Z_0'/2.
+// ---------------------------------------------------------------------------
+inline void set_node_materials(HostMesh &m, const double *eps, const double *mu, double C0) {
+    const int n1 = m.n1, n2 = n1 * n1, n3 = n2 * n1;
+    const int64_t nelt = m.nelt, npts = nelt * n3;
+    m.mat = 1;
+    m.ieps.assign(npts, 0.0);
+    m.imu.assign(npts, 0.0);
+    m.fco.assign(nelt * 6 * n2 * 4, 0.0);
+    for (int64_t p = 0; p < npts; ++p) {
+        m.ieps[p] = 1.0 / eps[p];
+        m.imu[p] = 1.0 / mu[p];
+    }
+    for (int64_t e = 0; e < nelt; ++e)
+        for (int kf = 0; kf < 6; ++kf) {
+            const int32_t nb = m.nbr[e * 6 + kf];
+            for (int q = 0; q < n2; ++q) {
+                const int64_t v = e * n3 + face_node(n1, kf, q);
+                const double Zm = std::sqrt(mu[v] / eps[v]), Ym = std::sqrt(eps[v] / mu[v]);
+                double Z0 = Zm, Y0 = Ym;
+                if (nb >= 0) {
+                    const int64_t w = (int64_t)nb * n3 + face_node(n1, kf ^ 1, q);
+                    Z0 += std::sqrt(mu[w] / eps[w]);
+                    Y0 += std::sqrt(eps[w] / mu[w]);
+                } else if (nb == HOST_FACE_PEC) {
+                    Z0 *= 2.0;
+                    Y0 *= 2.0;
+                }
+                flux_coef(0.5 * Y0, Y0 - Ym, 0.5 * Z0, Z0 - Zm, C0, &m.fco[((e * 6 + kf) * n2 + q) * 4]);
+            }
+        }
+}
+
+// ---------------------------------------------------------------------------
 // GLL basis (ZWGLL / DGLL, src/nek5_speclib.F:107-121, 807-838): interior
 // nodes are the roots of P_N' (Newton with deflation), w = 2/(N(N+1)P_N^2),
 // D_ij = P_N(z_i)/(P_N(z_j)(z_i - z_j)), D_00 = -N(N+1)/4, D_NN = N(N+1)/4.

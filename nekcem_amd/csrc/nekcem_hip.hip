@@ -13,11 +13,13 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <map>
 #include <string>
 #include <vector>
 
 #include "../../include/nekcem_hip.h"
+#include "adjoint_kernel.hpp"
 #include "mesh_tables.hpp"
 #include "stage_kernel.hpp"
 
@@ -225,6 +227,22 @@ struct nkc_handle {
     int32_t *drude_off = nullptr;
     double *dJ = nullptr, *dkJ = nullptr, *dpar = nullptr, *dresJ = nullptr;
     std::vector<int64_t> drude_slot;      // listed node -> compact (elem*3*n3 + vloc)
+    // host copy of the element tables (material updates, adjoint)
+    HostMesh hm;
+    double C0 = 1.0;
+    double *yz = nullptr;                 // (sqrt(eps/mu), sqrt(mu/eps)) per ieps entry, adjoint only
+    // adjoint: checkpoints of the state every `every` steps of the last
+    // nkc_adjoint_forward, the stage store of one segment, adjoint buffers
+    struct Adj {
+        int64_t nsteps = 0, every = 0, nseg = 0;
+        double dt = 0.0, t0 = 0.0;
+        double *ckpt = nullptr;           // nseg x 6 fs: the state at the start of each segment
+        double *st_u = nullptr;           // (5 every + 1) x 6 fs: every stage input of a segment
+        double *st_r = nullptr;           // 5 every x 6 fs: every stage residual
+        double *lam[2] = {nullptr, nullptr}, *kap[2] = {nullptr, nullptr};
+        double *geps = nullptr, *gmu = nullptr;    // fs each
+        double *ufin = nullptr;           // final forward state (restored after backward)
+    } adj;
 };
 
 namespace {
@@ -292,8 +310,31 @@ int32_t capture(nkc_handle *h, double dt) {
     return NKC_OK;
 }
 
+void free_adjoint(nkc_handle *h) {
+    auto &A = h->adj;
+    double *p[] = {A.ckpt, A.st_u, A.st_r, A.lam[0], A.lam[1], A.kap[0], A.kap[1], A.geps, A.gmu, A.ufin};
+    for (double *q : p) (void)hipFree(q);
+    A = nkc_handle::Adj();
+}
+
+// Admittance/impedance pairs for the adjoint's material gradient, built
+// from the same inverse-material tables the stage kernel reads.
+int32_t ensure_yz(nkc_handle *h) {
+    if (h->yz) return NKC_OK;
+    const size_t n = h->hm.ieps.size();
+    std::vector<double> yz(2 * n);
+    for (size_t x = 0; x < n; ++x) {
+        yz[2 * x] = std::sqrt(h->hm.imu[x] / h->hm.ieps[x]);
+        yz[2 * x + 1] = std::sqrt(h->hm.ieps[x] / h->hm.imu[x]);
+    }
+    return upload(&h->yz, yz);
+}
+
 void free_handle(nkc_handle *h) {
     drop_graphs(h);
+    free_adjoint(h);
+    (void)hipFree(h->yz);
+    h->yz = nullptr;
     dev_mesh_free(h->m);
     (void)hipFree(h->res); (void)hipFree(h->red); (void)hipFree(h->tdev);
     (void)hipFree(h->src_off); (void)hipFree(h->src_amp);
@@ -302,10 +343,12 @@ void free_handle(nkc_handle *h) {
     if (h->stream) (void)hipStreamDestroy(h->stream);
 }
 
-int32_t make_handle(const HostMesh &hm, nkc_handle **out) {
+int32_t make_handle(const HostMesh &hm, double C0, nkc_handle **out) {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(NKC_ERR_NODEVICE, "no HIP device visible");
     nkc_handle *h = new nkc_handle();
+    h->hm = hm;
+    h->C0 = C0;
     int32_t rc = NKC_OK;
     do {
         if (hipGetDevice(&h->device) != hipSuccess) { rc = fail(NKC_ERR_HIP, "hipGetDevice"); break; }
@@ -344,6 +387,69 @@ int32_t copy_fields_d2h(const double *src, int64_t fs, int64_t src_off, double *
         HIPCHK(hipMemcpy(hn + c * ld, src + c * fs + src_off, nb, hipMemcpyDeviceToHost));
         HIPCHK(hipMemcpy(en + c * ld, src + (3 + c) * fs + src_off, nb, hipMemcpyDeviceToHost));
     }
+    return NKC_OK;
+}
+
+template <int N1, int MAT>
+void launch_adj_t(const AdjArgs &a, int grid, hipStream_t s) {
+    hipLaunchKernelGGL((adjoint_kernel<N1, MAT>), dim3(grid), dim3(Shape<N1>::BS), 0, s, a);
+}
+
+int32_t launch_adjoint(const DevMesh &dm, const AdjArgs &a0, hipStream_t s) {
+    const int per_xcd = (int)((dm.nelt + 7) / 8);
+    const int grid = 8 * per_xcd;
+    AdjArgs a = a0;
+    a.per_xcd = per_xcd;
+    a.D = dm.D;
+    a.geo = dm.geo;
+    a.nbr = dm.nbr;
+    a.fco = dm.fco;
+    a.ieps = dm.ieps;
+    a.imu = dm.imu;
+    a.fs = dm.fs;
+    a.nelt = (int32_t)dm.nelt;
+#define NKC_ADJ_CASE(n)                                                            \
+    case n:                                                                        \
+        if (dm.mat == MAT_ELEM) launch_adj_t<n, MAT_ELEM>(a, grid, s);             \
+        else launch_adj_t<n, MAT_NODE>(a, grid, s);                                \
+        break;
+    switch (dm.n1) {
+        NKC_ADJ_CASE(4) NKC_ADJ_CASE(5) NKC_ADJ_CASE(6) NKC_ADJ_CASE(7)
+        NKC_ADJ_CASE(8) NKC_ADJ_CASE(9) NKC_ADJ_CASE(10)
+    default: return fail(NKC_ERR_UNSUPPORTED, "polynomial order N=%d not built", dm.n1 - 1);
+    }
+#undef NKC_ADJ_CASE
+    HIPCHK(hipGetLastError());
+    return NKC_OK;
+}
+
+// One forward step of the segment recompute: stage s of step `slot` reads
+// its input from stage store slot 5 slot + s and writes the next stage's
+// input to the following slot, so every stage input is kept at no extra
+// traffic; the stage residual goes to the matching st_r slot (not captured:
+// the store offsets change with the slot).
+int32_t queue_step_store(nkc_handle *h, double dt, int64_t slot) {
+    const int64_t blk = 6 * h->m.fs;
+    for (int s = 0; s < 5; ++s) {
+        StageArgs a = base_args(h);
+        a.uin = h->adj.st_u + (slot * 5 + s) * blk;
+        a.uout = h->adj.st_u + (slot * 5 + s + 1) * blk;
+        a.kbuf = h->m.k;
+        a.rka = RK4A[s];
+        a.rkb = RK4B[s];
+        a.rkc = RK4C[s];
+        a.dt = dt;
+        a.st_r = h->adj.st_r + (slot * 5 + s) * blk;
+        const int32_t rc = launch_stage(h->m, a, MODE_RK, h->stream);
+        if (rc) return rc;
+    }
+    hipLaunchKernelGGL(advance_time, dim3(1), dim3(1), 0, h->stream, h->tdev, dt);
+    HIPCHK(hipGetLastError());
+    return NKC_OK;
+}
+
+int32_t dev_alloc(double **p, size_t n) {
+    HIPCHK(hipMalloc((void **)p, sizeof(double) * n));
     return NKC_OK;
 }
 
@@ -457,7 +563,7 @@ int32_t nkc_create(const nkc_desc *d, nkc_handle **out) {
     HostMesh hm;
     const int32_t rc = compact_reference(d, hm);
     if (rc) return fail(rc, "%s", hm.err.c_str());
-    return make_handle(hm, out);
+    return make_handle(hm, d->ifupwind ? 1.0 : 0.0, out);
 }
 
 int32_t nkc_create_box(const nkc_box_desc *d, nkc_handle **out) {
@@ -471,7 +577,7 @@ int32_t nkc_create_box(const nkc_box_desc *d, nkc_handle **out) {
     HostMesh hm;
     const int32_t rc = build_box(d, hm);
     if (rc) return fail(rc, "%s", hm.err.c_str());
-    return make_handle(hm, out);
+    return make_handle(hm, d->ifupwind ? 1.0 : 0.0, out);
 }
 
 int32_t nkc_destroy(nkc_handle *h) {
@@ -714,6 +820,160 @@ int32_t nkc_set_graph(nkc_handle *h, int32_t enable) {
     h->use_graph = enable != 0;
     if (!h->use_graph) return drop_graphs(h);
     return NKC_OK;
+}
+
+int32_t nkc_set_materials(nkc_handle *h, const double *eps, const double *mu) {
+    if (!h || !eps || !mu) return fail(NKC_ERR_ARG, "null handle or material array");
+    const int64_t npts = h->m.npts;
+    for (int64_t p = 0; p < npts; ++p)
+        if (!(eps[p] > 0.0) || !(mu[p] > 0.0)) return fail(NKC_ERR_ARG, "material not positive at node %lld", (long long)p);
+    HIPCHK(hipStreamSynchronize(h->stream));
+    set_node_materials(h->hm, eps, mu, h->C0);
+    (void)hipFree(h->m.fco); h->m.fco = nullptr;
+    (void)hipFree(h->m.ieps); h->m.ieps = nullptr;
+    (void)hipFree(h->m.imu); h->m.imu = nullptr;
+    (void)hipFree(h->yz); h->yz = nullptr;
+    int32_t rc;
+    if ((rc = upload(&h->m.fco, h->hm.fco))) return rc;
+    if ((rc = upload(&h->m.ieps, h->hm.ieps))) return rc;
+    if ((rc = upload(&h->m.imu, h->hm.imu))) return rc;
+    h->m.mat = MAT_NODE;
+    return drop_graphs(h);
+}
+
+int32_t nkc_adjoint_forward(nkc_handle *h, double *time, double dt, int64_t nsteps, int64_t every) {
+    if (!h) return fail(NKC_ERR_ARG, "null handle");
+    if (nsteps <= 0 || every <= 0 || !std::isfinite(dt)) return fail(NKC_ERR_ARG, "bad adjoint forward arguments");
+    if (h->ndrude) return fail(NKC_ERR_UNSUPPORTED, "adjoint with a Drude region is not built");
+    free_adjoint(h);
+    auto &A = h->adj;
+    A.nsteps = nsteps;
+    A.every = every < nsteps ? every : nsteps;
+    A.nseg = (nsteps + A.every - 1) / A.every;
+    A.dt = dt;
+    A.t0 = time ? *time : h->t_host;
+    const size_t blk = 6 * (size_t)h->m.fs;
+    int32_t rc;
+    if ((rc = dev_alloc(&A.ckpt, blk * A.nseg))) return rc;
+    if ((rc = dev_alloc(&A.ufin, blk))) return rc;
+    double t = A.t0;
+    for (int64_t s = 0; s < A.nseg; ++s) {
+        HIPCHK(hipMemcpyAsync(A.ckpt + s * blk, h->m.u[h->cur], sizeof(double) * blk,
+                              hipMemcpyDeviceToDevice, h->stream));
+        const int64_t n = std::min(A.every, nsteps - s * A.every);
+        if ((rc = nkc_step(h, &t, dt, n))) return rc;
+    }
+    HIPCHK(hipMemcpyAsync(A.ufin, h->m.u[h->cur], sizeof(double) * blk, hipMemcpyDeviceToDevice, h->stream));
+    if (time) *time = t;
+    return NKC_OK;
+}
+
+int32_t nkc_adjoint_backward(nkc_handle *h, const double *lamH, const double *lamE, int64_t ld,
+                             double *geps, double *gmu) {
+    if (!h || !lamH || !lamE) return fail(NKC_ERR_ARG, "null handle or adjoint seed");
+    auto &A = h->adj;
+    if (A.nseg == 0) return fail(NKC_ERR_ARG, "no nkc_adjoint_forward run to differentiate");
+    DevMesh &m = h->m;
+    if (ld < m.npts) return fail(NKC_ERR_ARG, "ld < npts");
+    const size_t blk = 6 * (size_t)m.fs;
+    int32_t rc;
+    if ((rc = ensure_yz(h))) return rc;
+    if (!A.st_u) {
+        if ((rc = dev_alloc(&A.st_u, blk * (5 * A.every + 1)))) return rc;
+        if ((rc = dev_alloc(&A.st_r, blk * 5 * A.every))) return rc;
+        for (int w = 0; w < 2; ++w) {
+            if ((rc = dev_alloc(&A.lam[w], blk))) return rc;
+            if ((rc = dev_alloc(&A.kap[w], blk))) return rc;
+        }
+        if ((rc = dev_alloc(&A.geps, m.fs))) return rc;
+        if ((rc = dev_alloc(&A.gmu, m.fs))) return rc;
+    }
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipMemset(A.lam[0], 0, sizeof(double) * blk));
+    HIPCHK(hipMemset(A.kap[0], 0, sizeof(double) * blk));
+    HIPCHK(hipMemset(A.geps, 0, sizeof(double) * m.fs));
+    HIPCHK(hipMemset(A.gmu, 0, sizeof(double) * m.fs));
+    if ((rc = copy_fields_h2d(A.lam[0], m.fs, 0, lamH, lamE, m.npts, ld))) return rc;
+    int li = 0;
+    for (int64_t seg = A.nseg - 1; seg >= 0; --seg) {
+        const int64_t n = std::min(A.every, A.nsteps - seg * A.every);
+        // recompute the segment from its checkpoint, keeping every stage
+        HIPCHK(hipMemcpyAsync(A.st_u, A.ckpt + seg * blk, sizeof(double) * blk, hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(hipMemsetAsync(m.k, 0, sizeof(double) * blk, h->stream));
+        double tseg = A.t0;
+        for (int64_t s = 0; s < seg * A.every; ++s) tseg = tseg + A.dt;   // the forward's time sums
+        HIPCHK(hipMemcpyAsync(h->tdev, &tseg, sizeof(double), hipMemcpyHostToDevice, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        for (int64_t s = 0; s < n; ++s)
+            if ((rc = queue_step_store(h, A.dt, s))) return rc;
+        // reversed stages
+        for (int64_t s = n - 1; s >= 0; --s)
+            for (int st = 4; st >= 0; --st) {
+                AdjArgs a{};
+                a.lam_in = A.lam[li];
+                a.kap_in = A.kap[li];
+                a.lam_out = A.lam[li ^ 1];
+                a.kap_out = A.kap[li ^ 1];
+                a.us = A.st_u + (s * 5 + st) * blk;
+                a.rs = A.st_r + (s * 5 + st) * blk;
+                a.geps = A.geps;
+                a.gmu = A.gmu;
+                a.rka = RK4A[st];
+                a.rkb = RK4B[st];
+                a.dt = A.dt;
+                a.C0 = h->C0;
+                a.yz = h->yz;
+                if ((rc = launch_adjoint(m, a, h->stream))) return rc;
+                li ^= 1;
+            }
+    }
+    // restore the forward's final state and time
+    HIPCHK(hipMemcpyAsync(m.u[h->cur], A.ufin, sizeof(double) * blk, hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->tdev, &h->t_host, sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (geps) HIPCHK(hipMemcpy(geps, A.geps, sizeof(double) * m.npts, hipMemcpyDeviceToHost));
+    if (gmu) HIPCHK(hipMemcpy(gmu, A.gmu, sizeof(double) * m.npts, hipMemcpyDeviceToHost));
+    return NKC_OK;
+}
+
+int32_t nkc_adjoint_apply(nkc_handle *h, const double *kH, const double *kE, double *aH, double *aE, int64_t ld) {
+    if (!h || !kH || !kE || !aH || !aE) return fail(NKC_ERR_ARG, "null argument");
+    DevMesh &m = h->m;
+    if (ld < m.npts) return fail(NKC_ERR_ARG, "ld < npts");
+    const size_t blk = 6 * (size_t)m.fs;
+    double *zero = nullptr, *kap = nullptr, *out = nullptr, *kout = nullptr, *g = nullptr;
+    int32_t rc = NKC_OK;
+    do {
+        if ((rc = ensure_yz(h))) break;
+        if ((rc = dev_alloc(&zero, blk)) || (rc = dev_alloc(&kap, blk)) || (rc = dev_alloc(&out, blk)) ||
+            (rc = dev_alloc(&kout, blk)) || (rc = dev_alloc(&g, 2 * m.fs)))
+            break;
+        if (hipMemset(zero, 0, sizeof(double) * blk) != hipSuccess || hipMemset(g, 0, sizeof(double) * 2 * m.fs) != hipSuccess) {
+            rc = fail(NKC_ERR_HIP, "hipMemset");
+            break;
+        }
+        if ((rc = copy_fields_h2d(kap, m.fs, 0, kH, kE, m.npts, ld))) break;
+        if (hipStreamSynchronize(h->stream) != hipSuccess) { rc = fail(NKC_ERR_HIP, "sync"); break; }
+        AdjArgs a{};
+        a.lam_in = zero;
+        a.kap_in = kap;
+        a.lam_out = out;
+        a.kap_out = kout;
+        a.us = m.u[h->cur];
+        a.rs = zero;
+        a.geps = g;
+        a.gmu = g + m.fs;
+        a.rka = 0.0;
+        a.rkb = 0.0;
+        a.dt = 1.0;
+        a.C0 = h->C0;
+        a.yz = h->yz;
+        if ((rc = launch_adjoint(m, a, h->stream))) break;
+        if (hipStreamSynchronize(h->stream) != hipSuccess) { rc = fail(NKC_ERR_HIP, "sync"); break; }
+        rc = copy_fields_d2h(out, m.fs, 0, aH, aE, m.npts, ld);
+    } while (0);
+    (void)hipFree(zero); (void)hipFree(kap); (void)hipFree(out); (void)hipFree(kout); (void)hipFree(g);
+    return rc;
 }
 
 // ---------------------------------------------------------------------------

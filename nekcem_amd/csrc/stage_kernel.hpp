@@ -95,6 +95,9 @@ struct StageArgs {
     double *__restrict__ dkJ;
     const double *__restrict__ dpar;
     double *__restrict__ dresJ;       // MODE_RESIDUAL: resJ out (same layout as J)
+    // MODE_RK, adjoint recompute: keep the stage residual (6 fields, stride
+    // fs) when non-null
+    double *__restrict__ st_r;
 };
 
 // n x v for n = sg * e_AX.
@@ -396,6 +399,10 @@ stage_kernel(StageArgs a) {
                 const double kk2 = a.rka * kp[c][kk] + dt * r[c];
                 st_stream(a.kbuf + p, kk2);
                 st_stream(a.uout + p, s_u[(co + c) * N3 + own] + a.rkb * kk2);
+            }
+            if (a.st_r) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) st_stream(a.st_r + (co + c) * fs + node, r[c]);
             }
         }
     }

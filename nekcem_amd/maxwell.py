@@ -203,6 +203,39 @@ class MaxwellSolver:
         _lib.check(self._L.nkc_get_drude(self._h, _d(J), _d(kJ), _d(rJ)))
         return J, kJ, rJ
 
+    def set_materials(self, eps, mu=None):
+        """Per-node permittivity/permeability (npts each)."""
+        eps = np.ascontiguousarray(np.broadcast_to(eps, (self.npts,)), dtype=np.float64)
+        mu = np.ones(self.npts) if mu is None else np.ascontiguousarray(np.broadcast_to(mu, (self.npts,)),
+                                                                        dtype=np.float64)
+        _lib.check(self._L.nkc_set_materials(self._h, _d(eps), _d(mu)))
+
+    def adjoint_forward(self, dt, nsteps, every):
+        """nsteps RK steps keeping a checkpoint every `every` steps."""
+        t = C.c_double(self.time)
+        _lib.check(self._L.nkc_adjoint_forward(self._h, C.byref(t), float(dt), int(nsteps), int(every)))
+        self.time = t.value
+        self.istep += int(nsteps)
+
+    def adjoint_backward(self, lamH, lamE):
+        """dL/deps, dL/dmu per node for a loss with dL/dH = lamH, dL/dE =
+        lamE at the end of the last adjoint_forward."""
+        n = self.npts
+        lamH = np.ascontiguousarray(lamH, dtype=np.float64).reshape(3, n)
+        lamE = np.ascontiguousarray(lamE, dtype=np.float64).reshape(3, n)
+        ge, gm = np.empty(n), np.empty(n)
+        _lib.check(self._L.nkc_adjoint_backward(self._h, _d(lamH), _d(lamE), n, _d(ge), _d(gm)))
+        return ge, gm
+
+    def adjoint_apply(self, kH, kE):
+        """B^T P (kH, kE): transpose of the (source-free) residual operator."""
+        n = self.npts
+        kH = np.ascontiguousarray(kH, dtype=np.float64).reshape(3, n)
+        kE = np.ascontiguousarray(kE, dtype=np.float64).reshape(3, n)
+        aH, aE = np.empty((3, n)), np.empty((3, n))
+        _lib.check(self._L.nkc_adjoint_apply(self._h, _d(kH), _d(kE), _d(aH), _d(aE), n))
+        return aH, aE
+
     def sync(self):
         _lib.check(self._L.nkc_sync(self._h))
 

@@ -84,3 +84,22 @@ def gaussian_ez(x, y, z, center=(0.5, 0.5, 0.5), width=0.2):
     E = np.zeros((3,) + x.shape)
     E[2] = np.exp(-r2 / (2.0 * width * width))
     return np.zeros_like(E), E
+
+
+def smooth_field(x, y, z, seed, nmodes=4, kmax=2):
+    """Smooth field on [0,1]^3: a few low-wavenumber sinusoids with seeded
+    integer wavevectors and phases, scaled to max |s| = 1."""
+    rng = np.random.default_rng(seed)
+    s = np.zeros_like(x)
+    for _ in range(nmodes):
+        k = rng.integers(-kmax, kmax + 1, size=3)
+        if not np.any(k):
+            k[0] = 1
+        s += rng.normal() * np.cos(np.pi * (k[0] * x + k[1] * y + k[2] * z) + rng.uniform(0, 2 * np.pi))
+    return s / np.max(np.abs(s))
+
+
+def gaussian_source(x, y, z, bm1, center=(0.4, 0.55, 0.45), width=0.1):
+    """bm1-weighted spatial amplitude of a Gaussian pulse source on Ez."""
+    r2 = (x - center[0]) ** 2 + (y - center[1]) ** 2 + (z - center[2]) ** 2
+    return np.exp(-r2 / (2.0 * width * width)) * bm1

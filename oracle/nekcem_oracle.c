@@ -712,3 +712,229 @@ void orc_set_num_threads(int n)
     (void)n;
 #endif
 }
+
+/* ------------------------------------------------------------------------ */
+/* Vector-Jacobian product of one residual evaluation (adjoint checker).     */
+/*                                                                           */
+/* r(u; eps, mu) = cem_maxwell_op above (no Drude).  Given kt = (kH, kE),    */
+/*   aH, aE    += (dr/du)^T kt                                               */
+/*   geps, gmu += (dr/deps)^T kt, (dr/dmu)^T kt      (per volume node)       */
+/* Each forward step is transposed in reverse order: inverse mass, lift,     */
+/* flux loop 2, PEC, gs (symmetric), loop 1 / restriction, and the weighted  */
+/* curl through the transpose of local_grad3; the eps/mu dependence runs     */
+/* through ebm1/hbm1 and through Z_0/Z_1/Y_0/Y_1 of cem_maxwell_materials.   */
+/* ------------------------------------------------------------------------ */
+
+/* u(l,jk) += sum_i D(i,l) ur(i,jk); u(i,l,k) += sum_j D(j,l) us(i,j,k);
+ * u(ij,l) += sum_k D(k,l) ut(ij,k) */
+static void local_grad3_t(int n1, const double *D, const double *ur, const double *us,
+                          const double *ut, double *u)
+{
+    int n2 = n1 * n1;
+    for (int jk = 0; jk < n2; ++jk)
+        for (int l = 0; l < n1; ++l) {
+            double s = 0.0;
+            for (int i = 0; i < n1; ++i) s += D[i + n1 * l] * ur[i + n1 * jk];
+            u[l + n1 * jk] += s;
+        }
+    for (int k = 0; k < n1; ++k)
+        for (int l = 0; l < n1; ++l)
+            for (int i = 0; i < n1; ++i) {
+                double s = 0.0;
+                for (int j = 0; j < n1; ++j) s += D[j + n1 * l] * us[i + n1 * j + n2 * k];
+                u[i + n1 * l + n2 * k] += s;
+            }
+    for (int l = 0; l < n1; ++l)
+        for (int ij = 0; ij < n2; ++ij) {
+            double s = 0.0;
+            for (int k = 0; k < n1; ++k) s += D[k + n1 * l] * ut[ij + n2 * k];
+            u[ij + n2 * l] += s;
+        }
+}
+
+/* a1,a2,a3 += W^T (v1,v2,v3) for the weighted curl W of maxwell_wght_curl */
+static void wght_curl_t(const orc_mesh *m, const double *v1, const double *v2, const double *v3,
+                        double sign, double *a1, double *a2, double *a3)
+{
+    int n1 = m->n1, n3 = n1 * n1 * n1;
+#pragma omp parallel
+    {
+        double *b = (double *)malloc(sizeof(double) * 9 * n3);
+        double *r1 = b, *s1 = b + n3, *t1 = b + 2 * n3;
+        double *r2 = b + 3 * n3, *s2 = b + 4 * n3, *t2 = b + 5 * n3;
+        double *r3 = b + 6 * n3, *s3 = b + 7 * n3, *t3 = b + 8 * n3;
+#pragma omp for schedule(static)
+        for (int64_t e = 0; e < m->nelt; ++e) {
+            int64_t o = (int64_t)n3 * e;
+            for (int i = 0; i < n3; ++i) {
+                int64_t k = o + i;
+                double w = m->w3[i] * sign;
+                double q1 = w * v1[k], q2 = w * v2[k], q3 = w * v3[k];
+                r1[i] = m->rz[k] * q2 - m->ry[k] * q3;
+                s1[i] = m->sz[k] * q2 - m->sy[k] * q3;
+                t1[i] = m->tz[k] * q2 - m->ty[k] * q3;
+                r2[i] = m->rx[k] * q3 - m->rz[k] * q1;
+                s2[i] = m->sx[k] * q3 - m->sz[k] * q1;
+                t2[i] = m->tx[k] * q3 - m->tz[k] * q1;
+                r3[i] = m->ry[k] * q1 - m->rx[k] * q2;
+                s3[i] = m->sy[k] * q1 - m->sx[k] * q2;
+                t3[i] = m->ty[k] * q1 - m->tx[k] * q2;
+            }
+            local_grad3_t(n1, m->D, r1, s1, t1, a1 + o);
+            local_grad3_t(n1, m->D, r2, s2, t2, a2 + o);
+            local_grad3_t(n1, m->D, r3, s3, t3, a3 + o);
+        }
+        free(b);
+    }
+}
+
+/* n x v */
+static inline void crossn(double nx, double ny, double nz, const double *v, double *o)
+{
+    o[0] = ny * v[2] - nz * v[1];
+    o[1] = nz * v[0] - nx * v[2];
+    o[2] = nx * v[1] - ny * v[0];
+}
+
+void orc_maxwell_vjp(const orc_mesh *m, double rktime, const double *eps, const double *mu,
+                     const int8_t *pec_face, const double *hn, const double *en,
+                     const double *kH, const double *kE, double *aH, double *aE,
+                     double *geps, double *gmu)
+{
+    int64_t npts = m->npts, k = m->nxzfl;
+    int n2 = m->n1 * m->n1;
+    /* forward residual at u (for the inverse-mass terms) and the face jumps */
+    double *rH = (double *)malloc(sizeof(double) * 3 * npts);
+    double *rE = (double *)malloc(sizeof(double) * 3 * npts);
+    orc_maxwell_op(m, rktime, hn, en, rH, rE, NULL, NULL);
+    /* jumps after gs + PEC: recompute loop 1, gs, PEC (srflx holds loop-2 output) */
+    double *s = (double *)malloc(sizeof(double) * 6 * k);
+    {
+        const double *H1 = hn, *H2 = hn + npts, *H3 = hn + 2 * npts;
+        const double *E1 = en, *E2 = en + npts, *E3 = en + 2 * npts;
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < k; ++i) {
+            int64_t v = m->cemface[i];
+            double nx = m->unx[i], ny = m->uny[i], nz = m->unz[i];
+            s[0 * k + i] = -ny * E3[v] + nz * E2[v];
+            s[1 * k + i] = -nz * E1[v] + nx * E3[v];
+            s[2 * k + i] = -nx * E2[v] + ny * E1[v];
+            s[3 * k + i] = -ny * H3[v] + nz * H2[v];
+            s[4 * k + i] = -nz * H1[v] + nx * H3[v];
+            s[5 * k + i] = -nx * H2[v] + ny * H1[v];
+        }
+        orc_gs_add(m->partner, s, k, 6, k);
+        for (int64_t j = 0; j < m->ncempec; ++j) {
+            int64_t i = m->cempec[j];
+            for (int c = 0; c < 3; ++c) { s[c * k + i] *= 2.0; s[(3 + c) * k + i] = 0.0; }
+        }
+    }
+    /* v = P kt ; inverse-mass gradients: d(ebm1 R)/deps = -r/eps */
+    double *vH = (double *)malloc(sizeof(double) * 3 * npts);
+    double *vE = (double *)malloc(sizeof(double) * 3 * npts);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < npts; ++i) {
+        double dE = 0.0, dH = 0.0;
+        for (int c = 0; c < 3; ++c) {
+            vH[c * npts + i] = m->hbm1[i] * kH[c * npts + i];
+            vE[c * npts + i] = m->ebm1[i] * kE[c * npts + i];
+            dE += kE[c * npts + i] * rE[c * npts + i];
+            dH += kH[c * npts + i] * rH[c * npts + i];
+        }
+        geps[i] -= dE / eps[i];
+        gmu[i] -= dH / mu[i];
+    }
+    /* volume: R_E = W(H), R_H = -W(E) */
+    wght_curl_t(m, vE, vE + npts, vE + 2 * npts, 1.0, aH, aH + npts, aH + 2 * npts);
+    wght_curl_t(m, vH, vH + npts, vH + 2 * npts, -1.0, aE, aE + npts, aE + 2 * npts);
+    /* lift^T and flux loop 2^T; coefficient gradients */
+    double *sb = (double *)malloc(sizeof(double) * 6 * k);
+    double *Yb0 = (double *)malloc(sizeof(double) * k), *Yb1 = (double *)malloc(sizeof(double) * k);
+    double *Zb0 = (double *)malloc(sizeof(double) * k), *Zb1 = (double *)malloc(sizeof(double) * k);
+    double C0 = m->C0;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < k; ++i) {
+        int64_t v = m->cemface[i];
+        double a = m->area[i];
+        double nx = m->unx[i], ny = m->uny[i], nz = m->unz[i];
+        double oE[3], oH[3], sE[3], sH[3], t[3], u2[3];
+        for (int c = 0; c < 3; ++c) {
+            oE[c] = a * vH[c * npts + v];      /* slots 0..2 lift into resH */
+            oH[c] = a * vE[c * npts + v];      /* slots 3..5 lift into resE */
+            sE[c] = s[c * k + i];
+            sH[c] = s[(3 + c) * k + i];
+        }
+        double Y0 = m->Y0[i], Y1 = m->Y1[i], Z0 = m->Z0[i], Z1 = m->Z1[i];
+        double Y02 = -0.5 / Y0 * Y1, Z02 = 0.5 / Z0 * Z1, C02Y = 0.5 / Y0 * C0, C02Z = 0.5 / Z0 * C0;
+        /* s_E-bar = Y02 oE + C02Z n x oH ; s_H-bar = C02Y n x oE + Z02 oH */
+        crossn(nx, ny, nz, oH, t);
+        crossn(nx, ny, nz, oE, u2);
+        for (int c = 0; c < 3; ++c) {
+            sb[c * k + i] = Y02 * oE[c] + C02Z * t[c];
+            sb[(3 + c) * k + i] = C02Y * u2[c] + Z02 * oH[c];
+        }
+        /* coefficient gradients */
+        double nsH[3], nsE[3];
+        crossn(nx, ny, nz, sH, nsH);
+        crossn(nx, ny, nz, sE, nsE);
+        double gY02 = 0, gC02Y = 0, gZ02 = 0, gC02Z = 0;
+        for (int c = 0; c < 3; ++c) {
+            gY02 += oE[c] * sE[c];
+            gC02Y -= oE[c] * nsH[c];
+            gZ02 += oH[c] * sH[c];
+            gC02Z -= oH[c] * nsE[c];
+        }
+        Yb0[i] = gY02 * (0.5 * Y1 / (Y0 * Y0)) + gC02Y * (-0.5 * C0 / (Y0 * Y0));
+        Yb1[i] = gY02 * (-0.5 / Y0);
+        Zb0[i] = gZ02 * (-0.5 * Z1 / (Z0 * Z0)) + gC02Z * (-0.5 * C0 / (Z0 * Z0));
+        Zb1[i] = gZ02 * (0.5 / Z0);
+    }
+    /* PEC^T: slots 0..2 doubled, 3..5 zeroed */
+    for (int64_t j = 0; j < m->ncempec; ++j) {
+        int64_t i = m->cempec[j];
+        for (int c = 0; c < 3; ++c) { sb[c * k + i] *= 2.0; sb[(3 + c) * k + i] = 0.0; }
+    }
+    /* gs^T = gs */
+    orc_gs_add(m->partner, sb, k, 6, k);
+    /* loop 1^T: s_E = -n x E  ->  E-bar += n x s_E-bar (same for H); elements independent */
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < m->nelt; ++e)
+        for (int64_t i = e * 6 * n2; i < (e + 1) * 6 * n2; ++i) {
+            int64_t v = m->cemface[i];
+            double nx = m->unx[i], ny = m->uny[i], nz = m->unz[i];
+            double bE[3] = {sb[0 * k + i], sb[1 * k + i], sb[2 * k + i]};
+            double bH[3] = {sb[3 * k + i], sb[4 * k + i], sb[5 * k + i]};
+            double tE[3], tH[3];
+            crossn(nx, ny, nz, bE, tE);
+            crossn(nx, ny, nz, bH, tH);
+            for (int c = 0; c < 3; ++c) {
+                aE[c * npts + v] += tE[c];
+                aH[c * npts + v] += tH[c];
+            }
+        }
+    /* materials^T: Z0'' = S G zimp, Z_1 = Z0'' - zimp, Z_0 = 0.5 Z0'' */
+    double *zb = (double *)malloc(sizeof(double) * k), *yb = (double *)malloc(sizeof(double) * k);
+    for (int64_t i = 0; i < k; ++i) {
+        zb[i] = 0.5 * Zb0[i] + Zb1[i];
+        yb[i] = 0.5 * Yb0[i] + Yb1[i];
+    }
+    for (int64_t e = 0; e < m->nelt; ++e)
+        for (int f = 0; f < 6; ++f)
+            if (pec_face[6 * e + f])
+                for (int j = 0; j < n2; ++j) {
+                    int64_t i = e * 6 * n2 + (int64_t)f * n2 + j;
+                    zb[i] *= 2.0;
+                    yb[i] *= 2.0;
+                }
+    orc_gs_add(m->partner, zb, k, 1, k);
+    orc_gs_add(m->partner, yb, k, 1, k);
+    for (int64_t i = 0; i < k; ++i) {
+        int64_t v = m->cemface[i];
+        double zimp = sqrt(mu[v] / eps[v]), ycon = sqrt(eps[v] / mu[v]);
+        double zbar = zb[i] - Zb1[i], ybar = yb[i] - Yb1[i];
+        geps[v] += zbar * (-0.5 * zimp / eps[v]) + ybar * (0.5 * ycon / eps[v]);
+        gmu[v] += zbar * (0.5 * zimp / mu[v]) + ybar * (-0.5 * ycon / mu[v]);
+    }
+    free(rH); free(rE); free(s); free(vH); free(vE); free(sb);
+    free(Yb0); free(Yb1); free(Zb0); free(Zb1); free(zb); free(yb);
+}

@@ -71,6 +71,8 @@ def lib():
                                  C.c_double, C.c_int64, _dp]
         L.orc_rk_coeffs.argtypes = [_dp, _dp, _dp]
         L.orc_cem_error.argtypes = [_dp, _dp, _dp, C.c_int64, _dp, _dp]
+        L.orc_maxwell_vjp.argtypes = [C.c_void_p, C.c_double, _dp, _dp, _bp, _dp, _dp, _dp, _dp,
+                                      _dp, _dp, _dp, _dp]
         L.orc_num_threads.restype = C.c_int
         L.orc_set_num_threads.argtypes = [C.c_int]
         _lib = L
@@ -351,6 +353,58 @@ class BoxCase:
         if J is not None:
             out.update(J=J, kJ=kJ)
         return out
+
+    def vjp(self, H, E, kH, kE, t=0.0):
+        """(aH, aE, geps, gmu): transpose of d residual/d(H,E) applied to
+        (kH, kE), and the eps/mu gradients of <(kH,kE), residual>."""
+        n = self.npts
+        arr = lambda a: np.ascontiguousarray(a, dtype=np.float64).reshape(3, n)
+        H, E, kH, kE = arr(H), arr(E), arr(kH), arr(kE)
+        aH, aE = np.zeros((3, n)), np.zeros((3, n))
+        ge, gm = np.zeros(n), np.zeros(n)
+        m = self._mesh()
+        lib().orc_maxwell_vjp(C.byref(m), t, _p(self.eps), _p(self.mu), _p(self.pec_face),
+                              _p(H), _p(E), _p(kH), _p(kE), _p(aH), _p(aE), _p(ge), _p(gm))
+        return aH, aE, ge, gm
+
+    def loss_and_grad(self, H0, E0, dt, nsteps, probes, target, t0=0.0):
+        """Probe loss L = sum_p (Ez(T)[p] - target[p])^2 after nsteps RK steps
+        from (H0, E0), and its gradient with respect to eps and mu per node by
+        the discrete adjoint of the 2N-storage RK (every stage input kept)."""
+        a, b, c = rk_coeffs()
+        H = np.array(H0, dtype=np.float64).reshape(3, self.npts)
+        E = np.array(E0, dtype=np.float64).reshape(3, self.npts)
+        kH, kE = np.zeros_like(H), np.zeros_like(E)
+        stages = []
+        t = t0
+        for _ in range(nsteps):
+            for i in range(5):
+                stages.append((H.copy(), E.copy(), t + dt * c[i]))
+                rH, rE = self.residual(H, E, t=t + dt * c[i])
+                kH = a[i] * kH + dt * rH
+                kE = a[i] * kE + dt * rE
+                H = H + b[i] * kH
+                E = E + b[i] * kE
+            t = t + dt
+        diff = E[2, probes] - target
+        loss = float(np.sum(diff * diff))
+        lH, lE = np.zeros_like(H), np.zeros_like(E)
+        np.add.at(lE[2], probes, 2.0 * diff)
+        qH, qE = np.zeros_like(H), np.zeros_like(E)
+        ge, gm = np.zeros(self.npts), np.zeros(self.npts)
+        for sidx in range(len(stages) - 1, -1, -1):
+            i = sidx % 5
+            Hs, Es, ts = stages[sidx]
+            tH = qH + b[i] * lH
+            tE = qE + b[i] * lE
+            aH, aE, g1, g2 = self.vjp(Hs, Es, tH, tE, t=ts)
+            lH = lH + dt * aH
+            lE = lE + dt * aE
+            qH = a[i] * tH
+            qE = a[i] * tE
+            ge += dt * g1
+            gm += dt * g2
+        return loss, ge, gm, (H, E)
 
     def cem_error(self, u, exact):
         l2 = np.zeros(1)

@@ -297,3 +297,67 @@ def test_gaussian_source_uniform(oracle):
     t = out["time"]
     exact = a * 0.2 * math.sqrt(math.pi) / 2 * (math.erf((t - 0.5) / 0.2) - math.erf(-0.5 / 0.2))
     assert np.max(np.abs(out["E"][2] - exact)) < 1e-10
+
+
+# --------------------------------------------- adjoint checker (oracle side) --
+@pytest.mark.parametrize("bc,upwind", [(("P", "P", "P"), True), (("PEC", "P", "PEC"), True),
+                                       (("P", "PEC", "P"), False)])
+def test_vjp_transpose_and_material_gradients(oracle, bc, upwind):
+    """orc_maxwell_vjp is the exact transpose of the residual in (H, E)
+    (dot-product test to round-off) and its eps/mu parts match central
+    differences of <k, residual>."""
+    rng = np.random.default_rng(0)
+    c = oracle.BoxCase(oracle.BoxSpec(nel=(2, 2, 3), N=4, bc=bc, upwind=upwind))
+    n = c.npts
+    eps, mu = rng.uniform(1, 3, n), rng.uniform(1, 2, n)
+    c.set_materials(eps, mu)
+    u = (rng.standard_normal((3, n)), rng.standard_normal((3, n)))
+    k = (rng.standard_normal((3, n)), rng.standard_normal((3, n)))
+    rH, rE = c.residual(*u)
+    aH, aE, ge, gm = c.vjp(*u, *k)
+    lhs = np.sum(k[0] * rH) + np.sum(k[1] * rE)
+    rhs = np.sum(aH * u[0]) + np.sum(aE * u[1])
+    assert abs(lhs - rhs) <= 1e-13 * abs(lhs)
+
+    def f(e, m):
+        c.set_materials(e, m)
+        r = c.residual(*u)
+        return np.sum(k[0] * r[0]) + np.sum(k[1] * r[1])
+
+    h = 1e-6
+    de, dm = rng.standard_normal(n), rng.standard_normal(n)
+    fd_e = (f(eps + h * de, mu) - f(eps - h * de, mu)) / (2 * h)
+    fd_m = (f(eps, mu + h * dm) - f(eps, mu - h * dm)) / (2 * h)
+    assert abs(fd_e - np.dot(ge, de)) <= 1e-7 * abs(fd_e)
+    assert abs(fd_m - np.dot(gm, dm)) <= 1e-7 * abs(fd_m)
+
+
+def test_rk_adjoint_gradient_matches_central_differences(oracle):
+    """Discrete adjoint of 20 RK45 steps with a Gaussian source: the
+    gradient of a probe loss w.r.t. per-node eps matches central FD (h=1e-6)
+    of the forward to 1e-6 relative in random directions."""
+    rng = np.random.default_rng(1)
+    c = oracle.BoxCase(oracle.BoxSpec(nel=(2, 2, 2), N=4, lo=(0, 0, 0), hi=(1, 1, 1), bc=("PEC",) * 3))
+    n = c.npts
+    x, y, z = c.xm1, c.ym1, c.zm1
+    eps = 2 + 0.3 * np.sin(2 * np.pi * x) * np.cos(np.pi * y) * np.sin(np.pi * z + 0.3)
+    mu = np.ones(n)
+    c.set_source(np.exp(-((x - .4) ** 2 + (y - .5) ** 2 + (z - .6) ** 2) / (2 * .15 ** 2)) * c.bm1,
+                 (0, 0, 1), 0.05, 0.03)
+    probes = rng.choice(n, 8, replace=False)
+    dt, K = c.dt_from_cfl(0.25), 20
+    H0, E0 = np.zeros((3, n)), np.zeros((3, n))
+    c.set_materials(eps * 1.05, mu)
+    target = c.run(H0, E0, dt, K)["E"][2, probes].copy()
+    c.set_materials(eps, mu)
+    L, ge, gm, _ = c.loss_and_grad(H0, E0, dt, K, probes, target)
+
+    def loss(e):
+        c.set_materials(e, mu)
+        d = c.run(H0, E0, dt, K)["E"][2, probes] - target
+        return float(np.sum(d * d))
+
+    for _ in range(3):
+        d = rng.standard_normal(n)
+        fd = (loss(eps + 1e-6 * d) - loss(eps - 1e-6 * d)) / 2e-6
+        assert abs(fd - np.dot(ge, d)) <= 1e-6 * abs(fd)
