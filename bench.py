@@ -225,10 +225,112 @@ def adjoint(nel=16, N=7, nsteps=500, every=50, ndir=32, nprobe=64, seed=4, log=s
         print(f"[bench] adjoint vs FD over {ndir} directions: max rel err {err:.2e}", file=log, flush=True)
     return {"metric": "adjoint/fwd time", "value": t_adj / t_fwd, "unit": "x",
             "t_forward_s": t_fwd, "t_forward_checkpointed_s": t_fwd_ckpt, "t_adjoint_s": t_adj,
+            "gradient_over_forward": (t_fwd_ckpt + t_adj) / t_fwd,
             "fd_directions": ndir, "fd_h": h, "fd_max_rel_err": err, "loss": loss,
             "workload": f"PEC box {nel}^3 elements N={N} ({n} points), per-node eps = 2 + 0.3 s(x), "
                         f"Gaussian-pulse Ez source, {nprobe} probe nodes, {nsteps} steps, "
                         f"checkpoint every {every}"}
+
+
+def drude_region(mesh, half=0.25):
+    """Elements whose centre lies in the central cube of half side (of the
+    unit box): (listed nodes, owning Drude element per node, element ids)."""
+    n3 = mesh.n1 ** 3
+    inside = np.ones(mesh.nelt, bool)
+    for v in mesh.coords():
+        inside &= np.abs(v.reshape(-1, n3).mean(axis=1) - 0.5) < half
+    elems = np.nonzero(inside)[0]
+    dindex = (elems[:, None] * n3 + np.arange(n3)[None, :]).ravel()
+    owner = np.repeat(np.arange(len(elems)), n3)
+    return dindex, owner, elems
+
+
+def dispersive(nel=24, N=9, nsteps=2000, every=100, ndir=2, nprobe=64, seed=5, log=sys.stderr):
+    """Dispersive box: a Drude medium (omega_p = 1, gamma = 0.1 per element)
+    in the central cube of half side, vacuum elsewhere, PEC walls, upwind,
+    Gaussian-pulse Ez source.  Forward throughput over nsteps, then the
+    checkpointed adjoint of the probe loss with respect to per-element
+    gamma and omega_p (target from a forward with seeded perturbed
+    parameters), checked against central differences (h = 1e-6) of the
+    HIP forward along ndir seeded directions."""
+    from nekcem_amd import usr
+    from nekcem_amd.maxwell import MaxwellSolver
+    from nekcem_amd.mesh import BoxMesh
+    rng = np.random.default_rng(seed)
+    mesh = BoxMesh((nel, nel, nel), N, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), ("PEC",) * 3)
+    n = mesh.npts
+    x, y, z = mesh.coords()
+    dindex, owner, elems = drude_region(mesh)
+    ne = len(elems)
+    gam, wp = np.full(ne, 0.1), np.full(ne, 1.0)
+    gam_t = gam * rng.uniform(0.8, 1.2, ne)
+    wp_t = wp * rng.uniform(0.9, 1.1, ne)
+    probes = rng.choice(n, nprobe, replace=False)
+    dt = mesh.dt(0.25)
+    T = nsteps * dt
+    H0, E0 = np.zeros((3, n)), np.zeros((3, n))
+    s = MaxwellSolver.from_box((nel, nel, nel), N, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), ("PEC",) * 3)
+    s.set_source(usr.gaussian_source(x, y, z, mesh.bmn), direction=(0.0, 0.0, 1.0), t0=0.15 * T, tw=0.05 * T)
+
+    def forward(g, w, timed=False):
+        s.set_drude(dindex, g[owner], (w * w)[owner])
+        s.time = 0.0
+        s.set_state(H0, E0)
+        s.sync()
+        t0 = time.perf_counter()
+        s.step(dt, nsteps)
+        s.sync()
+        el = time.perf_counter() - t0
+        Ez = s.get_state()[1][2, probes]
+        return (Ez, el) if timed else Ez
+
+    target = forward(gam_t, wp_t)
+    _, t_fwd = forward(gam, wp, timed=True)
+    s.set_drude(dindex, gam[owner], (wp * wp)[owner])
+    s.time = 0.0
+    s.set_state(H0, E0)
+    s.sync()
+    t0 = time.perf_counter()
+    s.adjoint_forward(dt, nsteps, every)
+    s.sync()
+    t_fwd_ckpt = time.perf_counter() - t0
+    info = s.adjoint_info()
+    diff = s.get_state()[1][2, probes] - target
+    loss = float(np.sum(diff * diff))
+    lamE = np.zeros((3, n))
+    np.add.at(lamE[2], probes, 2.0 * diff)
+    t0 = time.perf_counter()
+    s.adjoint_backward(np.zeros((3, n)), lamE)
+    ga, gb = s.adjoint_drude_grad()
+    t_adj = time.perf_counter() - t0
+    g_gam = np.bincount(owner, weights=ga, minlength=ne)
+    g_wp = 2.0 * wp * np.bincount(owner, weights=gb, minlength=ne)
+    print(f"[bench] dispersive: forward {t_fwd:.3f} s ({nsteps} steps), forward+checkpoints "
+          f"{t_fwd_ckpt:.3f} s, backward {t_adj:.3f} s, loss {loss:.4e}, adjoint {info}",
+          file=log, flush=True)
+    h = 1e-6
+    errs = []
+    for _ in range(ndir):
+        dg, dw = rng.standard_normal(ne), rng.standard_normal(ne)
+        lp = float(np.sum((forward(gam + h * dg, wp + h * dw) - target) ** 2))
+        lm = float(np.sum((forward(gam - h * dg, wp - h * dw) - target) ** 2))
+        fd = (lp - lm) / (2 * h)
+        errs.append(abs(fd - float(np.dot(g_gam, dg) + np.dot(g_wp, dw))) / abs(fd))
+    s.close()
+    err = max(errs) if errs else None
+    if errs:
+        print(f"[bench] dispersive adjoint vs FD over {ndir} directions: max rel err {err:.2e}",
+              file=log, flush=True)
+    return {"metric": "G(GLL-pt*field*stage)/s FP64 forward; adjoint/fwd time",
+            "value": n * 30 * nsteps / t_fwd / 1e9, "unit": "G(GLL-pt*field*stage)/s",
+            "adjoint_over_forward": t_adj / t_fwd,
+            "t_forward_s": t_fwd, "t_forward_checkpointed_s": t_fwd_ckpt, "t_adjoint_s": t_adj,
+            "adjoint_window_steps": info["window"], "adjoint_device_bytes": info["bytes"],
+            "fd_directions": ndir, "fd_h": h, "fd_max_rel_err": err, "loss": loss,
+            "workload": f"PEC box {nel}^3 elements N={N} ({n} points), Drude (omega_p=1, gamma=0.1) in "
+                        f"{ne} central elements, vacuum elsewhere, Gaussian-pulse Ez source, "
+                        f"{nprobe} probe nodes, {nsteps} steps, checkpoint every {every}; "
+                        f"gradient w.r.t. per-element gamma and omega_p"}
 
 
 def load_traffic(npts):
@@ -258,6 +360,10 @@ def main():
     ap.add_argument("--adjoint-dirs", type=int, default=32,
                     help="FD directions of the adjoint check (0 = skip the adjoint measurement)")
     ap.add_argument("--adjoint-only", action="store_true")
+    ap.add_argument("--dispersive-dirs", type=int, default=2,
+                    help="FD directions of the dispersive (Drude) adjoint check (-1 = skip the measurement)")
+    ap.add_argument("--dispersive-steps", type=int, default=2000)
+    ap.add_argument("--dispersive-only", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (counter calibration)")
     args = ap.parse_args()
@@ -269,6 +375,10 @@ def main():
         return
     if args.adjoint_only:
         print(json.dumps(adjoint(ndir=args.adjoint_dirs)), flush=True)
+        return
+    if args.dispersive_only:
+        print(json.dumps(dispersive(nsteps=args.dispersive_steps, ndir=max(args.dispersive_dirs, 0))),
+              flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -342,6 +452,8 @@ def main():
 
     bat = batched(steps=args.batched_steps) if args.batched_steps > 0 else None
     adj = adjoint(ndir=args.adjoint_dirs) if args.adjoint_dirs > 0 else None
+    disp = (dispersive(nsteps=args.dispersive_steps, ndir=args.dispersive_dirs)
+            if args.dispersive_dirs >= 0 else None)
 
     line = {
         "metric": "G(GLL-pt*field*stage)/s FP64, 32^3el N=7",
@@ -370,6 +482,7 @@ def main():
         "parity": parity,
         "batched": bat,
         "adjoint": adj,
+        "dispersive": disp,
     }
     solver.close()
     print(json.dumps(line), flush=True)

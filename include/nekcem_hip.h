@@ -145,11 +145,25 @@ int32_t nkc_set_materials(nkc_handle *h, const double *eps, const double *mu);
  * at stride ld) and returns dL/deps and dL/dmu per node (npts, may be NULL):
  * segments are recomputed from their checkpoints keeping every stage, then
  * the RK stages are reversed (transposed curl, transposed flux/lift,
- * reversed low-storage RK).  The forward's final state is restored. */
+ * reversed low-storage RK).  The forward's final state, RK registers and
+ * time are restored.  With a Drude region (nkc_set_drude) its current is
+ * checkpointed and reversed too, and nkc_adjoint_get_drude returns the
+ * gradients with respect to the per-node a (gamma) and b (omega_p^2) in the
+ * order of the listed nodes; the loss is taken not to depend on J(T).
+ * nkc_set_materials and nkc_set_drude discard a stored forward. */
 int32_t nkc_adjoint_forward(nkc_handle *h, double *time, double dt, int64_t nsteps,
                             int64_t every);
 int32_t nkc_adjoint_backward(nkc_handle *h, const double *lamH, const double *lamE,
                              int64_t ld, double *geps, double *gmu);
+int32_t nkc_adjoint_get_drude(nkc_handle *h, double *ga, double *gb);
+/* A segment is reversed in windows of S <= every steps whose 5 S stages (state
+ * and residual, plus the Drude current) are kept at once; window starts are
+ * recomputed from the segment checkpoint.  S is the largest that fits in
+ * 70 % of the free device memory, or at most max_steps when that is > 0
+ * (0 restores the default); it applies from the next nkc_adjoint_forward.
+ * nkc_adjoint_info: out[4] = {every, segments, S, device bytes held}. */
+int32_t nkc_adjoint_set_store(nkc_handle *h, int64_t max_steps);
+int32_t nkc_adjoint_info(nkc_handle *h, int64_t *out);
 /* aH, aE = B^T P (kH, kE) on the current state: the transpose of the
  * operator whose residual is nkc_residual (without sources).  Test entry. */
 int32_t nkc_adjoint_apply(nkc_handle *h, const double *kH, const double *kE, double *aH,

@@ -12,6 +12,9 @@ torch on the probe fields can then be differentiated with autograd, e.g.
     loss = ((out[5] - target) ** 2).sum()             # probe Ez
     loss.backward()                                   # eps.grad, mu.grad
 
+``DrudeProbeFields.apply(gamma, wp, problem)`` does the same with the
+per-element Drude damping and plasma frequency as the parameters.
+
 The reference has no adjoint; the forward it differentiates is the step of
 cem_maxwell_op_rk (src/cem_maxwell.F:327-345) with a usersrc-style source.
 PyTorch is only the autograd plumbing: fields move between torch and the
@@ -54,6 +57,41 @@ def _run_forward(problem: ProbeProblem, eps: np.ndarray, mu: np.ndarray):
     return np.concatenate([H[:, problem.probes], E[:, problem.probes]])
 
 
+@dataclass
+class DrudeProbeProblem:
+    """A probe problem whose parameters are the Drude gamma and omega_p of
+    each Drude element (set_drude's a = gamma, b = omega_p^2 at every listed
+    node of the element); eps and mu stay as the solver holds them."""
+    solver: MaxwellSolver
+    H0: np.ndarray
+    E0: np.ndarray
+    dt: float
+    nsteps: int
+    every: int
+    probes: np.ndarray
+    dindex: np.ndarray        # listed volume nodes (0-based)
+    owner: np.ndarray         # per listed node: its Drude element (0..ne-1)
+    t0: float = 0.0
+
+
+def _run_forward_drude(problem: DrudeProbeProblem, gamma: np.ndarray, wp: np.ndarray):
+    s = problem.solver
+    s.set_drude(problem.dindex, gamma[problem.owner], (wp * wp)[problem.owner])
+    s.time = problem.t0
+    s.set_state(problem.H0, problem.E0)
+    s.adjoint_forward(problem.dt, problem.nsteps, problem.every)
+    H, E = s.get_state()
+    return np.concatenate([H[:, problem.probes], E[:, problem.probes]])
+
+
+def _seed(probes, n, g):
+    lamH, lamE = np.zeros((3, n)), np.zeros((3, n))
+    for c in range(3):
+        np.add.at(lamH[c], probes, g[c])
+        np.add.at(lamE[c], probes, g[3 + c])
+    return lamH, lamE
+
+
 if torch is not None:
 
     class ProbeFields(torch.autograd.Function):
@@ -70,12 +108,33 @@ if torch is not None:
         @staticmethod
         def backward(ctx, grad_out):
             p = ctx.problem
-            n = p.solver.npts
             g = grad_out.detach().cpu().numpy().astype(np.float64)
-            lamH, lamE = np.zeros((3, n)), np.zeros((3, n))
-            for c in range(3):
-                np.add.at(lamH[c], p.probes, g[c])
-                np.add.at(lamE[c], p.probes, g[3 + c])
-            ge, gm = p.solver.adjoint_backward(lamH, lamE)
+            ge, gm = p.solver.adjoint_backward(*_seed(p.probes, p.solver.npts, g))
             dev = grad_out.device
             return torch.from_numpy(ge).to(dev), torch.from_numpy(gm).to(dev), None
+
+    class DrudeProbeFields(torch.autograd.Function):
+        """(gamma, omega_p) per Drude element -> fields at the probes; the
+        backward runs the same checkpointed adjoint, whose Drude part
+        reverses the polarisation-current ODE (dr_J/da = -J, dr_J/db = E)."""
+
+        @staticmethod
+        def forward(ctx, gamma, wp, problem: DrudeProbeProblem):
+            g = gamma.detach().cpu().numpy().astype(np.float64)
+            w = wp.detach().cpu().numpy().astype(np.float64)
+            out = _run_forward_drude(problem, g, w)
+            ctx.problem = problem
+            ctx.wp = w
+            return torch.from_numpy(out).to(gamma.device)
+
+        @staticmethod
+        def backward(ctx, grad_out):
+            p = ctx.problem
+            g = grad_out.detach().cpu().numpy().astype(np.float64)
+            p.solver.adjoint_backward(*_seed(p.probes, p.solver.npts, g))
+            ga, gb = p.solver.adjoint_drude_grad()
+            ne = len(ctx.wp)
+            g_gamma = np.bincount(p.owner, weights=ga, minlength=ne)
+            g_wp = 2.0 * ctx.wp * np.bincount(p.owner, weights=gb, minlength=ne)
+            dev = grad_out.device
+            return torch.from_numpy(g_gamma).to(dev), torch.from_numpy(g_wp).to(dev), None

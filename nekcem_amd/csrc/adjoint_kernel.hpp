@@ -32,7 +32,6 @@ struct AdjArgs {
     const double *__restrict__ D;
     const double *__restrict__ geo;
     const int32_t *__restrict__ nbr;
-    const double *__restrict__ fco;
     const double *__restrict__ ieps;
     const double *__restrict__ imu;
     const double *__restrict__ yz;      // (Y, Z) = (sqrt(eps/mu), sqrt(mu/eps)) pairs, like ieps
@@ -40,13 +39,18 @@ struct AdjArgs {
     int32_t nelt;
     int32_t per_xcd;
     double rka, rkb, dt, C0;
+    // Drude region (cem_maxwell_drude): per element offset into the compact
+    // per-element blocks (-1: none), per-node (a, b); the adjoint current
+    // and its RK register (in place), the stage-input current, and the
+    // per-node gradients with respect to a and b (accumulated)
+    const int32_t *__restrict__ drude_off;
+    const double *__restrict__ dpar;
+    double *__restrict__ lamJ;
+    double *__restrict__ kapJ;
+    const double *__restrict__ stJ;
+    double *__restrict__ gA;
+    double *__restrict__ gB;
 };
-
-template <int N1, int MAT>
-__device__ __forceinline__ const double *coef_ptr(const AdjArgs &a, int e, int f, int q) {
-    return (MAT == MAT_ELEM) ? a.fco + ((int64_t)e * 6 + f) * 4
-                             : a.fco + (((int64_t)e * 6 + f) * (N1 * N1) + q) * 4;
-}
 
 // Material data at one node: inverse permittivity and permeability and the
 // admittance/impedance pair, from the per-element or per-node tables.
@@ -61,15 +65,24 @@ __device__ __forceinline__ NodeMat node_mat(const AdjArgs &a, int e, int64_t nod
     return NodeMat{a.ieps[x], a.imu[x], yz.x, yz.y};
 }
 
-// Gradient of <a, F(coef)> with respect to (Y0, Y1, Z0, Z1) through
-// (Y02, C02Y, Z02, C02Z) = (-Y1/(2Y0), C0/(2Y0), Z1/(2Z0), C0/(2Z0)).
+// Flux coefficients of one side of a face, (Y02, C02Y, Z02, C02Z) =
+// (-Y1/(2Y0), C0/(2Y0), Z1/(2Z0), C0/(2Z0)), from hy = 1/(2Y0) and
+// hz = 1/(2Z0); the same products the host tables hold.
+__device__ __forceinline__ void side_coef(double hy, double hz, double Y1, double Z1, double C0, double c[4]) {
+    c[0] = -hy * Y1;
+    c[1] = hy * C0;
+    c[2] = hz * Z1;
+    c[3] = hz * C0;
+}
+
+// Gradient of <a, F(coef)> with respect to (Y0, Y1, Z0, Z1) through the
+// coefficients above (d(1/(2Y0))/dY0 = -2 hy^2).
 __device__ __forceinline__ void coef_bar(double gY02, double gC02Y, double gZ02, double gC02Z,
-                                         double Y0, double Y1, double Z0, double Z1, double C0,
+                                         double hy, double hz, double Y1, double Z1, double C0,
                                          double &Yb0, double &Yb1, double &Zb0, double &Zb1) {
-    const double hy = 0.5 / Y0, hz = 0.5 / Z0;
-    Yb0 = (gY02 * Y1 - gC02Y * C0) * (hy / Y0);
+    Yb0 = (gY02 * Y1 - gC02Y * C0) * (2.0 * hy * hy);
     Yb1 = -gY02 * hy;
-    Zb0 = -(gZ02 * Z1 + gC02Z * C0) * (hz / Z0);
+    Zb0 = -(gZ02 * Z1 + gC02Z * C0) * (2.0 * hz * hz);
     Zb1 = gZ02 * hz;
 }
 
@@ -77,15 +90,26 @@ __device__ __forceinline__ void coef_bar(double gY02, double gC02Y, double gZ02,
 // halves of the shared face, plus the eps/mu gradient at the own node.
 //   vo, uo: own v = P kt and own stage state (H then E)
 //   vn, un: the neighbour's at the paired node (interior faces)
-//   cf, cfn: own / neighbour flux coefficients, L, Ln: lift factors
+//   L, Ln: own / neighbour lift factors
 //   mo: own material; Yn, Zn: the neighbour's admittance and impedance
+// The flux coefficients are formed here from the admittances/impedances as
+// cem_maxwell_materials averages them (src/cem_maxwell.F:262-325): interior
+// Y0 = (Y- + Y+)/2, Y1 = Y+; PEC Y0 = Y1 = Y-; no partner Y0 = Y-/2, Y1 = 0.
 // Out: adj (H then E) added to B^T v at the own node; ge, gm gradients.
 template <int AX>
 __device__ __forceinline__ void adj_face(double sg, int type, const double vo[6], const double uo[6],
-                                         const double vn[6], const double un[6], const double cf[4],
-                                         const double cfn[4], double L, double Ln, double C0,
+                                         const double vn[6], const double un[6],
+                                         double L, double Ln, double C0,
                                          const NodeMat &mo, double Yn, double Zn,
                                          double adj[6], double &ge, double &gm) {
+    const double Yo = mo.Y, Zo = mo.Z;
+    double Ys, Zs;                       // twice the averages
+    if (type >= 0) { Ys = Yo + Yn; Zs = Zo + Zn; }
+    else if (type == FACE_PEC) { Ys = 2.0 * Yo; Zs = 2.0 * Zo; }
+    else { Ys = Yo; Zs = Zo; }
+    const double hy = 0.5 / (0.5 * Ys), hz = 0.5 / (0.5 * Zs);
+    double cf[4];
+    side_coef(hy, hz, Ys - Yo, Zs - Zo, C0, cf);
     double aH[3], aE[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) { aH[c] = L * vo[c]; aE[c] = L * vo[3 + c]; }
@@ -128,7 +152,6 @@ __device__ __forceinline__ void adj_face(double sg, int type, const double vo[6]
             adj[c] = cf[1] * NNaH[c] + cf[2] * NaE[c];
         }
     }
-    const double Yo = mo.Y, Zo = mo.Z;
     double NsH[3], NsE[3];
     ncross<AX>(sg, sH, NsH);
     ncross<AX>(sg, sE, NsE);
@@ -143,10 +166,12 @@ __device__ __forceinline__ void adj_face(double sg, int type, const double vo[6]
     double dY, dZ;
     if (type >= 0) {
         double Yb0, Yb1, Zb0, Zb1;
-        coef_bar(gY02, gC02Y, gZ02, gC02Z, 0.5 * (Yo + Yn), Yn, 0.5 * (Zo + Zn), Zn, C0, Yb0, Yb1, Zb0, Zb1);
+        coef_bar(gY02, gC02Y, gZ02, gC02Z, hy, hz, Ys - Yo, Zs - Zo, C0, Yb0, Yb1, Zb0, Zb1);
         dY = 0.5 * Yb0;
         dZ = 0.5 * Zb0;
         // neighbour half: its normal is -n, its jump vectors equal ours
+        double cfn[4];
+        side_coef(hy, hz, Ys - Yn, Zs - Zn, C0, cfn);
         double bH[3], bE[3], NbH[3], NbE[3], NNbH[3], NNbE[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) { bH[c] = Ln * vn[c]; bE[c] = Ln * vn[3 + c]; }
@@ -167,17 +192,17 @@ __device__ __forceinline__ void adj_face(double sg, int type, const double vo[6]
             hZ02 += bE[c] * sH[c];
             hC02Z += bE[c] * NsE[c];
         }
-        coef_bar(hY02, hC02Y, hZ02, hC02Z, 0.5 * (Yn + Yo), Yo, 0.5 * (Zn + Zo), Zo, C0, Yb0, Yb1, Zb0, Zb1);
+        coef_bar(hY02, hC02Y, hZ02, hC02Z, hy, hz, Ys - Yn, Zs - Zn, C0, Yb0, Yb1, Zb0, Zb1);
         dY += 0.5 * Yb0 + Yb1;
         dZ += 0.5 * Zb0 + Zb1;
     } else if (type == FACE_PEC) {
         double Yb0, Yb1, Zb0, Zb1;
-        coef_bar(gY02, gC02Y, gZ02, gC02Z, Yo, Yo, Zo, Zo, C0, Yb0, Yb1, Zb0, Zb1);
+        coef_bar(gY02, gC02Y, gZ02, gC02Z, hy, hz, Yo, Zo, C0, Yb0, Yb1, Zb0, Zb1);
         dY = Yb0 + Yb1;
         dZ = Zb0 + Zb1;
     } else {
         double Yb0, Yb1, Zb0, Zb1;
-        coef_bar(gY02, gC02Y, gZ02, gC02Z, 0.5 * Yo, 0.0, 0.5 * Zo, 0.0, C0, Yb0, Yb1, Zb0, Zb1);
+        coef_bar(gY02, gC02Y, gZ02, gC02Z, hy, hz, 0.0, 0.0, C0, Yb0, Yb1, Zb0, Zb1);
         dY = 0.5 * Yb0;
         dZ = 0.5 * Zb0;
     }
@@ -192,12 +217,13 @@ __global__ void __launch_bounds__(Shape<N1>::BS) NKC_OCC
 adjoint_kernel(AdjArgs a) {
     using S = Shape<N1>;
     constexpr int N2 = S::N2, N3 = S::N3, KH = S::KH, LANES = S::LANES;
-    // level loop unrolled except at N1 = 8, where the unrolled loop
-    // is scheduled into 244 VGPRs (2 waves/SIMD) against 164 rolled
-    constexpr int LVL_UNROLL = (N1 == 8) ? 1 : KH;
+    // level loop unrolled except at N1 = 10, where unrolling costs
+    // a wave per SIMD (240 VGPRs against 179)
+    constexpr int LVL_UNROLL = (N1 >= 10) ? 1 : KH;
     __shared__ double s_v[6 * N3];      // v = P kt, swizzled like s_u
     __shared__ double s_f[32 * N2];     // [face 0..3][adjH 3, adjE 3, geps, gmu][q]
     __shared__ double s_D[N2];
+    __shared__ double s_g[2 * N3];      // [grp][k N2 + q]: gradient without the face terms
 
     const int e = remap_element(blockIdx.x, a.nelt, a.per_xcd);
     if (e >= a.nelt) return;
@@ -212,22 +238,45 @@ adjoint_kernel(AdjArgs a) {
     const int64_t fs = a.fs;
     const int fo = 3 * grp;              // this role's field offset
 
-    // ---- phase 0: this role's fields and levels; v = P (kappa + b lambda) into LDS ----
+    // ---- phase 0: this role's fields and levels.  v = P kt goes to LDS;
+    // the kappa output and the inverse-mass part of the material gradient
+    // (d r/d m = -r/m, so -kt.r/m) need no other lane and are finished
+    // here, so that the role's lambda, kappa, r and g streams are all
+    // loaded up front ----
     if (t < N2) s_D[t] = a.D[t];
     if (act) {
+        double lm[3][KH], kp[3][KH], rr[3][KH], gg[KH], pm[KH];
+        double *g = grp ? a.geps : a.gmu;
+#pragma unroll
+        for (int kk = 0; kk < KH; ++kk) {
+            const int k = k0 + kk;
+            const bool ok = k < N1;
+            const int64_t node = eo + k * N2 + q;
+            if (MAT == MAT_ELEM) pm[kk] = grp ? a.ieps[e] : a.imu[e];
+            else pm[kk] = ok ? (grp ? a.ieps[node] : a.imu[node]) : 0.0;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const int64_t p = (fo + c) * fs + node;
+                lm[c][kk] = ok ? a.lam_in[p] : 0.0;
+                kp[c][kk] = ok ? a.kap_in[p] : 0.0;
+                rr[c][kk] = ok ? ld_stream(a.rs + p) : 0.0;
+            }
+            gg[kk] = ok ? g[node] : 0.0;
+        }
 #pragma unroll
         for (int kk = 0; kk < KH; ++kk) {
             const int k = k0 + kk;
             if (k >= N1) break;
             const int64_t node = eo + k * N2 + q;
-            double pm;
-            if (MAT == MAT_ELEM) pm = grp ? a.ieps[e] : a.imu[e];
-            else pm = grp ? a.ieps[node] : a.imu[node];
+            double kr = 0.0;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                const int64_t p = (fo + c) * fs + node;
-                s_v[(fo + c) * N3 + S::idx(i, j, k)] = pm * (a.kap_in[p] + a.rkb * a.lam_in[p]);
+                const double kt = kp[c][kk] + a.rkb * lm[c][kk];
+                s_v[(fo + c) * N3 + S::idx(i, j, k)] = pm[kk] * kt;
+                kr += kt * rr[c][kk];
+                st_stream(a.kap_out + (fo + c) * fs + node, a.rka * kt);
             }
+            s_g[grp * N3 + k * N2 + q] = gg[kk] - a.dt * kr * pm[kk];
         }
     }
     const double *geo = a.geo + (int64_t)e * 9;
@@ -244,7 +293,7 @@ adjoint_kernel(AdjArgs a) {
         if (r == 2) { vi = i;      vj = 0;      vk = j; nbn = i + N1 * (N1 - 1) + N2 * j; }
         if (r == 3) { vi = i;      vj = N1 - 1; vk = j; nbn = i + N2 * j; }
         const int64_t own = eo + vi + N1 * vj + N2 * vk;
-        double vo[6], uo[6], vn[6], un[6], cfn[4] = {0, 0, 0, 0};
+        double vo[6], uo[6], vn[6], un[6];
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
             vo[c] = s_v[c * N3 + S::idx(vi, vj, vk)];
@@ -263,19 +312,15 @@ adjoint_kernel(AdjArgs a) {
                 vn[c] = pm * (a.kap_in[c * fs + pn] + a.rkb * a.lam_in[c * fs + pn]);
                 un[c] = a.us[c * fs + pn];
             }
-            const double *pc = coef_ptr<N1, MAT>(a, nb, r ^ 1, q);
-            cfn[0] = pc[0]; cfn[1] = pc[1]; cfn[2] = pc[2]; cfn[3] = pc[3];
             Ln = a.geo[(int64_t)nb * 9 + 3 + (r ^ 1)];
         } else {
 #pragma unroll
             for (int c = 0; c < 6; ++c) { vn[c] = 0.0; un[c] = 0.0; }
         }
-        const double *pc = coef_ptr<N1, MAT>(a, e, r, q);
-        const double cf[4] = {pc[0], pc[1], pc[2], pc[3]};
         double adj[6], ge, gm;
         const double sg = (r & 1) ? 1.0 : -1.0;
-        if (r < 2) adj_face<0>(sg, nb, vo, uo, vn, un, cf, cfn, geo[3 + r], Ln, a.C0, mo, Yn, Zn, adj, ge, gm);
-        else       adj_face<1>(sg, nb, vo, uo, vn, un, cf, cfn, geo[3 + r], Ln, a.C0, mo, Yn, Zn, adj, ge, gm);
+        if (r < 2) adj_face<0>(sg, nb, vo, uo, vn, un, geo[3 + r], Ln, a.C0, mo, Yn, Zn, adj, ge, gm);
+        else       adj_face<1>(sg, nb, vo, uo, vn, un, geo[3 + r], Ln, a.C0, mo, Yn, Zn, adj, ge, gm);
 #pragma unroll
         for (int c = 0; c < 6; ++c) s_f[(r * 8 + c) * N2 + q] = adj[c];
         s_f[(r * 8 + 6) * N2 + q] = ge;
@@ -301,7 +346,7 @@ adjoint_kernel(AdjArgs a) {
         const int k = kz;
         const int64_t node = eo + k * N2 + q;
         const int64_t nbn = half ? (int64_t)q : (int64_t)(N1 - 1) * N2 + q;
-        double vo[6], uo[6], vn[6], un[6], cfn[4] = {0, 0, 0, 0};
+        double vo[6], uo[6], vn[6], un[6];
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
             vo[c] = s_v[c * N3 + S::idx(i, j, k)];
@@ -320,17 +365,13 @@ adjoint_kernel(AdjArgs a) {
                 vn[c] = pm * (a.kap_in[c * fs + pn] + a.rkb * a.lam_in[c * fs + pn]);
                 un[c] = a.us[c * fs + pn];
             }
-            const double *pc = coef_ptr<N1, MAT>(a, nbz, zface ^ 1, q);
-            cfn[0] = pc[0]; cfn[1] = pc[1]; cfn[2] = pc[2]; cfn[3] = pc[3];
             Ln = a.geo[(int64_t)nbz * 9 + 3 + (zface ^ 1)];
         } else {
 #pragma unroll
             for (int c = 0; c < 6; ++c) { vn[c] = 0.0; un[c] = 0.0; }
         }
-        const double *pc = coef_ptr<N1, MAT>(a, e, zface, q);
-        const double cf[4] = {pc[0], pc[1], pc[2], pc[3]};
         double adj[6], ge, gm;
-        adj_face<2>(half ? 1.0 : -1.0, nbz, vo, uo, vn, un, cf, cfn, geo[3 + zface], Ln, a.C0,
+        adj_face<2>(half ? 1.0 : -1.0, nbz, vo, uo, vn, un, geo[3 + zface], Ln, a.C0,
                     mo, Yn, Zn, adj, ge, gm);
 #pragma unroll
         for (int c = 0; c < 3; ++c) zadj[c] = adj[fo + c];
@@ -388,27 +429,37 @@ adjoint_kernel(AdjArgs a) {
             for (int c = 0; c < 3; ++c) w[c] += s_f[(3 * 8 + fo + c) * N2 + i + N1 * k];
             gface += s_f[(3 * 8 + 6 + (1 - grp)) * N2 + i + N1 * k];
         }
-        // inverse-mass part of the material gradient: d r/d m = -r/m; the
-        // role's lambda, kappa (re-read, cached from phase 0) and the stage
-        // residual
-        const double im = (MAT == MAT_ELEM) ? (grp ? a.ieps[e] : a.imu[e])
-                                            : (grp ? a.ieps[node] : a.imu[node]);
-        double lam[3], kt[3], kr = 0.0;
+        if (grp == 1 && a.drude_off) {
+            // r_J = -a J + b E, r_E -= J/eps: lambda_E += dt b ktJ,
+            // lambda_J += dt (-v_E - a ktJ), kappa_J = a_i ktJ;
+            // d/da = -ktJ.J, d/db = ktJ.E at the stage input
+            const int dd = a.drude_off[e];
+            if (dd >= 0) {
+                const int64_t jb = (int64_t)dd * 3 * N3 + vloc;
+                const int64_t pp = (int64_t)dd * N3 + vloc;
+                const double pa = a.dpar[pp * 2 + 0], pb = a.dpar[pp * 2 + 1];
+                double sJ = 0.0, sE = 0.0;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const int64_t p = (fo + c) * fs + node;
-            lam[c] = a.lam_in[p];
-            kt[c] = a.kap_in[p] + a.rkb * lam[c];
-            kr += kt[c] * ld_stream(a.rs + p);
+                for (int c = 0; c < 3; ++c) {
+                    const double lj = a.lamJ[jb + c * N3];
+                    const double kt = a.kapJ[jb + c * N3] + a.rkb * lj;
+                    const double vE = s_v[(3 + c) * N3 + S::idx(i, j, k)];
+                    w[c] += pb * kt;
+                    a.lamJ[jb + c * N3] = lj + a.dt * (-vE - pa * kt);
+                    a.kapJ[jb + c * N3] = a.rka * kt;
+                    sJ += kt * a.stJ[jb + c * N3];
+                    sE += kt * a.us[(3 + c) * fs + node];
+                }
+                a.gA[pp] -= a.dt * sJ;
+                a.gB[pp] += a.dt * sE;
+            }
         }
-        const double gmat = gface - kr * im;
         double *g = grp ? a.geps : a.gmu;
-        g[node] += a.dt * gmat;
+        g[node] = s_g[grp * N3 + vloc] + a.dt * gface;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const int64_t p = (fo + c) * fs + node;
-            st_stream(a.lam_out + p, lam[c] + a.dt * w[c]);
-            st_stream(a.kap_out + p, a.rka * kt[c]);
+            st_stream(a.lam_out + p, a.lam_in[p] + a.dt * w[c]);
         }
     }
 }

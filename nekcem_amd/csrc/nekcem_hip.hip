@@ -231,21 +231,46 @@ struct nkc_handle {
     HostMesh hm;
     double C0 = 1.0;
     double *yz = nullptr;                 // (sqrt(eps/mu), sqrt(mu/eps)) per ieps entry, adjoint only
+    int64_t adj_store_cap = 0;            // cap on the adjoint's stage-store window (0: from free memory)
     // adjoint: checkpoints of the state every `every` steps of the last
     // nkc_adjoint_forward, the stage store of one segment, adjoint buffers
     struct Adj {
         int64_t nsteps = 0, every = 0, nseg = 0;
+        // A segment (every steps) is reversed in windows of S steps: the
+        // window's start states are recomputed from the segment checkpoint
+        // into `sub`, then each window is recomputed through the stage store
+        // (5 S stages) and reversed.  S = every when the store fits.
+        int64_t S = 0, nsub_max = 0;
+        int64_t sub_seg = -1;             // segment whose window starts `sub` holds
+        int64_t stored_seg = -1, stored_sub = -1;   // window the stage store holds
+        size_t bytes = 0;                 // device bytes held by the adjoint
+        bool have_grad = false;           // a backward sweep has filled the gradients
         double dt = 0.0, t0 = 0.0;
         double *ckpt = nullptr;           // nseg x 6 fs: the state at the start of each segment
-        double *st_u = nullptr;           // (5 every + 1) x 6 fs: every stage input of a segment
-        double *st_r = nullptr;           // 5 every x 6 fs: every stage residual
+        double *sub = nullptr;            // nsub_max x 6 fs: window start states
+        double *subJ = nullptr;           // nsub_max x jsz
+        double *st_u = nullptr;           // (5 S + 1) x 6 fs: every stage input of a window
+        double *st_r = nullptr;           // 5 S x 6 fs: every stage residual
         double *lam[2] = {nullptr, nullptr}, *kap[2] = {nullptr, nullptr};
         double *geps = nullptr, *gmu = nullptr;    // fs each
         double *ufin = nullptr;           // final forward state (restored after backward)
+        double *kfin = nullptr;           // final RK register
+        // Drude region (jsz = ndelem 3 n3 doubles per current block)
+        double *ckJ = nullptr;            // nseg x jsz: the current at each segment start
+        double *stJ = nullptr;            // 5 S x jsz: every stage-input current
+        double *lamJ = nullptr, *kapJ = nullptr;   // adjoint current and its RK register
+        double *gA = nullptr, *gB = nullptr;       // ndelem n3 each: d/da, d/db per node
+        double *Jfin = nullptr, *kJfin = nullptr;  // final current and its RK register
     } adj;
 };
 
 namespace {
+
+// doubles in one compact Drude current block (ndelem x 3 x n3)
+size_t drude_size(const nkc_handle *h) {
+    const size_t n3 = (size_t)h->m.n1 * h->m.n1 * h->m.n1;
+    return (size_t)h->ndelem * 3 * n3;
+}
 
 StageArgs base_args(nkc_handle *h) {
     StageArgs a = mesh_args(h->m);
@@ -312,7 +337,8 @@ int32_t capture(nkc_handle *h, double dt) {
 
 void free_adjoint(nkc_handle *h) {
     auto &A = h->adj;
-    double *p[] = {A.ckpt, A.st_u, A.st_r, A.lam[0], A.lam[1], A.kap[0], A.kap[1], A.geps, A.gmu, A.ufin};
+    double *p[] = {A.ckpt, A.st_u, A.st_r, A.lam[0], A.lam[1], A.kap[0], A.kap[1], A.geps, A.gmu, A.ufin,
+                   A.kfin, A.ckJ, A.stJ, A.lamJ, A.kapJ, A.gA, A.gB, A.Jfin, A.kJfin, A.sub, A.subJ};
     for (double *q : p) (void)hipFree(q);
     A = nkc_handle::Adj();
 }
@@ -403,7 +429,6 @@ int32_t launch_adjoint(const DevMesh &dm, const AdjArgs &a0, hipStream_t s) {
     a.D = dm.D;
     a.geo = dm.geo;
     a.nbr = dm.nbr;
-    a.fco = dm.fco;
     a.ieps = dm.ieps;
     a.imu = dm.imu;
     a.fs = dm.fs;
@@ -440,6 +465,7 @@ int32_t queue_step_store(nkc_handle *h, double dt, int64_t slot) {
         a.rkc = RK4C[s];
         a.dt = dt;
         a.st_r = h->adj.st_r + (slot * 5 + s) * blk;
+        if (h->ndrude) a.st_J = h->adj.stJ + (slot * 5 + s) * drude_size(h);
         const int32_t rc = launch_stage(h->m, a, MODE_RK, h->stream);
         if (rc) return rc;
     }
@@ -715,6 +741,7 @@ int32_t nkc_set_drude(nkc_handle *h, int64_t ndrude, const int32_t *dindex, cons
     (void)hipFree(h->dkJ); h->dkJ = nullptr;
     (void)hipFree(h->dpar); h->dpar = nullptr;
     (void)hipFree(h->dresJ); h->dresJ = nullptr;
+    free_adjoint(h);                      // the stored trajectory has the old region
     h->ndrude = 0;
     h->ndelem = 0;
     h->drude_slot.clear();
@@ -828,6 +855,7 @@ int32_t nkc_set_materials(nkc_handle *h, const double *eps, const double *mu) {
     for (int64_t p = 0; p < npts; ++p)
         if (!(eps[p] > 0.0) || !(mu[p] > 0.0)) return fail(NKC_ERR_ARG, "material not positive at node %lld", (long long)p);
     HIPCHK(hipStreamSynchronize(h->stream));
+    free_adjoint(h);                      // the stored trajectory has the old materials
     set_node_materials(h->hm, eps, mu, h->C0);
     (void)hipFree(h->m.fco); h->m.fco = nullptr;
     (void)hipFree(h->m.ieps); h->m.ieps = nullptr;
@@ -841,10 +869,188 @@ int32_t nkc_set_materials(nkc_handle *h, const double *eps, const double *mu) {
     return drop_graphs(h);
 }
 
+namespace {
+
+int32_t dev_alloc_counted(nkc_handle *h, double **p, size_t n) {
+    h->adj.bytes += sizeof(double) * n;
+    return dev_alloc(p, n);
+}
+
+// Every buffer of the adjoint for the layout fixed by nkc_adjoint_forward:
+// segment checkpoints, window starts, the stage store of one window, the
+// adjoint state and gradients.  The window S is the largest (at most every)
+// whose stage store fits in 70 % of the free memory left after the rest.
+int32_t alloc_adjoint(nkc_handle *h) {
+    auto &A = h->adj;
+    const DevMesh &m = h->m;
+    const size_t blk = 6 * (size_t)m.fs, jsz = drude_size(h);
+    int32_t rc;
+    A.bytes = 0;
+    if ((rc = dev_alloc_counted(h, &A.ckpt, blk * A.nseg))) return rc;
+    if ((rc = dev_alloc_counted(h, &A.ufin, blk))) return rc;
+    if ((rc = dev_alloc_counted(h, &A.kfin, blk))) return rc;
+    for (int w = 0; w < 2; ++w) {
+        if ((rc = dev_alloc_counted(h, &A.lam[w], blk))) return rc;
+        if ((rc = dev_alloc_counted(h, &A.kap[w], blk))) return rc;
+    }
+    if ((rc = dev_alloc_counted(h, &A.geps, m.fs))) return rc;
+    if ((rc = dev_alloc_counted(h, &A.gmu, m.fs))) return rc;
+    if (h->ndrude) {
+        if ((rc = dev_alloc_counted(h, &A.ckJ, jsz * A.nseg))) return rc;
+        if ((rc = dev_alloc_counted(h, &A.Jfin, jsz))) return rc;
+        if ((rc = dev_alloc_counted(h, &A.kJfin, jsz))) return rc;
+        if ((rc = dev_alloc_counted(h, &A.lamJ, jsz))) return rc;
+        if ((rc = dev_alloc_counted(h, &A.kapJ, jsz))) return rc;
+        if ((rc = dev_alloc_counted(h, &A.gA, jsz / 3))) return rc;
+        if ((rc = dev_alloc_counted(h, &A.gB, jsz / 3))) return rc;
+    }
+    const size_t per_step = sizeof(double) * 5 * (2 * blk + jsz);
+    int64_t S = A.every;
+    if (h->adj_store_cap > 0) {
+        S = std::min(S, h->adj_store_cap);
+    } else {
+        size_t fr = 0, tot = 0;
+        HIPCHK(hipMemGetInfo(&fr, &tot));
+        const size_t state = sizeof(double) * (blk + jsz);
+        const size_t room = (size_t)(0.7 * (double)fr);
+        // S steps of store plus ceil(every / S) window starts
+        while (S > 1 && per_step * S + state * (size_t)((A.every + S - 1) / S + 1) > room) S = (S + 1) / 2;
+    }
+    A.S = S;
+    A.nsub_max = (A.every + S - 1) / S;
+    if ((rc = dev_alloc_counted(h, &A.st_u, blk * (5 * S + 1)))) return rc;
+    if ((rc = dev_alloc_counted(h, &A.st_r, blk * 5 * S))) return rc;
+    if ((rc = dev_alloc_counted(h, &A.sub, blk * A.nsub_max))) return rc;
+    if (h->ndrude) {
+        if ((rc = dev_alloc_counted(h, &A.stJ, jsz * 5 * S))) return rc;
+        if ((rc = dev_alloc_counted(h, &A.subJ, jsz * A.nsub_max))) return rc;
+    }
+    return NKC_OK;
+}
+
+hipError_t d2d(nkc_handle *h, double *dst, const double *src, size_t n) {
+    if (n == 0) return hipSuccess;
+    return hipMemcpyAsync(dst, src, sizeof(double) * n, hipMemcpyDeviceToDevice, h->stream);
+}
+
+// Time after `step` steps from t0, summed as the forward sums it.
+double step_time(const nkc_handle *h, int64_t step) {
+    double t = h->adj.t0;
+    for (int64_t s = 0; s < step; ++s) t = t + h->adj.dt;
+    return t;
+}
+
+int32_t set_time(nkc_handle *h, double t) {
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipMemcpy(h->tdev, &t, sizeof(double), hipMemcpyHostToDevice));
+    h->t_host = t;
+    return NKC_OK;
+}
+
+// Steps in segment seg, in its window q, and windows in the segment.
+int64_t seg_steps(const nkc_handle *h, int64_t seg) {
+    return std::min(h->adj.every, h->adj.nsteps - seg * h->adj.every);
+}
+int64_t win_steps(const nkc_handle *h, int64_t seg, int64_t q) {
+    return std::min(h->adj.S, seg_steps(h, seg) - q * h->adj.S);
+}
+int64_t nwin(const nkc_handle *h, int64_t seg) { return (seg_steps(h, seg) + h->adj.S - 1) / h->adj.S; }
+
+// Window q of segment seg through the stage store, from the state now in
+// st_u slot 0 (and dJ) at the device time of its first step.
+int32_t store_window(nkc_handle *h, int64_t seg, int64_t q) {
+    int32_t rc;
+    const int64_t n = win_steps(h, seg, q);
+    for (int64_t s = 0; s < n; ++s) {
+        if ((rc = queue_step_store(h, h->adj.dt, s))) return rc;
+        h->t_host = h->t_host + h->adj.dt;
+    }
+    h->adj.stored_seg = seg;
+    h->adj.stored_sub = q;
+    return NKC_OK;
+}
+
+// Segment seg from the state in u[cur] (and dJ) at its start time: keep
+// every window's start state, run all windows but the last with the plain
+// step, and the last through the stage store.
+int32_t run_segment_windows(nkc_handle *h, int64_t seg) {
+    auto &A = h->adj;
+    const size_t blk = 6 * (size_t)h->m.fs, jsz = drude_size(h);
+    const int64_t nq = nwin(h, seg);
+    int32_t rc;
+    for (int64_t q = 0; q < nq; ++q) {
+        HIPCHK(d2d(h, A.sub + q * blk, h->m.u[h->cur], blk));
+        if (h->ndrude) HIPCHK(d2d(h, A.subJ + q * jsz, h->dJ, jsz));
+        if (q + 1 < nq) {
+            if ((rc = nkc_step(h, nullptr, A.dt, win_steps(h, seg, q)))) return rc;
+        } else {
+            HIPCHK(d2d(h, A.st_u, h->m.u[h->cur], blk));
+            if ((rc = store_window(h, seg, q))) return rc;
+        }
+    }
+    A.sub_seg = seg;
+    return NKC_OK;
+}
+
+// Reverse the stages of window q of segment seg (held by the stage store).
+int32_t reverse_window(nkc_handle *h, int64_t seg, int64_t q, int &li) {
+    auto &A = h->adj;
+    const DevMesh &m = h->m;
+    const size_t blk = 6 * (size_t)m.fs, jsz = drude_size(h);
+    int32_t rc;
+    for (int64_t s = win_steps(h, seg, q) - 1; s >= 0; --s)
+        for (int st = 4; st >= 0; --st) {
+            AdjArgs a{};
+            a.lam_in = A.lam[li];
+            a.kap_in = A.kap[li];
+            a.lam_out = A.lam[li ^ 1];
+            a.kap_out = A.kap[li ^ 1];
+            a.us = A.st_u + (s * 5 + st) * blk;
+            a.rs = A.st_r + (s * 5 + st) * blk;
+            a.geps = A.geps;
+            a.gmu = A.gmu;
+            a.rka = RK4A[st];
+            a.rkb = RK4B[st];
+            a.dt = A.dt;
+            a.C0 = h->C0;
+            a.yz = h->yz;
+            if (h->ndrude) {
+                a.drude_off = h->drude_off;
+                a.dpar = h->dpar;
+                a.lamJ = A.lamJ;
+                a.kapJ = A.kapJ;
+                a.stJ = A.stJ + (s * 5 + st) * jsz;
+                a.gA = A.gA;
+                a.gB = A.gB;
+            }
+            if ((rc = launch_adjoint(m, a, h->stream))) return rc;
+            li ^= 1;
+        }
+    return NKC_OK;
+}
+
+}  // namespace
+
+int32_t nkc_adjoint_set_store(nkc_handle *h, int64_t max_steps) {
+    if (!h) return fail(NKC_ERR_ARG, "null handle");
+    if (max_steps < 0) return fail(NKC_ERR_ARG, "max_steps < 0");
+    h->adj_store_cap = max_steps;
+    return NKC_OK;
+}
+
+int32_t nkc_adjoint_info(nkc_handle *h, int64_t *out) {
+    if (!h || !out) return fail(NKC_ERR_ARG, "null argument");
+    const auto &A = h->adj;
+    out[0] = A.every;
+    out[1] = A.nseg;
+    out[2] = A.S;
+    out[3] = (int64_t)A.bytes;
+    return NKC_OK;
+}
+
 int32_t nkc_adjoint_forward(nkc_handle *h, double *time, double dt, int64_t nsteps, int64_t every) {
     if (!h) return fail(NKC_ERR_ARG, "null handle");
     if (nsteps <= 0 || every <= 0 || !std::isfinite(dt)) return fail(NKC_ERR_ARG, "bad adjoint forward arguments");
-    if (h->ndrude) return fail(NKC_ERR_UNSUPPORTED, "adjoint with a Drude region is not built");
     free_adjoint(h);
     auto &A = h->adj;
     A.nsteps = nsteps;
@@ -852,19 +1058,33 @@ int32_t nkc_adjoint_forward(nkc_handle *h, double *time, double dt, int64_t nste
     A.nseg = (nsteps + A.every - 1) / A.every;
     A.dt = dt;
     A.t0 = time ? *time : h->t_host;
-    const size_t blk = 6 * (size_t)h->m.fs;
+    const size_t blk = 6 * (size_t)h->m.fs, jsz = drude_size(h);
     int32_t rc;
-    if ((rc = dev_alloc(&A.ckpt, blk * A.nseg))) return rc;
-    if ((rc = dev_alloc(&A.ufin, blk))) return rc;
-    double t = A.t0;
-    for (int64_t s = 0; s < A.nseg; ++s) {
-        HIPCHK(hipMemcpyAsync(A.ckpt + s * blk, h->m.u[h->cur], sizeof(double) * blk,
-                              hipMemcpyDeviceToDevice, h->stream));
-        const int64_t n = std::min(A.every, nsteps - s * A.every);
-        if ((rc = nkc_step(h, &t, dt, n))) return rc;
+    if ((rc = alloc_adjoint(h))) {
+        free_adjoint(h);
+        return rc;
     }
-    HIPCHK(hipMemcpyAsync(A.ufin, h->m.u[h->cur], sizeof(double) * blk, hipMemcpyDeviceToDevice, h->stream));
-    if (time) *time = t;
+    if (!(A.t0 == h->t_host) && (rc = set_time(h, A.t0))) return rc;
+    for (int64_t seg = 0; seg < A.nseg; ++seg) {
+        HIPCHK(d2d(h, A.ckpt + seg * blk, h->m.u[h->cur], blk));
+        if (h->ndrude) HIPCHK(d2d(h, A.ckJ + seg * jsz, h->dJ, jsz));
+        if (seg + 1 < A.nseg) {
+            if ((rc = nkc_step(h, nullptr, dt, A.every))) return rc;
+        } else {
+            // the last segment keeps its window starts and its last window's
+            // stages, so that the backward sweep starts without recomputing
+            if ((rc = run_segment_windows(h, seg))) return rc;
+        }
+    }
+    const int64_t nl = win_steps(h, A.nseg - 1, nwin(h, A.nseg - 1) - 1);
+    HIPCHK(d2d(h, h->m.u[h->cur], A.st_u + 5 * nl * blk, blk));
+    HIPCHK(d2d(h, A.ufin, A.st_u + 5 * nl * blk, blk));
+    HIPCHK(d2d(h, A.kfin, h->m.k, blk));
+    if (h->ndrude) {
+        HIPCHK(d2d(h, A.Jfin, h->dJ, jsz));
+        HIPCHK(d2d(h, A.kJfin, h->dkJ, jsz));
+    }
+    if (time) *time = h->t_host;
     return NKC_OK;
 }
 
@@ -875,64 +1095,79 @@ int32_t nkc_adjoint_backward(nkc_handle *h, const double *lamH, const double *la
     if (A.nseg == 0) return fail(NKC_ERR_ARG, "no nkc_adjoint_forward run to differentiate");
     DevMesh &m = h->m;
     if (ld < m.npts) return fail(NKC_ERR_ARG, "ld < npts");
-    const size_t blk = 6 * (size_t)m.fs;
+    const size_t blk = 6 * (size_t)m.fs, jsz = drude_size(h);
+    const double t_end = h->t_host;
     int32_t rc;
     if ((rc = ensure_yz(h))) return rc;
-    if (!A.st_u) {
-        if ((rc = dev_alloc(&A.st_u, blk * (5 * A.every + 1)))) return rc;
-        if ((rc = dev_alloc(&A.st_r, blk * 5 * A.every))) return rc;
-        for (int w = 0; w < 2; ++w) {
-            if ((rc = dev_alloc(&A.lam[w], blk))) return rc;
-            if ((rc = dev_alloc(&A.kap[w], blk))) return rc;
-        }
-        if ((rc = dev_alloc(&A.geps, m.fs))) return rc;
-        if ((rc = dev_alloc(&A.gmu, m.fs))) return rc;
-    }
     HIPCHK(hipStreamSynchronize(h->stream));
     HIPCHK(hipMemset(A.lam[0], 0, sizeof(double) * blk));
     HIPCHK(hipMemset(A.kap[0], 0, sizeof(double) * blk));
     HIPCHK(hipMemset(A.geps, 0, sizeof(double) * m.fs));
     HIPCHK(hipMemset(A.gmu, 0, sizeof(double) * m.fs));
+    if (h->ndrude) {
+        HIPCHK(hipMemset(A.lamJ, 0, sizeof(double) * jsz));
+        HIPCHK(hipMemset(A.kapJ, 0, sizeof(double) * jsz));
+        HIPCHK(hipMemset(A.gA, 0, sizeof(double) * jsz / 3));
+        HIPCHK(hipMemset(A.gB, 0, sizeof(double) * jsz / 3));
+    }
     if ((rc = copy_fields_h2d(A.lam[0], m.fs, 0, lamH, lamE, m.npts, ld))) return rc;
+    // the RK registers restart from zero at a segment or window start (the
+    // first stage multiplies them by a_1 = 0)
+    auto start_from = [&](const double *J, int64_t step) -> int32_t {
+        if (hipMemsetAsync(m.k, 0, sizeof(double) * blk, h->stream) != hipSuccess ||
+            (h->ndrude && (hipMemsetAsync(h->dkJ, 0, sizeof(double) * jsz, h->stream) != hipSuccess ||
+                           d2d(h, h->dJ, J, jsz) != hipSuccess)))
+            return fail(NKC_ERR_HIP, "adjoint restart copies");
+        return set_time(h, step_time(h, step));
+    };
     int li = 0;
     for (int64_t seg = A.nseg - 1; seg >= 0; --seg) {
-        const int64_t n = std::min(A.every, A.nsteps - seg * A.every);
-        // recompute the segment from its checkpoint, keeping every stage
-        HIPCHK(hipMemcpyAsync(A.st_u, A.ckpt + seg * blk, sizeof(double) * blk, hipMemcpyDeviceToDevice, h->stream));
-        HIPCHK(hipMemsetAsync(m.k, 0, sizeof(double) * blk, h->stream));
-        double tseg = A.t0;
-        for (int64_t s = 0; s < seg * A.every; ++s) tseg = tseg + A.dt;   // the forward's time sums
-        HIPCHK(hipMemcpyAsync(h->tdev, &tseg, sizeof(double), hipMemcpyHostToDevice, h->stream));
-        HIPCHK(hipStreamSynchronize(h->stream));
-        for (int64_t s = 0; s < n; ++s)
-            if ((rc = queue_step_store(h, A.dt, s))) return rc;
-        // reversed stages
-        for (int64_t s = n - 1; s >= 0; --s)
-            for (int st = 4; st >= 0; --st) {
-                AdjArgs a{};
-                a.lam_in = A.lam[li];
-                a.kap_in = A.kap[li];
-                a.lam_out = A.lam[li ^ 1];
-                a.kap_out = A.kap[li ^ 1];
-                a.us = A.st_u + (s * 5 + st) * blk;
-                a.rs = A.st_r + (s * 5 + st) * blk;
-                a.geps = A.geps;
-                a.gmu = A.gmu;
-                a.rka = RK4A[st];
-                a.rkb = RK4B[st];
-                a.dt = A.dt;
-                a.C0 = h->C0;
-                a.yz = h->yz;
-                if ((rc = launch_adjoint(m, a, h->stream))) return rc;
-                li ^= 1;
+        const int64_t nq = nwin(h, seg);
+        if (A.sub_seg != seg) {
+            HIPCHK(d2d(h, m.u[h->cur], A.ckpt + seg * blk, blk));
+            if ((rc = start_from(A.ckJ + seg * jsz, seg * A.every))) return rc;
+            if ((rc = run_segment_windows(h, seg))) return rc;
+        }
+        for (int64_t q = nq - 1; q >= 0; --q) {
+            if (!(A.stored_seg == seg && A.stored_sub == q)) {
+                HIPCHK(d2d(h, A.st_u, A.sub + q * blk, blk));
+                if ((rc = start_from(A.subJ + q * jsz, seg * A.every + q * A.S))) return rc;
+                if ((rc = store_window(h, seg, q))) return rc;
             }
+            if ((rc = reverse_window(h, seg, q, li))) return rc;
+        }
     }
-    // restore the forward's final state and time
-    HIPCHK(hipMemcpyAsync(m.u[h->cur], A.ufin, sizeof(double) * blk, hipMemcpyDeviceToDevice, h->stream));
-    HIPCHK(hipMemcpyAsync(h->tdev, &h->t_host, sizeof(double), hipMemcpyHostToDevice, h->stream));
-    HIPCHK(hipStreamSynchronize(h->stream));
+    // restore the forward's final state, RK registers and time
+    HIPCHK(d2d(h, m.u[h->cur], A.ufin, blk));
+    HIPCHK(d2d(h, m.k, A.kfin, blk));
+    if (h->ndrude) {
+        HIPCHK(d2d(h, h->dJ, A.Jfin, jsz));
+        HIPCHK(d2d(h, h->dkJ, A.kJfin, jsz));
+    }
+    if ((rc = set_time(h, t_end))) return rc;
+    A.have_grad = true;
     if (geps) HIPCHK(hipMemcpy(geps, A.geps, sizeof(double) * m.npts, hipMemcpyDeviceToHost));
     if (gmu) HIPCHK(hipMemcpy(gmu, A.gmu, sizeof(double) * m.npts, hipMemcpyDeviceToHost));
+    return NKC_OK;
+}
+
+int32_t nkc_adjoint_get_drude(nkc_handle *h, double *ga, double *gb) {
+    if (!h) return fail(NKC_ERR_ARG, "null handle");
+    if (h->ndrude == 0) return fail(NKC_ERR_ARG, "no Drude region set");
+    if (!h->adj.gA || !h->adj.have_grad) return fail(NKC_ERR_ARG, "no nkc_adjoint_backward run with this Drude region");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    const int64_t n3 = (int64_t)h->m.n1 * h->m.n1 * h->m.n1;
+    std::vector<double> buf(drude_size(h) / 3);
+    double *dst[2] = {ga, gb};
+    const double *srcp[2] = {h->adj.gA, h->adj.gB};
+    for (int w = 0; w < 2; ++w) {
+        if (!dst[w]) continue;
+        HIPCHK(hipMemcpy(buf.data(), srcp[w], sizeof(double) * buf.size(), hipMemcpyDeviceToHost));
+        for (int64_t l = 0; l < h->ndrude; ++l) {
+            const int64_t b = h->drude_slot[l];          // dd 3 n3 + vloc
+            dst[w][l] = buf[(b / (3 * n3)) * n3 + b % (3 * n3)];
+        }
+    }
     return NKC_OK;
 }
 

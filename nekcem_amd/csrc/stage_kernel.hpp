@@ -96,8 +96,9 @@ struct StageArgs {
     const double *__restrict__ dpar;
     double *__restrict__ dresJ;       // MODE_RESIDUAL: resJ out (same layout as J)
     // MODE_RK, adjoint recompute: keep the stage residual (6 fields, stride
-    // fs) when non-null
+    // fs) and the stage-input Drude current (layout of dJ) when non-null
     double *__restrict__ st_r;
+    double *__restrict__ st_J;
 };
 
 // n x v for n = sg * e_AX.
@@ -377,6 +378,7 @@ stage_kernel(StageArgs a) {
                     if (MODE == MODE_RESIDUAL) {
                         a.dresJ[jb + c * N3] = rJ;
                     } else {
+                        if (a.st_J) a.st_J[jb + c * N3] = Jc;
                         const double kj = a.rka * a.dkJ[jb + c * N3] + dt * rJ;
                         a.dkJ[jb + c * N3] = kj;
                         a.dJ[jb + c * N3] = Jc + a.rkb * kj;

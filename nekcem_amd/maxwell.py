@@ -227,6 +227,24 @@ class MaxwellSolver:
         _lib.check(self._L.nkc_adjoint_backward(self._h, _d(lamH), _d(lamE), n, _d(ge), _d(gm)))
         return ge, gm
 
+    def adjoint_set_store(self, max_steps):
+        """Cap the stage-store window of the adjoint (0: from free memory)."""
+        _lib.check(self._L.nkc_adjoint_set_store(self._h, int(max_steps)))
+
+    def adjoint_info(self):
+        """dict(every, segments, window, bytes) of the last adjoint_forward."""
+        out = (C.c_int64 * 4)()
+        _lib.check(self._L.nkc_adjoint_info(self._h, out))
+        return dict(every=out[0], segments=out[1], window=out[2], bytes=out[3])
+
+    def adjoint_drude_grad(self):
+        """(dL/da, dL/db) per listed Drude node (a = gamma, b = omega_p^2),
+        in the order given to set_drude, from the last adjoint_backward."""
+        nd = self.ndrude
+        ga, gb = np.empty(nd), np.empty(nd)
+        _lib.check(self._L.nkc_adjoint_get_drude(self._h, _d(ga), _d(gb)))
+        return ga, gb
+
     def adjoint_apply(self, kH, kE):
         """B^T P (kH, kE): transpose of the (source-free) residual operator."""
         n = self.npts

@@ -938,3 +938,34 @@ void orc_maxwell_vjp(const orc_mesh *m, double rktime, const double *eps, const 
     free(rH); free(rE); free(s); free(vH); free(vE); free(sb);
     free(Yb0); free(Yb1); free(Zb0); free(Zb1); free(zb); free(yb);
 }
+
+/* Drude part of the vector-Jacobian product (cem_maxwell_drude,
+ * cem_maxwell.F:3095-3147).  At the listed nodes i = dindex[l] the residual
+ * holds r_E += -ebm1 bm1 J and r_J = -a J + b E.  Given kt = (kE, kJ):
+ *   aE[i] += b kJ ;  aJ[l] += -ebm1 bm1 kE[i] - a kJ
+ *   ga[l] += -kJ.J ;  gb[l] += kJ.E[i]
+ *   geps[i] += kE.(ebm1 bm1 J)/eps[i]   (the eps-derivative of -J/eps)
+ * J, kJ, aJ: 3 blocks of ndrude; en, kE, aE: 3 blocks of npts. */
+void orc_drude_vjp(const orc_mesh *m, const double *eps, const double *en, const double *J,
+                   const double *kE, const double *kJ, double *aE, double *aJ, double *geps,
+                   double *ga, double *gb)
+{
+    int64_t nd = m->ndrude, npts = m->npts;
+    for (int64_t l = 0; l < nd; ++l) {
+        int64_t i = m->dindex[l];
+        double s = m->ebm1[i] * m->bm1[i];
+        double pa = m->drude_a[l], pb = m->drude_b[l];
+        double dJ = 0.0, dE = 0.0, dP = 0.0;
+        for (int c = 0; c < 3; ++c) {
+            double kj = kJ[c * nd + l], ke = kE[c * npts + i];
+            aE[c * npts + i] += pb * kj;
+            aJ[c * nd + l] += -s * ke - pa * kj;
+            dJ += kj * J[c * nd + l];
+            dE += kj * en[c * npts + i];
+            dP += ke * s * J[c * nd + l];
+        }
+        ga[l] -= dJ;
+        gb[l] += dE;
+        geps[i] += dP / eps[i];
+    }
+}

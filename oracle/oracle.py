@@ -73,6 +73,7 @@ def lib():
         L.orc_cem_error.argtypes = [_dp, _dp, _dp, C.c_int64, _dp, _dp]
         L.orc_maxwell_vjp.argtypes = [C.c_void_p, C.c_double, _dp, _dp, _bp, _dp, _dp, _dp, _dp,
                                       _dp, _dp, _dp, _dp]
+        L.orc_drude_vjp.argtypes = [C.c_void_p] + [_dp] * 10
         L.orc_num_threads.restype = C.c_int
         L.orc_set_num_threads.argtypes = [C.c_int]
         _lib = L
@@ -354,9 +355,12 @@ class BoxCase:
             out.update(J=J, kJ=kJ)
         return out
 
-    def vjp(self, H, E, kH, kE, t=0.0):
+    def vjp(self, H, E, kH, kE, t=0.0, J=None, kJ=None):
         """(aH, aE, geps, gmu): transpose of d residual/d(H,E) applied to
-        (kH, kE), and the eps/mu gradients of <(kH,kE), residual>."""
+        (kH, kE), and the eps/mu gradients of <(kH,kE), residual>.  With a
+        Drude region and (J, kJ) given, also (aJ, ga, gb): the transpose on
+        the polarisation current and the gradients with respect to the
+        per-node a (gamma) and b (omega_p^2)."""
         n = self.npts
         arr = lambda a: np.ascontiguousarray(a, dtype=np.float64).reshape(3, n)
         H, E, kH, kE = arr(H), arr(E), arr(kH), arr(kE)
@@ -365,22 +369,40 @@ class BoxCase:
         m = self._mesh()
         lib().orc_maxwell_vjp(C.byref(m), t, _p(self.eps), _p(self.mu), _p(self.pec_face),
                               _p(H), _p(E), _p(kH), _p(kE), _p(aH), _p(aE), _p(ge), _p(gm))
-        return aH, aE, ge, gm
+        if J is None:
+            return aH, aE, ge, gm
+        nd = len(self.dindex)
+        J = np.ascontiguousarray(J, dtype=np.float64).reshape(3 * nd)
+        kJ = np.ascontiguousarray(kJ, dtype=np.float64).reshape(3 * nd)
+        aJ, ga, gb = np.zeros(3 * nd), np.zeros(nd), np.zeros(nd)
+        lib().orc_drude_vjp(C.byref(m), _p(self.eps), _p(E), _p(J), _p(kE), _p(kJ), _p(aE), _p(aJ),
+                            _p(ge), _p(ga), _p(gb))
+        return aH, aE, ge, gm, aJ, ga, gb
 
-    def loss_and_grad(self, H0, E0, dt, nsteps, probes, target, t0=0.0):
+    def loss_and_grad(self, H0, E0, dt, nsteps, probes, target, t0=0.0, drude=False):
         """Probe loss L = sum_p (Ez(T)[p] - target[p])^2 after nsteps RK steps
         from (H0, E0), and its gradient with respect to eps and mu per node by
-        the discrete adjoint of the 2N-storage RK (every stage input kept)."""
+        the discrete adjoint of the 2N-storage RK (every stage input kept).
+        With drude=True the Drude current (zero at t0) is advanced too and
+        the gradients with respect to its per-node a and b are returned as
+        well: (loss, geps, gmu, (H, E), ga, gb)."""
         a, b, c = rk_coeffs()
         H = np.array(H0, dtype=np.float64).reshape(3, self.npts)
         E = np.array(E0, dtype=np.float64).reshape(3, self.npts)
         kH, kE = np.zeros_like(H), np.zeros_like(E)
+        nd = len(self.dindex) if drude else 0
+        J, kJ = np.zeros(3 * nd), np.zeros(3 * nd)
         stages = []
         t = t0
         for _ in range(nsteps):
             for i in range(5):
-                stages.append((H.copy(), E.copy(), t + dt * c[i]))
-                rH, rE = self.residual(H, E, t=t + dt * c[i])
+                stages.append((H.copy(), E.copy(), J.copy(), t + dt * c[i]))
+                if drude:
+                    rH, rE, rJ = self.residual(H, E, t=t + dt * c[i], J=J)
+                    kJ = a[i] * kJ + dt * rJ
+                    J = J + b[i] * kJ
+                else:
+                    rH, rE = self.residual(H, E, t=t + dt * c[i])
                 kH = a[i] * kH + dt * rH
                 kE = a[i] * kE + dt * rE
                 H = H + b[i] * kH
@@ -391,19 +413,31 @@ class BoxCase:
         lH, lE = np.zeros_like(H), np.zeros_like(E)
         np.add.at(lE[2], probes, 2.0 * diff)
         qH, qE = np.zeros_like(H), np.zeros_like(E)
+        lJ, qJ = np.zeros(3 * nd), np.zeros(3 * nd)
         ge, gm = np.zeros(self.npts), np.zeros(self.npts)
+        ga, gb = np.zeros(nd), np.zeros(nd)
         for sidx in range(len(stages) - 1, -1, -1):
             i = sidx % 5
-            Hs, Es, ts = stages[sidx]
+            Hs, Es, Js, ts = stages[sidx]
             tH = qH + b[i] * lH
             tE = qE + b[i] * lE
-            aH, aE, g1, g2 = self.vjp(Hs, Es, tH, tE, t=ts)
+            if drude:
+                tJ = qJ + b[i] * lJ
+                aH, aE, g1, g2, aJ, g3, g4 = self.vjp(Hs, Es, tH, tE, t=ts, J=Js, kJ=tJ)
+                lJ = lJ + dt * aJ
+                qJ = a[i] * tJ
+                ga += dt * g3
+                gb += dt * g4
+            else:
+                aH, aE, g1, g2 = self.vjp(Hs, Es, tH, tE, t=ts)
             lH = lH + dt * aH
             lE = lE + dt * aE
             qH = a[i] * tH
             qE = a[i] * tE
             ge += dt * g1
             gm += dt * g2
+        if drude:
+            return loss, ge, gm, (H, E), ga, gb
         return loss, ge, gm, (H, E)
 
     def cem_error(self, u, exact):

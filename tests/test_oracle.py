@@ -361,3 +361,121 @@ def test_rk_adjoint_gradient_matches_central_differences(oracle):
         d = rng.standard_normal(n)
         fd = (loss(eps + 1e-6 * d) - loss(eps - 1e-6 * d)) / 2e-6
         assert abs(fd - np.dot(ge, d)) <= 1e-6 * abs(fd)
+
+
+def _drude_region(c, rng):
+    """Listed nodes: every node of the elements whose centre lies in the
+    central half-size cube, with per-element a (gamma) and b (omega_p^2)."""
+    n3 = c.n1 ** 3
+    xc = c.xm1.reshape(-1, n3).mean(axis=1)
+    yc = c.ym1.reshape(-1, n3).mean(axis=1)
+    zc = c.zm1.reshape(-1, n3).mean(axis=1)
+    lo = [float(v.min()) for v in (c.xm1, c.ym1, c.zm1)]
+    hi = [float(v.max()) for v in (c.xm1, c.ym1, c.zm1)]
+    inside = np.ones(c.nelt, bool)
+    for v, a, b in zip((xc, yc, zc), lo, hi):
+        inside &= np.abs(v - 0.5 * (a + b)) < 0.25 * (b - a)
+    elems = np.nonzero(inside)[0]
+    dindex = (elems[:, None] * n3 + np.arange(n3)[None, :]).ravel().astype(np.int32)
+    ae = rng.uniform(0.05, 0.2, len(elems))
+    be = rng.uniform(0.5, 1.5, len(elems))
+    return dindex, np.repeat(ae, n3), np.repeat(be, n3), elems
+
+
+def test_drude_vjp_transpose_and_parameter_gradients(oracle):
+    """orc_drude_vjp with orc_maxwell_vjp is the exact transpose of the
+    residual in (H, E, J) (dot-product test, source off), and its a, b and
+    eps parts match central differences of <k, residual>: exact for a and b
+    (the residual is linear in them), 1e-7 for eps."""
+    rng = np.random.default_rng(5)
+    c = oracle.BoxCase(oracle.BoxSpec(nel=(3, 3, 3), N=3, lo=(0, 0, 0), hi=(1, 1, 1), bc=("PEC", "P", "P")))
+    n = c.npts
+    dindex, pa, pb, _ = _drude_region(c, rng)
+    assert 0 < len(dindex) < n
+    nd = len(dindex)
+    eps = rng.uniform(1, 3, n)
+    c.set_materials(eps, None)
+    c.set_drude(dindex, pa, pb)
+    H, E, J = rng.standard_normal((3, n)), rng.standard_normal((3, n)), rng.standard_normal(3 * nd)
+    kH, kE, kJ = rng.standard_normal((3, n)), rng.standard_normal((3, n)), rng.standard_normal(3 * nd)
+
+    def f():
+        rH, rE, rJ = c.residual(H, E, J=J)
+        return np.sum(kH * rH) + np.sum(kE * rE) + np.dot(kJ, rJ)
+
+    aH, aE, ge, gm, aJ, ga, gb = c.vjp(H, E, kH, kE, J=J, kJ=kJ)
+    lhs = f()
+    rhs = np.sum(aH * H) + np.sum(aE * E) + np.dot(aJ, J)
+    assert abs(lhs - rhs) <= 1e-13 * abs(lhs)
+    for which, g in (("a", ga), ("b", gb)):
+        d = rng.standard_normal(nd)
+        base = pa if which == "a" else pb
+        vals = []
+        for sgn in (1, -1):
+            p = base + sgn * 1e-3 * d
+            c.set_drude(dindex, p, pb) if which == "a" else c.set_drude(dindex, pa, p)
+            vals.append(f())
+        c.set_drude(dindex, pa, pb)
+        fd = (vals[0] - vals[1]) / 2e-3
+        assert abs(fd - np.dot(g, d)) <= 1e-10 * abs(fd), which
+    d = rng.standard_normal(n)
+    vals = []
+    for sgn in (1, -1):
+        c.set_materials(eps + sgn * 1e-6 * d, None)
+        vals.append(f())
+    c.set_materials(eps, None)
+    fd = (vals[0] - vals[1]) / 2e-6
+    assert abs(fd - np.dot(ge, d)) <= 1e-7 * abs(fd)
+
+
+def test_drude_rk_adjoint_gradient_matches_central_differences(oracle):
+    """Discrete adjoint of 20 RK45 steps with a Drude region and a Gaussian
+    source: gradients of the probe loss with respect to per-element gamma
+    and omega_p (through a = gamma, b = omega_p^2 at every node of the
+    element) and to per-node eps match central FD (h = 1e-6) of the
+    forward to 1e-6 relative."""
+    rng = np.random.default_rng(6)
+    c = oracle.BoxCase(oracle.BoxSpec(nel=(3, 3, 3), N=3, lo=(0, 0, 0), hi=(1, 1, 1), bc=("PEC",) * 3))
+    n, n3 = c.npts, c.n1 ** 3
+    x, y, z = c.xm1, c.ym1, c.zm1
+    dindex, _, _, elems = _drude_region(c, rng)
+    ne = len(elems)
+    gam = rng.uniform(0.05, 0.2, ne)
+    wp = rng.uniform(0.8, 1.2, ne)
+    eps = 1.5 + 0.2 * np.sin(np.pi * x) * np.cos(np.pi * y)
+    c.set_materials(eps, None)
+    c.set_source(np.exp(-((x - .5) ** 2 + (y - .5) ** 2 + (z - .5) ** 2) / (2 * .15 ** 2)) * c.bm1,
+                 (0, 0, 1), 0.05, 0.03)
+    probes = rng.choice(n, 8, replace=False)
+    dt, K = c.dt_from_cfl(0.25), 20
+    H0, E0 = np.zeros((3, n)), np.zeros((3, n))
+
+    def setp(g, w, e):
+        c.set_materials(e, None)
+        c.set_drude(dindex, np.repeat(g, n3), np.repeat(w * w, n3))
+
+    setp(gam * 1.3, wp * 0.8, eps)
+    target = c.run(H0, E0, dt, K)["E"][2, probes].copy()
+    setp(gam, wp, eps)
+    L, ge, gm, _, ga, gb = c.loss_and_grad(H0, E0, dt, K, probes, target, drude=True)
+    g_gam = ga.reshape(ne, n3).sum(axis=1)
+    g_wp = 2 * wp * gb.reshape(ne, n3).sum(axis=1)
+
+    def loss(g, w, e):
+        setp(g, w, e)
+        d = c.run(H0, E0, dt, K)["E"][2, probes] - target
+        return float(np.sum(d * d))
+
+    h = 1e-6
+    for which in ("gamma", "wp", "eps"):
+        d = rng.standard_normal(n if which == "eps" else ne)
+        if which == "gamma":
+            fd = (loss(gam + h * d, wp, eps) - loss(gam - h * d, wp, eps)) / (2 * h)
+            an = np.dot(g_gam, d)
+        elif which == "wp":
+            fd = (loss(gam, wp + h * d, eps) - loss(gam, wp - h * d, eps)) / (2 * h)
+            an = np.dot(g_wp, d)
+        else:
+            fd = (loss(gam, wp, eps + h * d) - loss(gam, wp, eps - h * d)) / (2 * h)
+            an = np.dot(ge, d)
+        assert abs(fd - an) <= 1e-6 * abs(fd), which
