@@ -189,6 +189,19 @@ int32_t nkc_info(nkc_handle *h, int64_t out[8]);
 /* Enable (1, default) or disable (0) hipGraph replay in nkc_step. */
 int32_t nkc_set_graph(nkc_handle *h, int32_t enable);
 
+/* Stage-kernel layout (no counterpart in the reference).  Bit 0: the four
+ * lane roles of an element get whole waves each (0) or their (N+1)^2
+ * lanes packed back to back in fewer waves (1).  Bit 1: the register
+ * allocator is left free (0) or held to 128 VGPRs for 4 waves per SIMD
+ * (1).  The first nkc_step times the layouts on the handle's mesh and keeps
+ * the fastest (an autotuner; a packed layout with the same workgroup as
+ * the unpacked one is not timed).  nkc_set_layout forces 0..3, or -1 to
+ * tune again at the next step; nkc_tune tunes now: out[5] = {us per stage
+ * for layouts 0..3 (NaN: not timed), chosen layout} (out may be NULL).
+ * All layouts compute the same arithmetic: results are bit-identical. */
+int32_t nkc_set_layout(nkc_handle *h, int32_t layout);
+int32_t nkc_tune(nkc_handle *h, double *out);
+
 /* ---- batches of independent cavities --------------------------------
  * Many independent box cavities of different sizes and polynomial orders
  * advance together: cavities of one order share one element table, each
@@ -212,6 +225,9 @@ int32_t nkc_batch_stream(nkc_batch *b, void **stream);
 /* out[0] = total points, out[1] = cavities, out[2] = polynomial orders
  * (launches per stage), out[3] = total elements. */
 int32_t nkc_batch_info(nkc_batch *b, int64_t out[8]);
+/* Per polynomial order of the batch (tuned at its first step, -1 before):
+ * out[2 i] = N, out[2 i + 1] = stage-kernel layout; cap = ints in out. */
+int32_t nkc_batch_layout(nkc_batch *b, int32_t *out, int64_t cap);
 
 /* Message of the last failed call on this thread ("" if none). */
 const char *nkc_last_error(void);

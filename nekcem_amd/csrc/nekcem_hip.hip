@@ -66,7 +66,11 @@ const double RK4C[5] = {0.0,
                         2802321613138.0 / 2924317926251.0};
 
 bool supported_n1(int n1) { return n1 >= 4 && n1 <= 10; }
-int block_threads(int n1) { return ((n1 * n1 + 63) / 64) * 64 * 4; }
+
+int32_t dev_alloc(double **p, size_t n) {
+    HIPCHK(hipMalloc((void **)p, sizeof(double) * n));
+    return NKC_OK;
+}
 
 template <class T>
 int32_t upload(T **dst, const std::vector<T> &src) {
@@ -84,6 +88,7 @@ struct DevMesh {
     double *D = nullptr, *geo = nullptr, *fco = nullptr, *ieps = nullptr, *imu = nullptr;
     double *dt_e = nullptr;       // per-element dt (batches), or null
     int32_t *nbr = nullptr;
+    int layout = -1;              // stage-kernel layout (PACK | OCC << 1); -1: not tuned yet
     std::vector<double> jac, w;   // host copies (source weights)
 };
 
@@ -119,35 +124,57 @@ void dev_mesh_free(DevMesh &dm) {
     dm = DevMesh();
 }
 
-template <int N1, int MAT, int MODE>
+// Stage-kernel layouts: bit 0 = PACK (lane roles packed), bit 1 = OCC
+// (4 waves/SIMD register budget).  The residual test entry uses layout 0.
+constexpr int NLAYOUT = 4;
+
+template <int N1, int MAT, int MODE, int PACK, int OCC>
 void launch_t(const StageArgs &a, int grid, hipStream_t s) {
-    hipLaunchKernelGGL((stage_kernel<N1, MAT, MODE>), dim3(grid), dim3(Shape<N1>::BS), 0, s, a);
+    hipLaunchKernelGGL((stage_kernel<N1, MAT, MODE, PACK, OCC>), dim3(grid), dim3(Shape<N1, PACK>::BS), 0, s, a);
+}
+
+template <int N1, int MAT, int PACK, int OCC>
+void launch_m(const StageArgs &a, int mode, int grid, hipStream_t s) {
+    if (mode == MODE_RK) launch_t<N1, MAT, MODE_RK, PACK, OCC>(a, grid, s);
+    else launch_t<N1, MAT, MODE_RESIDUAL, 0, 0>(a, grid, s);
+}
+
+template <int N1, int PACK, int OCC>
+void launch_p(const StageArgs &a, int mat, int mode, int grid, hipStream_t s) {
+    if (mat == MAT_ELEM) launch_m<N1, MAT_ELEM, PACK, OCC>(a, mode, grid, s);
+    else launch_m<N1, MAT_NODE, PACK, OCC>(a, mode, grid, s);
 }
 
 template <int N1>
-void launch_n(const StageArgs &a, int mat, int mode, int grid, hipStream_t s) {
-    if (mat == MAT_ELEM) {
-        if (mode == MODE_RK) launch_t<N1, MAT_ELEM, MODE_RK>(a, grid, s);
-        else launch_t<N1, MAT_ELEM, MODE_RESIDUAL>(a, grid, s);
-    } else {
-        if (mode == MODE_RK) launch_t<N1, MAT_NODE, MODE_RK>(a, grid, s);
-        else launch_t<N1, MAT_NODE, MODE_RESIDUAL>(a, grid, s);
+void launch_n(const StageArgs &a, int mat, int mode, int layout, int grid, hipStream_t s) {
+    switch (layout) {
+    case 1: launch_p<N1, 1, 0>(a, mat, mode, grid, s); break;
+    case 2: launch_p<N1, 0, 1>(a, mat, mode, grid, s); break;
+    case 3: launch_p<N1, 1, 1>(a, mat, mode, grid, s); break;
+    default: launch_p<N1, 0, 0>(a, mat, mode, grid, s); break;
     }
 }
 
-int32_t launch_stage(const DevMesh &dm, const StageArgs &a, int mode, hipStream_t s) {
+// Lanes a workgroup of the given layout has.
+int stage_block(int n1, int layout) {
+    const int n2 = n1 * n1;
+    return (layout & 1) ? (4 * n2 + 63) / 64 * 64 : 4 * ((n2 + 63) / 64 * 64);
+}
+
+int32_t launch_stage(const DevMesh &dm, const StageArgs &a, int mode, hipStream_t s, int layout = -2) {
     const int per_xcd = (int)((dm.nelt + 7) / 8);
     const int grid = 8 * per_xcd;
     StageArgs b = a;
     b.per_xcd = per_xcd;
+    const int pk = layout == -2 ? (dm.layout < 0 ? 0 : dm.layout) : layout;
     switch (dm.n1) {
-    case 4: launch_n<4>(b, dm.mat, mode, grid, s); break;
-    case 5: launch_n<5>(b, dm.mat, mode, grid, s); break;
-    case 6: launch_n<6>(b, dm.mat, mode, grid, s); break;
-    case 7: launch_n<7>(b, dm.mat, mode, grid, s); break;
-    case 8: launch_n<8>(b, dm.mat, mode, grid, s); break;
-    case 9: launch_n<9>(b, dm.mat, mode, grid, s); break;
-    case 10: launch_n<10>(b, dm.mat, mode, grid, s); break;
+    case 4: launch_n<4>(b, dm.mat, mode, pk, grid, s); break;
+    case 5: launch_n<5>(b, dm.mat, mode, pk, grid, s); break;
+    case 6: launch_n<6>(b, dm.mat, mode, pk, grid, s); break;
+    case 7: launch_n<7>(b, dm.mat, mode, pk, grid, s); break;
+    case 8: launch_n<8>(b, dm.mat, mode, pk, grid, s); break;
+    case 9: launch_n<9>(b, dm.mat, mode, pk, grid, s); break;
+    case 10: launch_n<10>(b, dm.mat, mode, pk, grid, s); break;
     default: return fail(NKC_ERR_UNSUPPORTED, "polynomial order N=%d not built", dm.n1 - 1);
     }
     HIPCHK(hipGetLastError());
@@ -166,6 +193,60 @@ StageArgs mesh_args(const DevMesh &dm) {
     a.fs = dm.fs;
     a.nelt = (int32_t)dm.nelt;
     return a;
+}
+
+// Autotuner over the stage kernel's layouts (lane packing x register
+// budget): time each on this mesh, alternating, best of three rounds of
+// four launches, and keep the fastest.  It writes only a scratch output and
+// a scratch RK register and leaves the source and the Drude current out, so
+// the state is untouched.  A packed layout whose workgroup equals the
+// unpacked one is not timed.
+int32_t tune_stage(DevMesh &dm, const double *uin, hipStream_t s, double us[NLAYOUT]) {
+    for (int l = 0; l < NLAYOUT; ++l) us[l] = NAN;
+    const bool same = stage_block(dm.n1, 0) == stage_block(dm.n1, 1);
+    const size_t blk = 6 * (size_t)dm.fs;
+    double *scr = nullptr;
+    int32_t rc = dev_alloc(&scr, 2 * blk);
+    if (rc) return rc;
+    StageArgs a = mesh_args(dm);
+    a.uin = uin;
+    a.uout = scr;
+    a.kbuf = scr + blk;
+    a.rka = 0.0;
+    a.rkb = RK4B[0];
+    a.dt = 1e-3;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    float best[NLAYOUT];
+    for (int l = 0; l < NLAYOUT; ++l) best[l] = INFINITY;
+    rc = NKC_OK;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = fail(NKC_ERR_HIP, "hipEventCreate");
+    for (int round = 0; round < 3 && rc == NKC_OK; ++round)
+        for (int p = 0; p < NLAYOUT && rc == NKC_OK; ++p) {
+            if (same && (p & 1)) continue;
+            if ((rc = launch_stage(dm, a, MODE_RK, s, p))) break;
+            if (hipEventRecord(e0, s) != hipSuccess) { rc = fail(NKC_ERR_HIP, "hipEventRecord"); break; }
+            for (int r = 0; r < 4 && rc == NKC_OK; ++r) rc = launch_stage(dm, a, MODE_RK, s, p);
+            if (rc) break;
+            float ms = 0.0f;
+            if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+                rc = fail(NKC_ERR_HIP, "tuner timing");
+                break;
+            }
+            best[p] = std::min(best[p], ms / 4.0f);
+        }
+    (void)hipStreamSynchronize(s);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(scr);
+    if (rc) return rc;
+    int pick = 0;
+    for (int l = 0; l < NLAYOUT; ++l) {
+        if (std::isfinite(best[l])) us[l] = 1e3 * best[l];
+        if (best[l] < best[pick]) pick = l;
+    }
+    dm.layout = pick;
+    return NKC_OK;
 }
 
 // time = time + dt at the end of a step (src/cem_drive.F:639), on the device
@@ -232,6 +313,7 @@ struct nkc_handle {
     double C0 = 1.0;
     double *yz = nullptr;                 // (sqrt(eps/mu), sqrt(mu/eps)) per ieps entry, adjoint only
     int64_t adj_store_cap = 0;            // cap on the adjoint's stage-store window (0: from free memory)
+    double tune_us[4] = {NAN, NAN, NAN, NAN};   // tuner's stage times per layout
     // adjoint: checkpoints of the state every `every` steps of the last
     // nkc_adjoint_forward, the stage store of one segment, adjoint buffers
     struct Adj {
@@ -474,11 +556,6 @@ int32_t queue_step_store(nkc_handle *h, double dt, int64_t slot) {
     return NKC_OK;
 }
 
-int32_t dev_alloc(double **p, size_t n) {
-    HIPCHK(hipMalloc((void **)p, sizeof(double) * n));
-    return NKC_OK;
-}
-
 }  // namespace
 
 // ===========================================================================
@@ -658,6 +735,10 @@ int32_t nkc_step(nkc_handle *h, double *time, double dt, int64_t nsteps) {
         HIPCHK(hipMemcpy(h->tdev, &t0, sizeof(double), hipMemcpyHostToDevice));
         h->t_host = t0;
     }
+    if (h->m.layout < 0) {
+        const int32_t rc = tune_stage(h->m, h->m.u[h->cur], h->stream, h->tune_us);
+        if (rc) return rc;
+    }
     if (h->use_graph) {
         if (!(h->graph_dt == dt)) {
             const int32_t rc = capture(h, dt);
@@ -833,7 +914,7 @@ int32_t nkc_info(nkc_handle *h, int64_t out[8]) {
     out[0] = h->m.npts;
     out[1] = h->m.mat;
     out[2] = 1;
-    out[3] = block_threads(n1);
+    out[3] = stage_block(n1, h->m.layout < 0 ? 0 : h->m.layout);
     out[4] = (int64_t)sizeof(double) * (6 * n1 * n1 * n1 + 25 * n1 * n1);
     out[5] = h->use_graph ? 1 : 0;
     out[6] = h->m.fs;
@@ -846,6 +927,27 @@ int32_t nkc_set_graph(nkc_handle *h, int32_t enable) {
     HIPCHK(hipStreamSynchronize(h->stream));
     h->use_graph = enable != 0;
     if (!h->use_graph) return drop_graphs(h);
+    return NKC_OK;
+}
+
+int32_t nkc_set_layout(nkc_handle *h, int32_t layout) {
+    if (!h) return fail(NKC_ERR_ARG, "null handle");
+    if (layout < -1 || layout >= NLAYOUT) return fail(NKC_ERR_ARG, "layout %d not in -1..%d", layout, NLAYOUT - 1);
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->m.layout = layout;
+    return drop_graphs(h);
+}
+
+int32_t nkc_tune(nkc_handle *h, double *out) {
+    if (!h) return fail(NKC_ERR_ARG, "null handle");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    int32_t rc = tune_stage(h->m, h->m.u[h->cur], h->stream, h->tune_us);
+    if (rc) return rc;
+    if ((rc = drop_graphs(h))) return rc;
+    if (out) {
+        for (int l = 0; l < NLAYOUT; ++l) out[l] = h->tune_us[l];
+        out[NLAYOUT] = h->m.layout;
+    }
     return NKC_OK;
 }
 
@@ -1336,6 +1438,12 @@ int32_t nkc_batch_step(nkc_batch *b, const double *dt, double *times, int64_t ns
         }
         for (int c = 0; c < ncav; ++c) b->cav[c].dt = dt[c];
     }
+    for (auto &dm : b->bins)
+        if (dm.layout < 0) {
+            double us[NLAYOUT];
+            const int32_t rc = tune_stage(dm, dm.u[b->cur], b->main, us);
+            if (rc) return rc;
+        }
     if (!b->graph_ok) {
         const int32_t rc = batch_capture(b);
         if (rc) return rc;
@@ -1372,6 +1480,17 @@ int32_t nkc_batch_info(nkc_batch *b, int64_t out[8]) {
     out[2] = (int64_t)b->bins.size();
     out[3] = nelt;
     out[4] = out[5] = out[6] = out[7] = 0;
+    return NKC_OK;
+}
+
+int32_t nkc_batch_layout(nkc_batch *b, int32_t *out, int64_t cap) {
+    if (!b || !out) return fail(NKC_ERR_ARG, "null batch or output");
+    const int64_t nb = (int64_t)b->bins.size();
+    if (cap < 2 * nb) return fail(NKC_ERR_ARG, "output holds %lld ints, %lld needed", (long long)cap, (long long)(2 * nb));
+    for (int64_t i = 0; i < nb; ++i) {
+        out[2 * i] = b->bins[i].n1 - 1;
+        out[2 * i + 1] = b->bins[i].layout;
+    }
     return NKC_OK;
 }
 

@@ -168,12 +168,16 @@ __device__ __forceinline__ void load_coef(const StageArgs &a, int e, int f, int 
 //   LDS element layout: u(i,j,k) of field f at f*N3 + k*N2 + j*N1 + (i ^ j)
 //   for power-of-two N1 (XOR swizzle: rows, columns and z-columns are all
 //   bank-conflict free), plain i otherwise.
-template <int N1>
+//   PACK = 0: each role owns whole waves (roundup(N2, 64) lanes); PACK = 1:
+//   the four roles' N2 lanes are packed back to back and the workgroup is
+//   roundup(4 N2, 64) lanes (fewer idle lanes when N2 is not a multiple of
+//   64, at the cost of waves that straddle two roles).
+template <int N1, int PACK = 0>
 struct Shape {
     static constexpr int N2 = N1 * N1, N3 = N2 * N1;
-    static constexpr int LANES = ((N2 + 63) / 64) * 64;
     static constexpr int ROLES = 4;
-    static constexpr int BS = LANES * ROLES;
+    static constexpr int LANES = PACK ? N2 : ((N2 + 63) / 64) * 64;
+    static constexpr int BS = PACK ? ((ROLES * N2 + 63) / 64) * 64 : LANES * ROLES;
     static constexpr int KH = (N1 + 1) / 2;           // levels in the lower half
     static constexpr bool SWZ = (N1 & (N1 - 1)) == 0;
     static constexpr int LDS_DOUBLES = 6 * N3 + 24 * N2 + N2;
@@ -188,10 +192,12 @@ struct Shape {
 #define NKC_OCC
 #endif
 
-template <int N1, int MAT, int MODE>
-__global__ void __launch_bounds__(Shape<N1>::BS) NKC_OCC
+// OCC = 1 asks the register allocator for 4 waves per SIMD (128 VGPRs,
+// spilling if it must); OCC = 0 leaves it free.  The autotuner picks.
+template <int N1, int MAT, int MODE, int PACK, int OCC>
+__global__ void __launch_bounds__((Shape<N1, PACK>::BS)) __attribute__((amdgpu_waves_per_eu(OCC ? 4 : 1)))
 stage_kernel(StageArgs a) {
-    using S = Shape<N1>;
+    using S = Shape<N1, PACK>;
     constexpr int N2 = S::N2, N3 = S::N3, KH = S::KH, BS = S::BS, LANES = S::LANES;
     __shared__ double s_u[6 * N3];      // element, swizzled
     __shared__ double s_f[24 * N2];     // [face 0..3][comp 0..5][a + n1 b]
@@ -201,10 +207,10 @@ stage_kernel(StageArgs a) {
     if (e >= a.nelt) return;   // whole workgroup leaves together
     const int t = threadIdx.x;
     const int role = t / LANES, q = t % LANES;
-    const bool act = q < N2;
+    const bool act = q < N2 && role < S::ROLES;
     const int i = q % N1, j = q / N1;
     const int grp = role & 1;
-    const int half = role >> 1;
+    const int half = (role >> 1) & 1;
     const int k0 = half ? KH : 0;
     const int64_t eo = (int64_t)e * N3;
     const int64_t fs = a.fs;
@@ -296,9 +302,14 @@ stage_kernel(StageArgs a) {
     const double *S0 = s_u + (src + 0) * N3;
     const double *S1 = s_u + (src + 1) * N3;
     const double *S2 = s_u + (src + 2) * N3;
-    double Dr[N1], Ds[N1];
+    // D rows of this lane's column cached in registers, except at high
+    // orders where 2 (N+1) doubles would cost occupancy: read from LDS
+    constexpr bool DREG = N1 <= 8;
+    double Dr[DREG ? N1 : 1], Ds[DREG ? N1 : 1];
+    if (DREG) {
 #pragma unroll
-    for (int l = 0; l < N1; ++l) { Dr[l] = s_D[i + N1 * l]; Ds[l] = s_D[j + N1 * l]; }
+        for (int l = 0; l < N1; ++l) { Dr[l] = s_D[i + N1 * l]; Ds[l] = s_D[j + N1 * l]; }
+    }
     double minv = 0.0;
     if (MAT == MAT_ELEM) minv = grp ? a.ieps[e] : a.imu[e];
     const double dt = a.dt_e ? a.dt_e[e] : a.dt;
@@ -313,10 +324,12 @@ stage_kernel(StageArgs a) {
         for (int l = 0; l < N1; ++l) {
             const int pr = S::idx(l, j, k), ps = S::idx(i, l, k), pt = S::idx(i, j, l);
             const double dkl = s_D[k + N1 * l];
-            S1_r += Dr[l] * S1[pr];
-            S2_r += Dr[l] * S2[pr];
-            S0_s += Ds[l] * S0[ps];
-            S2_s += Ds[l] * S2[ps];
+            const double drl = DREG ? Dr[DREG ? l : 0] : s_D[i + N1 * l];
+            const double dsl = DREG ? Ds[DREG ? l : 0] : s_D[j + N1 * l];
+            S1_r += drl * S1[pr];
+            S2_r += drl * S2[pr];
+            S0_s += dsl * S0[ps];
+            S2_s += dsl * S2[ps];
             S0_t += dkl * S0[pt];
             S1_t += dkl * S1[pt];
         }

@@ -110,6 +110,40 @@ module nekcem_hip
        type(c_ptr), value :: h, J, kJ, resJ
      end function nkc_get_drude
 
+     ! per-node permittivity and permeability, npts each
+     integer(c_int32_t) function nkc_set_materials(h, eps, mu) bind(C, name='nkc_set_materials')
+       import :: c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(in) :: eps(*), mu(*)
+     end function nkc_set_materials
+
+     ! discrete adjoint: checkpointed forward, then dL/deps, dL/dmu per
+     ! node from dL/dhn, dL/den at the final state (layout of hn/en, stride
+     ! ld); geps/gmu: c_loc of npts arrays or c_null_ptr
+     integer(c_int32_t) function nkc_adjoint_forward(h, time, dt, nsteps, every) &
+          bind(C, name='nkc_adjoint_forward')
+       import :: c_int32_t, c_int64_t, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(inout) :: time
+       real(c_double), value :: dt
+       integer(c_int64_t), value :: nsteps, every
+     end function nkc_adjoint_forward
+
+     integer(c_int32_t) function nkc_adjoint_backward(h, lamh, lame, ld, geps, gmu) &
+          bind(C, name='nkc_adjoint_backward')
+       import :: c_int32_t, c_int64_t, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(in) :: lamh(*), lame(*)
+       integer(c_int64_t), value :: ld
+       type(c_ptr), value :: geps, gmu
+     end function nkc_adjoint_backward
+
+     ! dL/dgamma, dL/d(omega_p^2) per listed Drude node (c_loc or c_null_ptr)
+     integer(c_int32_t) function nkc_adjoint_get_drude(h, ga, gb) bind(C, name='nkc_adjoint_get_drude')
+       import :: c_int32_t, c_ptr
+       type(c_ptr), value :: h, ga, gb
+     end function nkc_adjoint_get_drude
+
      integer(c_int32_t) function nkc_check_finite(h, maxabs) bind(C, name='nkc_check_finite')
        import :: c_int32_t, c_ptr, c_double
        type(c_ptr), value :: h

@@ -227,6 +227,16 @@ class MaxwellSolver:
         _lib.check(self._L.nkc_adjoint_backward(self._h, _d(lamH), _d(lamE), n, _d(ge), _d(gm)))
         return ge, gm
 
+    def set_layout(self, layout):
+        """Force the stage-kernel layout (0..3), or -1 to re-tune."""
+        _lib.check(self._L.nkc_set_layout(self._h, int(layout)))
+
+    def tune(self):
+        """Run the layout autotuner now: dict(us=(per layout 0..3), layout)."""
+        out = np.empty(5)
+        _lib.check(self._L.nkc_tune(self._h, _d(out)))
+        return dict(us=tuple(float(v) for v in out[:4]), layout=int(out[4]))
+
     def adjoint_set_store(self, max_steps):
         """Cap the stage-store window of the adjoint (0: from free memory)."""
         _lib.check(self._L.nkc_adjoint_set_store(self._h, int(max_steps)))
@@ -383,6 +393,13 @@ class BatchSolver:
         out = (C.c_int64 * 8)()
         _lib.check(self._L.nkc_batch_info(self._b, out))
         return dict(npts=out[0], ncav=out[1], norders=out[2], nelt=out[3])
+
+    def layout(self):
+        """{N: stage-kernel layout} per polynomial order (-1 before tuning)."""
+        nb = self.info()["norders"]
+        out = (C.c_int32 * (2 * nb))()
+        _lib.check(self._L.nkc_batch_layout(self._b, out, 2 * nb))
+        return {int(out[2 * i]): int(out[2 * i + 1]) for i in range(nb)}
 
     def close(self):
         if self._b:

@@ -233,8 +233,10 @@ def test_drude_adjoint_matches_oracle_and_fd(oracle):
     assert np.max(np.abs(gb - gb_ref)) <= 1e-9 * np.max(np.abs(gb_ref))
     g_gam = ga.reshape(ne, n3).sum(axis=1)
     g_wp = 2 * wp * gb.reshape(ne, n3).sum(axis=1)
-    # windows of 2 steps inside the 7-step segments: the same bits
+    # windows of 2 steps inside the 7-step segments: the same bits (from the
+    # same initial state: set_drude zeroes the current again)
     s.adjoint_set_store(2)
+    s.set_drude(dindex, np.repeat(gam, n3), np.repeat(wp * wp, n3))
     s.time = 0.0
     s.set_state(H0, E0)
     s.adjoint_forward(dt, K, every)

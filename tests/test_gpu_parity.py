@@ -274,3 +274,34 @@ def test_source_and_drude_steps_parity(oracle):
     assert np.sqrt(np.sum((gJ - oJ) ** 2)) <= 1e-11 * np.sqrt(np.sum(oJ ** 2))
     assert gpu.time == out["time"]
     gpu.close()
+
+
+@pytest.mark.parametrize("N", [4, 5, 6, 8, 9])
+def test_lane_layouts_bit_identical_and_tuner(N):
+    """All four stage-kernel layouts give the same bits (graph replay, 3
+    steps, per-element eps, periodic x, PEC y/z), and the tuner times them
+    and picks one."""
+    print(f"[case] lane layouts N={N}", flush=True)
+    from nekcem_amd.maxwell import MaxwellSolver
+    from nekcem_amd.mesh import BoxMesh
+    rng = np.random.default_rng(N)
+    mesh = BoxMesh((3, 2, 2), N, (0, 0, 0), (1.0, 0.8, 1.2), ("P", "PEC", "PEC"))
+    n = mesh.npts
+    eps = np.repeat(rng.uniform(1, 4, mesh.nelt), mesh.n1 ** 3)
+    H, E = rng.standard_normal((3, n)), rng.standard_normal((3, n))
+    dt = mesh.dt(0.25)
+    out = []
+    for layout in range(4):
+        s = MaxwellSolver(mesh, eps=eps)
+        s.set_layout(layout)
+        s.set_state(H, E)
+        s.step(dt, 3)
+        out.append(s.get_state())
+        if layout == 3:
+            t = s.tune()
+            assert np.isfinite(t["us"][0]) and np.isfinite(t["us"][2]) and t["layout"] in range(4)
+            assert s.get_state()[1].tobytes() == out[-1][1].tobytes()   # tuning leaves the state
+        s.close()
+    for o in out[1:]:
+        for a, b in zip(out[0], o):
+            assert np.array_equal(a, b)
