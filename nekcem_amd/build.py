@@ -20,6 +20,9 @@ HIPCC_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
     # keep hipcc from forming packed-FP32 instructions
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # fuse a*b+c only where one source expression says so: every kernel
+    # variant (layouts, batches) then rounds identically
+    "-ffp-contract=on",
     "-Wall", "-Wno-unused-function",
 ]
 

@@ -302,6 +302,6 @@ def test_lane_layouts_bit_identical_and_tuner(N):
             assert np.isfinite(t["us"][0]) and np.isfinite(t["us"][2]) and t["layout"] in range(4)
             assert s.get_state()[1].tobytes() == out[-1][1].tobytes()   # tuning leaves the state
         s.close()
-    for o in out[1:]:
+    for layout, o in enumerate(out[1:], start=1):
         for a, b in zip(out[0], o):
-            assert np.array_equal(a, b)
+            assert np.array_equal(a, b), (layout, float(np.max(np.abs(a - b))))
