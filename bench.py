@@ -201,6 +201,7 @@ def adjoint(nel=16, N=7, nsteps=500, every=50, ndir=32, nprobe=64, seed=4, log=s
     s.adjoint_forward(dt, nsteps, every)
     s.sync()
     t_fwd_ckpt = time.perf_counter() - t0
+    info = s.adjoint_info()
     Ez = s.get_state()[1][2, probes]
     diff = Ez - target
     loss = float(np.sum(diff * diff))
@@ -210,7 +211,7 @@ def adjoint(nel=16, N=7, nsteps=500, every=50, ndir=32, nprobe=64, seed=4, log=s
     ge, _ = s.adjoint_backward(np.zeros((3, n)), lamE)
     t_adj = time.perf_counter() - t0
     print(f"[bench] adjoint: forward {t_fwd:.3f} s, forward+checkpoints {t_fwd_ckpt:.3f} s, "
-          f"backward {t_adj:.3f} s, loss {loss:.4e}", file=log, flush=True)
+          f"backward {t_adj:.3f} s, loss {loss:.4e}, store {info}", file=log, flush=True)
     h = 1e-6
     errs = []
     for _ in range(ndir):
@@ -226,6 +227,7 @@ def adjoint(nel=16, N=7, nsteps=500, every=50, ndir=32, nprobe=64, seed=4, log=s
     return {"metric": "adjoint/fwd time", "value": t_adj / t_fwd, "unit": "x",
             "t_forward_s": t_fwd, "t_forward_checkpointed_s": t_fwd_ckpt, "t_adjoint_s": t_adj,
             "gradient_over_forward": (t_fwd_ckpt + t_adj) / t_fwd,
+            "adjoint_store": info,
             "fd_directions": ndir, "fd_h": h, "fd_max_rel_err": err, "loss": loss,
             "workload": f"PEC box {nel}^3 elements N={N} ({n} points), per-node eps = 2 + 0.3 s(x), "
                         f"Gaussian-pulse Ez source, {nprobe} probe nodes, {nsteps} steps, "
@@ -305,6 +307,8 @@ def dispersive(nel=24, N=9, nsteps=2000, every=100, ndir=2, nprobe=64, seed=5, l
     t_adj = time.perf_counter() - t0
     g_gam = np.bincount(owner, weights=ga, minlength=ne)
     g_wp = 2.0 * wp * np.bincount(owner, weights=gb, minlength=ne)
+    lay = s.layout()
+    print(f"[bench] dispersive: stage layout {lay}", file=log, flush=True)
     print(f"[bench] dispersive: forward {t_fwd:.3f} s ({nsteps} steps), forward+checkpoints "
           f"{t_fwd_ckpt:.3f} s, backward {t_adj:.3f} s, loss {loss:.4e}, adjoint {info}",
           file=log, flush=True)
@@ -326,6 +330,7 @@ def dispersive(nel=24, N=9, nsteps=2000, every=100, ndir=2, nprobe=64, seed=5, l
             "adjoint_over_forward": t_adj / t_fwd,
             "t_forward_s": t_fwd, "t_forward_checkpointed_s": t_fwd_ckpt, "t_adjoint_s": t_adj,
             "adjoint_window_steps": info["window"], "adjoint_device_bytes": info["bytes"],
+            "stage_layout": lay,
             "fd_directions": ndir, "fd_h": h, "fd_max_rel_err": err, "loss": loss,
             "workload": f"PEC box {nel}^3 elements N={N} ({n} points), Drude (omega_p=1, gamma=0.1) in "
                         f"{ne} central elements, vacuum elsewhere, Gaussian-pulse Ez source, "
@@ -400,6 +405,8 @@ def main():
 
     solver.step(dt, args.warmup)
     solver.sync()
+    layout = solver.layout()
+    print(f"[bench] stage layout (autotuned) {layout}", file=sys.stderr, flush=True)
     timer = StreamTimer(solver.stream())
     solver.sync()
     w0 = time.perf_counter()
@@ -472,7 +479,7 @@ def main():
                                f"{npts} GLL points x 6 fields, per-element eps U[1,4], "
                                f"8 plane waves, upwind, CFL 0.25, RK5 (5 stages/step)",
                    "nelt": mesh.nelt, "N": args.N, "npts": npts,
-                   "graph": not args.no_graph},
+                   "graph": not args.no_graph, "stage_layout": layout},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "nkc::stage_kernel (fused stage)",

@@ -150,7 +150,8 @@ int32_t nkc_set_materials(nkc_handle *h, const double *eps, const double *mu);
  * checkpointed and reversed too, and nkc_adjoint_get_drude returns the
  * gradients with respect to the per-node a (gamma) and b (omega_p^2) in the
  * order of the listed nodes; the loss is taken not to depend on J(T).
- * nkc_set_materials and nkc_set_drude discard a stored forward. */
+ * nkc_set_materials, nkc_set_source and nkc_set_drude discard a stored
+ * forward. */
 int32_t nkc_adjoint_forward(nkc_handle *h, double *time, double dt, int64_t nsteps,
                             int64_t every);
 int32_t nkc_adjoint_backward(nkc_handle *h, const double *lamH, const double *lamE,
@@ -161,7 +162,10 @@ int32_t nkc_adjoint_get_drude(nkc_handle *h, double *ga, double *gb);
  * recomputed from the segment checkpoint.  S is the largest that fits in
  * 70 % of the free device memory, or at most max_steps when that is > 0
  * (0 restores the default); it applies from the next nkc_adjoint_forward.
- * nkc_adjoint_info: out[4] = {every, segments, S, device bytes held}. */
+ * When whole segments fit (S = every, no cap), the forward keeps the stages
+ * of its last K segments, K as many as 80 % of the free memory holds, and
+ * the backward sweep recomputes only the others.
+ * nkc_adjoint_info: out[5] = {every, segments, S, K, device bytes held}. */
 int32_t nkc_adjoint_set_store(nkc_handle *h, int64_t max_steps);
 int32_t nkc_adjoint_info(nkc_handle *h, int64_t *out);
 /* aH, aE = B^T P (kH, kE) on the current state: the transpose of the
@@ -201,6 +205,9 @@ int32_t nkc_set_graph(nkc_handle *h, int32_t enable);
  * All layouts compute the same arithmetic: results are bit-identical. */
 int32_t nkc_set_layout(nkc_handle *h, int32_t layout);
 int32_t nkc_tune(nkc_handle *h, double *out);
+/* The current layout and the last tuner times without tuning: out[5] as
+ * for nkc_tune (layout -1 before the first step). */
+int32_t nkc_get_layout(nkc_handle *h, double *out);
 
 /* ---- batches of independent cavities --------------------------------
  * Many independent box cavities of different sizes and polynomial orders

@@ -79,6 +79,7 @@ SIGNATURES = [
     ("nkc_adjoint_get_drude", C.c_int32, [C.c_void_p, _dp, _dp]),
     ("nkc_set_layout", C.c_int32, [C.c_void_p, C.c_int32]),
     ("nkc_tune", C.c_int32, [C.c_void_p, _dp]),
+    ("nkc_get_layout", C.c_int32, [C.c_void_p, _dp]),
     ("nkc_batch_layout", C.c_int32, [C.c_void_p, C.POINTER(C.c_int32), C.c_int64]),
     ("nkc_adjoint_set_store", C.c_int32, [C.c_void_p, C.c_int64]),
     ("nkc_adjoint_info", C.c_int32, [C.c_void_p, C.POINTER(C.c_int64)]),

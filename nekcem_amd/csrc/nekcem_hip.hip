@@ -130,7 +130,10 @@ constexpr int NLAYOUT = 4;
 
 template <int N1, int MAT, int MODE, int PACK, int OCC>
 void launch_t(const StageArgs &a, int grid, hipStream_t s) {
-    hipLaunchKernelGGL((stage_kernel<N1, MAT, MODE, PACK, OCC>), dim3(grid), dim3(Shape<N1, PACK>::BS), 0, s, a);
+    if (a.src_off || a.drude_off)
+        hipLaunchKernelGGL((stage_kernel<N1, MAT, MODE, PACK, OCC, 1>), dim3(grid), dim3(Shape<N1, PACK>::BS), 0, s, a);
+    else
+        hipLaunchKernelGGL((stage_kernel<N1, MAT, MODE, PACK, OCC, 0>), dim3(grid), dim3(Shape<N1, PACK>::BS), 0, s, a);
 }
 
 template <int N1, int MAT, int PACK, int OCC>
@@ -198,17 +201,34 @@ StageArgs mesh_args(const DevMesh &dm) {
 // Autotuner over the stage kernel's layouts (lane packing x register
 // budget): time each on this mesh, alternating, best of three rounds of
 // four launches, and keep the fastest.  It writes only a scratch output and
-// a scratch RK register and leaves the source and the Drude current out, so
-// the state is untouched.  A packed layout whose workgroup equals the
-// unpacked one is not timed.
-int32_t tune_stage(DevMesh &dm, const double *uin, hipStream_t s, double us[NLAYOUT]) {
+// a scratch RK register, and with the handle's source and Drude region
+// (feat, may be null) it times the kernel instance the step runs, the Drude
+// current and its register redirected to scratch: the state is untouched.
+// A packed layout whose workgroup equals the unpacked one is not timed.
+int32_t tune_stage(DevMesh &dm, const StageArgs *feat, size_t jsz, const double *uin, hipStream_t s,
+                   double us[NLAYOUT]) {
     for (int l = 0; l < NLAYOUT; ++l) us[l] = NAN;
     const bool same = stage_block(dm.n1, 0) == stage_block(dm.n1, 1);
     const size_t blk = 6 * (size_t)dm.fs;
+    const size_t jx = (feat && feat->drude_off) ? jsz : 0;
     double *scr = nullptr;
-    int32_t rc = dev_alloc(&scr, 2 * blk);
+    int32_t rc = dev_alloc(&scr, 2 * blk + 2 * jx);
     if (rc) return rc;
     StageArgs a = mesh_args(dm);
+    if (feat) {
+        a.tdev = feat->tdev;
+        a.src_off = feat->src_off;
+        a.src_amp = feat->src_amp;
+        for (int c = 0; c < 3; ++c) a.src_dir[c] = feat->src_dir[c];
+        a.src_t0 = feat->src_t0;
+        a.src_tw = feat->src_tw;
+        if (jx) {
+            a.drude_off = feat->drude_off;
+            a.dpar = feat->dpar;
+            a.dJ = scr + 2 * blk;
+            a.dkJ = scr + 2 * blk + jx;
+        }
+    }
     a.uin = uin;
     a.uout = scr;
     a.kbuf = scr + blk;
@@ -324,22 +344,27 @@ struct nkc_handle {
         // (5 S stages) and reversed.  S = every when the store fits.
         int64_t S = 0, nsub_max = 0;
         int64_t sub_seg = -1;             // segment whose window starts `sub` holds
-        int64_t stored_seg = -1, stored_sub = -1;   // window the stage store holds
+        // The store has K window slots; slot_id[k] = seg * nsub_max + q of
+        // the window slot k holds (-1: none).  K > 1 only when windows are
+        // whole segments: the forward then keeps its last K segments.
+        int64_t K = 1;
+        std::vector<int64_t> slot_id;
+        int cur_slot = 0;                 // slot queue_step_store writes
         size_t bytes = 0;                 // device bytes held by the adjoint
         bool have_grad = false;           // a backward sweep has filled the gradients
         double dt = 0.0, t0 = 0.0;
         double *ckpt = nullptr;           // nseg x 6 fs: the state at the start of each segment
         double *sub = nullptr;            // nsub_max x 6 fs: window start states
         double *subJ = nullptr;           // nsub_max x jsz
-        double *st_u = nullptr;           // (5 S + 1) x 6 fs: every stage input of a window
-        double *st_r = nullptr;           // 5 S x 6 fs: every stage residual
+        double *st_u = nullptr;           // K x (5 S + 1) x 6 fs: every stage input of a window
+        double *st_r = nullptr;           // K x 5 S x 6 fs: every stage residual
         double *lam[2] = {nullptr, nullptr}, *kap[2] = {nullptr, nullptr};
         double *geps = nullptr, *gmu = nullptr;    // fs each
         double *ufin = nullptr;           // final forward state (restored after backward)
         double *kfin = nullptr;           // final RK register
         // Drude region (jsz = ndelem 3 n3 doubles per current block)
         double *ckJ = nullptr;            // nseg x jsz: the current at each segment start
-        double *stJ = nullptr;            // 5 S x jsz: every stage-input current
+        double *stJ = nullptr;            // K x 5 S x jsz: every stage-input current
         double *lamJ = nullptr, *kapJ = nullptr;   // adjoint current and its RK register
         double *gA = nullptr, *gB = nullptr;       // ndelem n3 each: d/da, d/db per node
         double *Jfin = nullptr, *kJfin = nullptr;  // final current and its RK register
@@ -537,17 +562,21 @@ int32_t launch_adjoint(const DevMesh &dm, const AdjArgs &a0, hipStream_t s) {
 // the store offsets change with the slot).
 int32_t queue_step_store(nkc_handle *h, double dt, int64_t slot) {
     const int64_t blk = 6 * h->m.fs;
+    auto &A = h->adj;
+    double *su = A.st_u + (int64_t)A.cur_slot * (5 * A.S + 1) * blk;
+    double *sr = A.st_r + (int64_t)A.cur_slot * 5 * A.S * blk;
+    double *sj = h->ndrude ? A.stJ + (int64_t)A.cur_slot * 5 * A.S * (int64_t)drude_size(h) : nullptr;
     for (int s = 0; s < 5; ++s) {
         StageArgs a = base_args(h);
-        a.uin = h->adj.st_u + (slot * 5 + s) * blk;
-        a.uout = h->adj.st_u + (slot * 5 + s + 1) * blk;
+        a.uin = su + (slot * 5 + s) * blk;
+        a.uout = su + (slot * 5 + s + 1) * blk;
         a.kbuf = h->m.k;
         a.rka = RK4A[s];
         a.rkb = RK4B[s];
         a.rkc = RK4C[s];
         a.dt = dt;
-        a.st_r = h->adj.st_r + (slot * 5 + s) * blk;
-        if (h->ndrude) a.st_J = h->adj.stJ + (slot * 5 + s) * drude_size(h);
+        a.st_r = sr + (slot * 5 + s) * blk;
+        if (h->ndrude) a.st_J = sj + (slot * 5 + s) * (int64_t)drude_size(h);
         const int32_t rc = launch_stage(h->m, a, MODE_RK, h->stream);
         if (rc) return rc;
     }
@@ -736,7 +765,8 @@ int32_t nkc_step(nkc_handle *h, double *time, double dt, int64_t nsteps) {
         h->t_host = t0;
     }
     if (h->m.layout < 0) {
-        const int32_t rc = tune_stage(h->m, h->m.u[h->cur], h->stream, h->tune_us);
+        const StageArgs f = base_args(h);
+        const int32_t rc = tune_stage(h->m, &f, drude_size(h), h->m.u[h->cur], h->stream, h->tune_us);
         if (rc) return rc;
     }
     if (h->use_graph) {
@@ -781,6 +811,8 @@ int32_t nkc_residual(nkc_handle *h, double rktime, double *reshn, double *resen,
 int32_t nkc_set_source(nkc_handle *h, const double *amp, const double dir[3], double t0, double tw) {
     if (!h) return fail(NKC_ERR_ARG, "null handle");
     HIPCHK(hipStreamSynchronize(h->stream));
+    free_adjoint(h);                      // the stored trajectory has the old source
+    h->m.layout = -1;                     // the step's kernel instance may change: tune again
     (void)hipFree(h->src_off); h->src_off = nullptr;
     (void)hipFree(h->src_amp); h->src_amp = nullptr;
     int32_t rc = drop_graphs(h);
@@ -823,6 +855,7 @@ int32_t nkc_set_drude(nkc_handle *h, int64_t ndrude, const int32_t *dindex, cons
     (void)hipFree(h->dpar); h->dpar = nullptr;
     (void)hipFree(h->dresJ); h->dresJ = nullptr;
     free_adjoint(h);                      // the stored trajectory has the old region
+    h->m.layout = -1;                     // the step's kernel instance may change: tune again
     h->ndrude = 0;
     h->ndelem = 0;
     h->drude_slot.clear();
@@ -938,10 +971,18 @@ int32_t nkc_set_layout(nkc_handle *h, int32_t layout) {
     return drop_graphs(h);
 }
 
+int32_t nkc_get_layout(nkc_handle *h, double *out) {
+    if (!h || !out) return fail(NKC_ERR_ARG, "null handle or output");
+    for (int l = 0; l < NLAYOUT; ++l) out[l] = h->tune_us[l];
+    out[NLAYOUT] = h->m.layout;
+    return NKC_OK;
+}
+
 int32_t nkc_tune(nkc_handle *h, double *out) {
     if (!h) return fail(NKC_ERR_ARG, "null handle");
     HIPCHK(hipStreamSynchronize(h->stream));
-    int32_t rc = tune_stage(h->m, h->m.u[h->cur], h->stream, h->tune_us);
+    const StageArgs f = base_args(h);
+    int32_t rc = tune_stage(h->m, &f, drude_size(h), h->m.u[h->cur], h->stream, h->tune_us);
     if (rc) return rc;
     if ((rc = drop_graphs(h))) return rc;
     if (out) {
@@ -1020,11 +1061,23 @@ int32_t alloc_adjoint(nkc_handle *h) {
     }
     A.S = S;
     A.nsub_max = (A.every + S - 1) / S;
-    if ((rc = dev_alloc_counted(h, &A.st_u, blk * (5 * S + 1)))) return rc;
-    if ((rc = dev_alloc_counted(h, &A.st_r, blk * 5 * S))) return rc;
+    // whole segments fit: keep as many of the last ones as 80 % of the free
+    // memory holds, so the backward sweep recomputes only the others
+    int64_t K = 1;
+    if (S == A.every && h->adj_store_cap == 0) {
+        size_t fr = 0, tot = 0;
+        HIPCHK(hipMemGetInfo(&fr, &tot));
+        const size_t win = per_step * S + sizeof(double) * blk;
+        K = std::max<int64_t>(1, std::min<int64_t>(A.nseg, (int64_t)((0.8 * (double)fr) / (double)win)));
+    }
+    A.K = K;
+    A.slot_id.assign(K, -1);
+    A.cur_slot = 0;
+    if ((rc = dev_alloc_counted(h, &A.st_u, blk * (5 * S + 1) * K))) return rc;
+    if ((rc = dev_alloc_counted(h, &A.st_r, blk * 5 * S * K))) return rc;
     if ((rc = dev_alloc_counted(h, &A.sub, blk * A.nsub_max))) return rc;
     if (h->ndrude) {
-        if ((rc = dev_alloc_counted(h, &A.stJ, jsz * 5 * S))) return rc;
+        if ((rc = dev_alloc_counted(h, &A.stJ, jsz * 5 * S * K))) return rc;
         if ((rc = dev_alloc_counted(h, &A.subJ, jsz * A.nsub_max))) return rc;
     }
     return NKC_OK;
@@ -1058,8 +1111,13 @@ int64_t win_steps(const nkc_handle *h, int64_t seg, int64_t q) {
 }
 int64_t nwin(const nkc_handle *h, int64_t seg) { return (seg_steps(h, seg) + h->adj.S - 1) / h->adj.S; }
 
-// Window q of segment seg through the stage store, from the state now in
-// st_u slot 0 (and dJ) at the device time of its first step.
+double *slot_u(nkc_handle *h, int k) {
+    return h->adj.st_u + (int64_t)k * (5 * h->adj.S + 1) * 6 * h->m.fs;
+}
+
+// Window q of segment seg through the stage store slot cur_slot, from the
+// state now in that slot's first stage input (and dJ) at the device time of
+// its first step.
 int32_t store_window(nkc_handle *h, int64_t seg, int64_t q) {
     int32_t rc;
     const int64_t n = win_steps(h, seg, q);
@@ -1067,9 +1125,15 @@ int32_t store_window(nkc_handle *h, int64_t seg, int64_t q) {
         if ((rc = queue_step_store(h, h->adj.dt, s))) return rc;
         h->t_host = h->t_host + h->adj.dt;
     }
-    h->adj.stored_seg = seg;
-    h->adj.stored_sub = q;
+    h->adj.slot_id[h->adj.cur_slot] = seg * h->adj.nsub_max + q;
     return NKC_OK;
+}
+
+int find_slot(const nkc_handle *h, int64_t seg, int64_t q) {
+    const int64_t id = seg * h->adj.nsub_max + q;
+    for (int k = 0; k < (int)h->adj.K; ++k)
+        if (h->adj.slot_id[k] == id) return k;
+    return -1;
 }
 
 // Segment seg from the state in u[cur] (and dJ) at its start time: keep
@@ -1086,7 +1150,8 @@ int32_t run_segment_windows(nkc_handle *h, int64_t seg) {
         if (q + 1 < nq) {
             if ((rc = nkc_step(h, nullptr, A.dt, win_steps(h, seg, q)))) return rc;
         } else {
-            HIPCHK(d2d(h, A.st_u, h->m.u[h->cur], blk));
+            A.cur_slot = 0;
+            HIPCHK(d2d(h, slot_u(h, 0), h->m.u[h->cur], blk));
             if ((rc = store_window(h, seg, q))) return rc;
         }
     }
@@ -1107,8 +1172,8 @@ int32_t reverse_window(nkc_handle *h, int64_t seg, int64_t q, int &li) {
             a.kap_in = A.kap[li];
             a.lam_out = A.lam[li ^ 1];
             a.kap_out = A.kap[li ^ 1];
-            a.us = A.st_u + (s * 5 + st) * blk;
-            a.rs = A.st_r + (s * 5 + st) * blk;
+            a.us = slot_u(h, A.cur_slot) + (s * 5 + st) * blk;
+            a.rs = A.st_r + ((int64_t)A.cur_slot * 5 * A.S + s * 5 + st) * blk;
             a.geps = A.geps;
             a.gmu = A.gmu;
             a.rka = RK4A[st];
@@ -1121,7 +1186,7 @@ int32_t reverse_window(nkc_handle *h, int64_t seg, int64_t q, int &li) {
                 a.dpar = h->dpar;
                 a.lamJ = A.lamJ;
                 a.kapJ = A.kapJ;
-                a.stJ = A.stJ + (s * 5 + st) * jsz;
+                a.stJ = A.stJ + ((int64_t)A.cur_slot * 5 * A.S + s * 5 + st) * jsz;
                 a.gA = A.gA;
                 a.gB = A.gB;
             }
@@ -1146,7 +1211,8 @@ int32_t nkc_adjoint_info(nkc_handle *h, int64_t *out) {
     out[0] = A.every;
     out[1] = A.nseg;
     out[2] = A.S;
-    out[3] = (int64_t)A.bytes;
+    out[3] = A.K;
+    out[4] = (int64_t)A.bytes;
     return NKC_OK;
 }
 
@@ -1167,20 +1233,29 @@ int32_t nkc_adjoint_forward(nkc_handle *h, double *time, double dt, int64_t nste
         return rc;
     }
     if (!(A.t0 == h->t_host) && (rc = set_time(h, A.t0))) return rc;
+    // the last K segments (K slots of whole segments), or the last segment's
+    // last window (K = 1), run through the stage store, so that the backward
+    // sweep starts without recomputing them
+    const int64_t first_kept = A.nseg - A.K;
+    const double *fin = nullptr;
     for (int64_t seg = 0; seg < A.nseg; ++seg) {
+        if (fin) HIPCHK(d2d(h, h->m.u[h->cur], fin, blk));   // state after the previous kept segment
         HIPCHK(d2d(h, A.ckpt + seg * blk, h->m.u[h->cur], blk));
         if (h->ndrude) HIPCHK(d2d(h, A.ckJ + seg * jsz, h->dJ, jsz));
-        if (seg + 1 < A.nseg) {
+        if (seg < first_kept) {
             if ((rc = nkc_step(h, nullptr, dt, A.every))) return rc;
+        } else if (A.K > 1) {
+            A.cur_slot = (int)(seg - first_kept);
+            HIPCHK(d2d(h, slot_u(h, A.cur_slot), h->m.u[h->cur], blk));
+            if ((rc = store_window(h, seg, 0))) return rc;
+            fin = slot_u(h, A.cur_slot) + 5 * seg_steps(h, seg) * blk;
         } else {
-            // the last segment keeps its window starts and its last window's
-            // stages, so that the backward sweep starts without recomputing
             if ((rc = run_segment_windows(h, seg))) return rc;
+            fin = slot_u(h, 0) + 5 * win_steps(h, seg, nwin(h, seg) - 1) * blk;
         }
     }
-    const int64_t nl = win_steps(h, A.nseg - 1, nwin(h, A.nseg - 1) - 1);
-    HIPCHK(d2d(h, h->m.u[h->cur], A.st_u + 5 * nl * blk, blk));
-    HIPCHK(d2d(h, A.ufin, A.st_u + 5 * nl * blk, blk));
+    HIPCHK(d2d(h, h->m.u[h->cur], fin, blk));
+    HIPCHK(d2d(h, A.ufin, fin, blk));
     HIPCHK(d2d(h, A.kfin, h->m.k, blk));
     if (h->ndrude) {
         HIPCHK(d2d(h, A.Jfin, h->dJ, jsz));
@@ -1225,17 +1300,27 @@ int32_t nkc_adjoint_backward(nkc_handle *h, const double *lamH, const double *la
     int li = 0;
     for (int64_t seg = A.nseg - 1; seg >= 0; --seg) {
         const int64_t nq = nwin(h, seg);
-        if (A.sub_seg != seg) {
-            HIPCHK(d2d(h, m.u[h->cur], A.ckpt + seg * blk, blk));
-            if ((rc = start_from(A.ckJ + seg * jsz, seg * A.every))) return rc;
-            if ((rc = run_segment_windows(h, seg))) return rc;
-        }
         for (int64_t q = nq - 1; q >= 0; --q) {
-            if (!(A.stored_seg == seg && A.stored_sub == q)) {
-                HIPCHK(d2d(h, A.st_u, A.sub + q * blk, blk));
-                if ((rc = start_from(A.subJ + q * jsz, seg * A.every + q * A.S))) return rc;
-                if ((rc = store_window(h, seg, q))) return rc;
+            int k = find_slot(h, seg, q);
+            if (k < 0 && nq > 1 && A.sub_seg != seg) {
+                // window starts of this segment (and its last window's stages)
+                HIPCHK(d2d(h, m.u[h->cur], A.ckpt + seg * blk, blk));
+                if ((rc = start_from(A.ckJ + seg * jsz, seg * A.every))) return rc;
+                if ((rc = run_segment_windows(h, seg))) return rc;
+                k = find_slot(h, seg, q);
             }
+            if (k < 0) {
+                // recompute the window into slot 0 (the kept segments, which
+                // come last in time, are reversed before it is reused)
+                A.cur_slot = 0;
+                const double *u0 = nq > 1 ? A.sub + q * blk : A.ckpt + seg * blk;
+                const double *J0 = nq > 1 ? A.subJ + q * jsz : A.ckJ + seg * jsz;
+                HIPCHK(d2d(h, slot_u(h, 0), u0, blk));
+                if ((rc = start_from(J0, seg * A.every + q * A.S))) return rc;
+                if ((rc = store_window(h, seg, q))) return rc;
+                k = 0;
+            }
+            A.cur_slot = k;
             if ((rc = reverse_window(h, seg, q, li))) return rc;
         }
     }
@@ -1441,7 +1526,7 @@ int32_t nkc_batch_step(nkc_batch *b, const double *dt, double *times, int64_t ns
     for (auto &dm : b->bins)
         if (dm.layout < 0) {
             double us[NLAYOUT];
-            const int32_t rc = tune_stage(dm, dm.u[b->cur], b->main, us);
+            const int32_t rc = tune_stage(dm, nullptr, 0, dm.u[b->cur], b->main, us);
             if (rc) return rc;
         }
     if (!b->graph_ok) {

@@ -194,7 +194,9 @@ struct Shape {
 
 // OCC = 1 asks the register allocator for 4 waves per SIMD (128 VGPRs,
 // spilling if it must); OCC = 0 leaves it free.  The autotuner picks.
-template <int N1, int MAT, int MODE, int PACK, int OCC>
+// FEAT = 1 compiles the in-stage source and Drude terms; FEAT = 0 leaves
+// them out, so the plain step carries none of their registers.
+template <int N1, int MAT, int MODE, int PACK, int OCC, int FEAT>
 __global__ void __launch_bounds__((Shape<N1, PACK>::BS)) __attribute__((amdgpu_waves_per_eu(OCC ? 4 : 1)))
 stage_kernel(StageArgs a) {
     using S = Shape<N1, PACK>;
@@ -237,6 +239,15 @@ stage_kernel(StageArgs a) {
                 const int k = k0 + kk;
                 kp[c][kk] = (k < N1) ? ld_stream(a.kbuf + (3 * grp + c) * fs + eo + k * N2 + q) : 0.0;
             }
+    }
+    // source amplitude of this role's levels (E roles), with the HBM
+    // round trip of the other streams
+    const int so = (FEAT && grp == 1 && a.src_off) ? a.src_off[e] : -1;
+    double samp[KH];
+#pragma unroll
+    for (int kk = 0; kk < KH; ++kk) {
+        const int k = k0 + kk;
+        samp[kk] = (so >= 0 && act && k < N1) ? a.src_amp[(int64_t)so * N3 + k * N2 + q] : 0.0;
     }
     // neighbour trace for this role's x/y face (node (a,b) = (i,j))
     double nxy[6];
@@ -315,10 +326,18 @@ stage_kernel(StageArgs a) {
     const double dt = a.dt_e ? a.dt_e[e] : a.dt;
     double cfz[4];
     load_coef<N1, MAT>(a, e, zface, q, cfz);
+    // usersrc time factor: one value per stage (and element, with per-element dt)
+    double sg = 0.0;
+    if (so >= 0) {
+        const double tt = (*a.tdev + a.rkc * dt - a.src_t0) / a.src_tw;
+        sg = exp(-tt * tt);
+    }
+    const int dd = (FEAT && grp == 1 && a.drude_off) ? a.drude_off[e] : -1;
 #pragma unroll
     for (int kk = 0; kk < KH; ++kk) {
         const int k = k0 + kk;
         if (k >= N1) break;
+        const int vloc = k * N2 + q;
         double S1_r = 0, S2_r = 0, S0_s = 0, S2_s = 0, S0_t = 0, S1_t = 0;
 #pragma unroll
         for (int l = 0; l < N1; ++l) {
@@ -363,39 +382,30 @@ stage_kernel(StageArgs a) {
 #pragma unroll
             for (int c = 0; c < 3; ++c) r[c] += s_f[(3 * 6 + co + c) * N2 + i + N1 * k];
         }
-        const int vloc = k * N2 + q;
-        if (grp == 1 && a.src_off) {
+        if (so >= 0) {
             // usersrc: resE += amp * bm1 * dir * g(t) before the inverse mass
-            const int so = a.src_off[e];
-            if (so >= 0) {
-                const double tt = (*a.tdev + a.rkc * a.dt - a.src_t0) / a.src_tw;
-                const double g = exp(-tt * tt);
-                const double am = a.src_amp[(int64_t)so * N3 + vloc] * g;
+            const double am = samp[kk] * sg;
 #pragma unroll
-                for (int c = 0; c < 3; ++c) r[c] += am * a.src_dir[c];
-            }
+            for (int c = 0; c < 3; ++c) r[c] += am * a.src_dir[c];
         }
-        if (grp == 1 && a.drude_off) {
+        if (dd >= 0) {
             // cem_maxwell_drude: resE -= J bm1; resJ = -a J + b E; J's RK
             // update uses the stage-start E and J
-            const int dd = a.drude_off[e];
-            if (dd >= 0) {
-                const int64_t jb = (int64_t)dd * 3 * N3 + vloc;
-                const double pa = a.dpar[((int64_t)dd * N3 + vloc) * 2 + 0];
-                const double pb = a.dpar[((int64_t)dd * N3 + vloc) * 2 + 1];
+            const int64_t jb = (int64_t)dd * 3 * N3 + vloc;
+            const double pa = a.dpar[((int64_t)dd * N3 + vloc) * 2 + 0];
+            const double pb = a.dpar[((int64_t)dd * N3 + vloc) * 2 + 1];
 #pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    const double Jc = a.dJ[jb + c * N3];
-                    const double rJ = -pa * Jc + pb * s_u[(3 + c) * N3 + own];
-                    r[c] -= Jc;
-                    if (MODE == MODE_RESIDUAL) {
-                        a.dresJ[jb + c * N3] = rJ;
-                    } else {
-                        if (a.st_J) a.st_J[jb + c * N3] = Jc;
-                        const double kj = a.rka * a.dkJ[jb + c * N3] + dt * rJ;
-                        a.dkJ[jb + c * N3] = kj;
-                        a.dJ[jb + c * N3] = Jc + a.rkb * kj;
-                    }
+            for (int c = 0; c < 3; ++c) {
+                const double Jc = a.dJ[jb + c * N3];
+                const double rJ = -pa * Jc + pb * s_u[(3 + c) * N3 + own];
+                r[c] -= Jc;
+                if (MODE == MODE_RESIDUAL) {
+                    a.dresJ[jb + c * N3] = rJ;
+                } else {
+                    if (a.st_J) a.st_J[jb + c * N3] = Jc;
+                    const double kj = a.rka * a.dkJ[jb + c * N3] + dt * rJ;
+                    a.dkJ[jb + c * N3] = kj;
+                    a.dJ[jb + c * N3] = Jc + a.rkb * kj;
                 }
             }
         }

@@ -231,6 +231,12 @@ class MaxwellSolver:
         """Force the stage-kernel layout (0..3), or -1 to re-tune."""
         _lib.check(self._L.nkc_set_layout(self._h, int(layout)))
 
+    def layout(self):
+        """dict(us=(tuner times per layout 0..3), layout) without tuning."""
+        out = np.empty(5)
+        _lib.check(self._L.nkc_get_layout(self._h, _d(out)))
+        return dict(us=tuple(float(v) for v in out[:4]), layout=int(out[4]))
+
     def tune(self):
         """Run the layout autotuner now: dict(us=(per layout 0..3), layout)."""
         out = np.empty(5)
@@ -242,10 +248,11 @@ class MaxwellSolver:
         _lib.check(self._L.nkc_adjoint_set_store(self._h, int(max_steps)))
 
     def adjoint_info(self):
-        """dict(every, segments, window, bytes) of the last adjoint_forward."""
-        out = (C.c_int64 * 4)()
+        """dict(every, segments, window, kept, bytes) of the last adjoint_forward
+        (kept: segments whose stages the forward kept)."""
+        out = (C.c_int64 * 5)()
         _lib.check(self._L.nkc_adjoint_info(self._h, out))
-        return dict(every=out[0], segments=out[1], window=out[2], bytes=out[3])
+        return dict(every=out[0], segments=out[1], window=out[2], kept=out[3], bytes=out[4])
 
     def adjoint_drude_grad(self):
         """(dL/da, dL/db) per listed Drude node (a = gamma, b = omega_p^2),
