@@ -365,6 +365,8 @@ def main():
     ap.add_argument("--adjoint-dirs", type=int, default=32,
                     help="FD directions of the adjoint check (0 = skip the adjoint measurement)")
     ap.add_argument("--adjoint-only", action="store_true")
+    ap.add_argument("--adjoint-steps", type=int, default=500)
+    ap.add_argument("--adjoint-every", type=int, default=50)
     ap.add_argument("--dispersive-dirs", type=int, default=2,
                     help="FD directions of the dispersive (Drude) adjoint check (-1 = skip the measurement)")
     ap.add_argument("--dispersive-steps", type=int, default=2000)
@@ -379,7 +381,8 @@ def main():
         print(json.dumps(batched(steps=args.batched_steps)), flush=True)
         return
     if args.adjoint_only:
-        print(json.dumps(adjoint(ndir=args.adjoint_dirs)), flush=True)
+        print(json.dumps(adjoint(nsteps=args.adjoint_steps, every=args.adjoint_every, ndir=args.adjoint_dirs)),
+              flush=True)
         return
     if args.dispersive_only:
         print(json.dumps(dispersive(nsteps=args.dispersive_steps, ndir=max(args.dispersive_dirs, 0))),
@@ -458,7 +461,8 @@ def main():
                   "norm": "per field, sqrt(sum bm1 (gpu-cpu)^2)/sqrt(sum bm1 cpu^2)"}
 
     bat = batched(steps=args.batched_steps) if args.batched_steps > 0 else None
-    adj = adjoint(ndir=args.adjoint_dirs) if args.adjoint_dirs > 0 else None
+    adj = (adjoint(nsteps=args.adjoint_steps, every=args.adjoint_every, ndir=args.adjoint_dirs)
+           if args.adjoint_dirs > 0 else None)
     disp = (dispersive(nsteps=args.dispersive_steps, ndir=args.dispersive_dirs)
             if args.dispersive_dirs >= 0 else None)
 
