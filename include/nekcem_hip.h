@@ -173,6 +173,27 @@ int32_t nkc_adjoint_info(nkc_handle *h, int64_t *out);
 int32_t nkc_adjoint_apply(nkc_handle *h, const double *kH, const double *kE, double *aH,
                           double *aE, int64_t ld);
 
+/* ---- on-device diagnostics (userchk without a host copy) ---------------
+ * Analytic solutions of the reference's test cases, evaluated on the device:
+ *   NKC_SOL_BOXPER: tests/3dboxper/3dboxper.usr:45-86, periodic (1,1,1)
+ *     mode, omega = sqrt(3);
+ *   NKC_SOL_BOXPEC: tests/3dboxpec/3dboxpec.usr:91-113, PEC cavity mode,
+ *     omega = pi sqrt(3).
+ * nkc_set_coords gives the node coordinates (xm1, ym1, zm1, npts each);
+ * box handles (nkc_create_box) have them already.  nkc_init_solution sets
+ * the state to the solution at `time` (userini; k := 0).  nkc_cem_error is
+ * cem_error (src/cem_common.F:1335-1355) of the state against the solution
+ * at `time`, nkc_cem_error_fields against caller-given exact fields (hn/en
+ * layout, stride ld): out[12] = mass-weighted L2 of Hx, Hy, Hz, Ex, Ey, Ez,
+ * then their Linf. */
+#define NKC_SOL_BOXPER 1
+#define NKC_SOL_BOXPEC 2
+int32_t nkc_set_coords(nkc_handle *h, const double *xm1, const double *ym1, const double *zm1);
+int32_t nkc_init_solution(nkc_handle *h, int32_t sol, double time);
+int32_t nkc_cem_error(nkc_handle *h, int32_t sol, double time, double *out);
+int32_t nkc_cem_error_fields(nkc_handle *h, const double *hx, const double *ex, int64_t ld,
+                             double *out);
+
 /* chk_blowup (src/cem_common.F:1150-1165): *maxabs = max |u| over the six
  * fields; returns NKC_ERR_BLOWUP if a value is non-finite or above 1e20. */
 int32_t nkc_check_finite(nkc_handle *h, double *maxabs);

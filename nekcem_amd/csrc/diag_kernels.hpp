@@ -1,0 +1,119 @@
+// diag_kernels.hpp -- on-device diagnostics: the analytic solutions of the
+// reference's own test cases (usersol) and cem_error's mass-weighted norms,
+// so that a checked run needs no host copy of the state.
+//
+//   cem_error (src/cem_common.F:1335-1355): err = exact - u,
+//     L2 = sqrt(sum(bm1 err^2) / sum(bm1)), Linf = max |err|, per field.
+//   3dboxper (tests/3dboxper/3dboxper.usr:45-86): periodic (1,1,1) mode,
+//     omega = sqrt(3).
+//   3dboxpec (tests/3dboxpec/3dboxpec.usr:91-113): PEC cavity mode on the
+//     unit cube, omega = pi sqrt(3).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace nkc {
+
+constexpr int SOL_FIELDS = 0;    // exact fields given by the caller
+constexpr int SOL_BOXPER = 1;
+constexpr int SOL_BOXPEC = 2;
+
+__device__ __forceinline__ void exact_solution(int sol, double t, double x, double y, double z, double H[3],
+                                               double E[3]) {
+    if (sol == SOL_BOXPER) {
+        const double om = sqrt(3.0);
+        const double th = sin(om * t) / om, te = cos(om * t);
+        const double sx = sin(x), cx = cos(x), sy = sin(y), cy = cos(y), sz = sin(z), cz = cos(z);
+        H[0] = 2.0 * cx * sy * cz * th;
+        H[1] = -sx * cy * cz * th;
+        H[2] = sx * sy * sz * th;
+        E[0] = 0.0;
+        E[1] = cx * sy * sz * te;
+        E[2] = cx * cy * cz * te;
+    } else {
+        const double ww = M_PI;
+        const double th = cos(ww * sqrt(3.0) * t) / sqrt(6.0), te = sin(ww * sqrt(3.0) * t) / sqrt(2.0);
+        const double sx = sin(ww * x), cx = cos(ww * x), sy = sin(ww * y), cy = cos(ww * y);
+        const double sz = sin(ww * z), cz = cos(ww * z);
+        H[0] = -sx * cy * cz * th;
+        H[1] = -cx * sy * cz * th;
+        H[2] = 2.0 * cx * cy * sz * th;
+        E[0] = -cx * sy * sz * te;
+        E[1] = sx * cy * sz * te;
+        E[2] = 0.0;
+    }
+}
+
+constexpr int DIAG_BS = 256;
+constexpr int DIAG_NV = 13;      // 6 sums of bm err^2, 6 maxima of |err|, sum of bm
+
+// Per-workgroup partial norms (grid-stride over the nodes, a fixed-order
+// tree in LDS), so the final sums do not depend on timing.
+__global__ void __launch_bounds__(DIAG_BS)
+error_partials(int sol, double t, const double *__restrict__ u, const double *__restrict__ exact, int64_t fs,
+               int64_t npts, int n1, const double *__restrict__ xyz, const double *__restrict__ jac,
+               const double *__restrict__ w, double *__restrict__ part) {
+    __shared__ double red[DIAG_NV][DIAG_BS];
+    const int n2 = n1 * n1, n3 = n2 * n1;
+    double acc[DIAG_NV];
+    for (int v = 0; v < DIAG_NV; ++v) acc[v] = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * DIAG_BS;
+    for (int64_t p = (int64_t)blockIdx.x * DIAG_BS + threadIdx.x; p < npts; p += stride) {
+        const int64_t e = p / n3;
+        const int r = (int)(p - e * n3);
+        const int i = r % n1, j = (r / n1) % n1, k = r / n2;
+        const double bm = jac[e] * (w[i] * w[j] * w[k]);
+        double ex[6];
+        if (sol == SOL_FIELDS) {
+            for (int c = 0; c < 6; ++c) ex[c] = exact[c * fs + p];
+        } else {
+            exact_solution(sol, t, xyz[p], xyz[fs + p], xyz[2 * fs + p], ex, ex + 3);
+        }
+        for (int c = 0; c < 6; ++c) {
+            const double d = ex[c] - u[c * fs + p];
+            acc[c] += d * bm * d;
+            acc[6 + c] = fmax(acc[6 + c], fabs(d));
+        }
+        acc[12] += bm;
+    }
+    for (int v = 0; v < DIAG_NV; ++v) red[v][threadIdx.x] = acc[v];
+    __syncthreads();
+    for (int h = DIAG_BS / 2; h > 0; h >>= 1) {
+        if (threadIdx.x < h)
+            for (int v = 0; v < DIAG_NV; ++v) {
+                const double o = red[v][threadIdx.x + h];
+                red[v][threadIdx.x] = (v >= 6 && v < 12) ? fmax(red[v][threadIdx.x], o) : red[v][threadIdx.x] + o;
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x < DIAG_NV) part[(int64_t)blockIdx.x * DIAG_NV + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// Sum (max) the partials in workgroup order: one thread per value.
+__global__ void error_final(const double *__restrict__ part, int nblk, double *__restrict__ out) {
+    const int v = threadIdx.x;
+    if (v >= DIAG_NV) return;
+    double s = 0.0;
+    for (int b = 0; b < nblk; ++b) {
+        const double x = part[(int64_t)b * DIAG_NV + v];
+        s = (v >= 6 && v < 12) ? fmax(s, x) : s + x;
+    }
+    out[v] = s;
+}
+
+// userini from an analytic solution: u := exact at time t, k := 0.
+__global__ void __launch_bounds__(DIAG_BS)
+init_solution(int sol, double t, double *__restrict__ u, double *__restrict__ kreg, int64_t fs, int64_t npts,
+              const double *__restrict__ xyz) {
+    const int64_t stride = (int64_t)gridDim.x * DIAG_BS;
+    for (int64_t p = (int64_t)blockIdx.x * DIAG_BS + threadIdx.x; p < npts; p += stride) {
+        double ex[6];
+        exact_solution(sol, t, xyz[p], xyz[fs + p], xyz[2 * fs + p], ex, ex + 3);
+        for (int c = 0; c < 6; ++c) {
+            u[c * fs + p] = ex[c];
+            kreg[c * fs + p] = 0.0;
+        }
+    }
+}
+
+}  // namespace nkc

@@ -20,6 +20,7 @@
 
 #include "../../include/nekcem_hip.h"
 #include "adjoint_kernel.hpp"
+#include "diag_kernels.hpp"
 #include "mesh_tables.hpp"
 #include "stage_kernel.hpp"
 
@@ -334,6 +335,13 @@ struct nkc_handle {
     double *yz = nullptr;                 // (sqrt(eps/mu), sqrt(mu/eps)) per ieps entry, adjoint only
     int64_t adj_store_cap = 0;            // cap on the adjoint's stage-store window (0: from free memory)
     double tune_us[4] = {NAN, NAN, NAN, NAN};   // tuner's stage times per layout
+    // on-device diagnostics: node coordinates (3 x fs; from the box or
+    // nkc_set_coords), element Jacobians and GLL weights (bm1 = jac w3),
+    // partial norms, the 13 final values
+    bool is_box = false;
+    int64_t box_nel[3] = {0, 0, 0};
+    double box_lo[3] = {0, 0, 0}, box_hi[3] = {0, 0, 0};
+    double *xyz = nullptr, *djac = nullptr, *dw = nullptr, *dpart = nullptr, *dout = nullptr;
     // adjoint: checkpoints of the state every `every` steps of the last
     // nkc_adjoint_forward, the stage store of one segment, adjoint buffers
     struct Adj {
@@ -473,7 +481,51 @@ void free_handle(nkc_handle *h) {
     (void)hipFree(h->src_off); (void)hipFree(h->src_amp);
     (void)hipFree(h->drude_off); (void)hipFree(h->dJ); (void)hipFree(h->dkJ);
     (void)hipFree(h->dpar); (void)hipFree(h->dresJ);
+    (void)hipFree(h->xyz); (void)hipFree(h->djac); (void)hipFree(h->dw);
+    (void)hipFree(h->dpart); (void)hipFree(h->dout);
     if (h->stream) (void)hipStreamDestroy(h->stream);
+}
+
+constexpr int DIAG_BLOCKS = 1024;
+
+int32_t ensure_diag(nkc_handle *h) {
+    int32_t rc;
+    if (!h->djac && (rc = upload(&h->djac, h->hm.jac))) return rc;
+    if (!h->dw && (rc = upload(&h->dw, h->hm.w))) return rc;
+    if (!h->dpart && (rc = dev_alloc(&h->dpart, (size_t)DIAG_BLOCKS * DIAG_NV))) return rc;
+    if (!h->dout && (rc = dev_alloc(&h->dout, DIAG_NV))) return rc;
+    return NKC_OK;
+}
+
+// Node coordinates on the device: a box handle builds them from its extents
+// (the GLL lines of genbox, element-major, i fastest); others need
+// nkc_set_coords.
+int32_t ensure_coords(nkc_handle *h) {
+    if (h->xyz) return NKC_OK;
+    if (!h->is_box) return fail(NKC_ERR_ARG, "no node coordinates: call nkc_set_coords first");
+    const int n1 = h->m.n1, n2 = n1 * n1, n3 = n2 * n1;
+    std::vector<double> z, w, D;
+    gll_basis(n1, z, w, D);
+    const int64_t fs = h->m.fs, npts = h->m.npts;
+    std::vector<double> xyz(3 * (size_t)fs, 0.0);
+    std::vector<double> line[3];
+    for (int a = 0; a < 3; ++a) {
+        const int64_t ne = h->box_nel[a];
+        line[a].resize(ne * n1);
+        const double lo = h->box_lo[a], len = h->box_hi[a] - h->box_lo[a];
+        for (int64_t ea = 0; ea < ne; ++ea) {
+            const double v0 = lo + len * (double)ea / (double)ne, v1 = lo + len * (double)(ea + 1) / (double)ne;
+            for (int i = 0; i < n1; ++i) line[a][ea * n1 + i] = v0 * (0.5 * (1.0 - z[i])) + v1 * (0.5 * (1.0 + z[i]));
+        }
+    }
+    for (int64_t p = 0; p < npts; ++p) {
+        const int64_t e = p / n3, r = p % n3;
+        const int64_t ex = e % h->box_nel[0], ey = (e / h->box_nel[0]) % h->box_nel[1], ez = e / (h->box_nel[0] * h->box_nel[1]);
+        xyz[p] = line[0][ex * n1 + r % n1];
+        xyz[fs + p] = line[1][ey * n1 + (r / n1) % n1];
+        xyz[2 * fs + p] = line[2][ez * n1 + r / n2];
+    }
+    return upload(&h->xyz, xyz);
 }
 
 int32_t make_handle(const HostMesh &hm, double C0, nkc_handle **out) {
@@ -707,9 +759,17 @@ int32_t nkc_create_box(const nkc_box_desc *d, nkc_handle **out) {
     const int64_t nelt = (int64_t)d->nel[0] * d->nel[1] * d->nel[2];
     if (nelt <= 0 || nelt * n1 * n1 * n1 > ((int64_t)1 << 31) - 1) return fail(NKC_ERR_ARG, "bad box size");
     HostMesh hm;
-    const int32_t rc = build_box(d, hm);
+    int32_t rc = build_box(d, hm);
     if (rc) return fail(rc, "%s", hm.err.c_str());
-    return make_handle(hm, d->ifupwind ? 1.0 : 0.0, out);
+    if ((rc = make_handle(hm, d->ifupwind ? 1.0 : 0.0, out))) return rc;
+    nkc_handle *h = *out;
+    h->is_box = true;
+    for (int a = 0; a < 3; ++a) {
+        h->box_nel[a] = d->nel[a];
+        h->box_lo[a] = d->lo[a];
+        h->box_hi[a] = d->hi[a];
+    }
+    return NKC_OK;
 }
 
 int32_t nkc_destroy(nkc_handle *h) {
@@ -911,6 +971,78 @@ int32_t nkc_get_drude(nkc_handle *h, double *J, double *kJ, double *resJ) {
             for (int c = 0; c < 3; ++c) dst[w][c * h->ndrude + l] = buf[h->drude_slot[l] + c * n3];
     }
     return NKC_OK;
+}
+
+int32_t nkc_set_coords(nkc_handle *h, const double *xm1, const double *ym1, const double *zm1) {
+    if (!h || !xm1 || !ym1 || !zm1) return fail(NKC_ERR_ARG, "null handle or coordinate array");
+    const int64_t fs = h->m.fs, npts = h->m.npts;
+    std::vector<double> xyz(3 * (size_t)fs, 0.0);
+    std::copy(xm1, xm1 + npts, xyz.begin());
+    std::copy(ym1, ym1 + npts, xyz.begin() + fs);
+    std::copy(zm1, zm1 + npts, xyz.begin() + 2 * fs);
+    HIPCHK(hipStreamSynchronize(h->stream));
+    (void)hipFree(h->xyz);
+    h->xyz = nullptr;
+    return upload(&h->xyz, xyz);
+}
+
+int32_t nkc_init_solution(nkc_handle *h, int32_t sol, double time) {
+    if (!h) return fail(NKC_ERR_ARG, "null handle");
+    if (sol != SOL_BOXPER && sol != SOL_BOXPEC) return fail(NKC_ERR_ARG, "unknown solution %d", sol);
+    int32_t rc;
+    if ((rc = ensure_coords(h))) return rc;
+    const DevMesh &m = h->m;
+    const int nb = (int)std::min<int64_t>(DIAG_BLOCKS, (m.npts + DIAG_BS - 1) / DIAG_BS);
+    hipLaunchKernelGGL(init_solution, dim3(nb), dim3(DIAG_BS), 0, h->stream, sol, time, m.u[h->cur], m.k, m.fs,
+                       m.npts, (const double *)h->xyz);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return NKC_OK;
+}
+
+namespace {
+
+int32_t run_error(nkc_handle *h, int sol, double time, const double *exact, double *out) {
+    int32_t rc;
+    if ((rc = ensure_diag(h))) return rc;
+    const DevMesh &m = h->m;
+    const int nb = (int)std::min<int64_t>(DIAG_BLOCKS, (m.npts + DIAG_BS - 1) / DIAG_BS);
+    hipLaunchKernelGGL(error_partials, dim3(nb), dim3(DIAG_BS), 0, h->stream, sol, time, (const double *)m.u[h->cur],
+                       exact, m.fs, m.npts, m.n1, (const double *)h->xyz, (const double *)h->djac,
+                       (const double *)h->dw, h->dpart);
+    hipLaunchKernelGGL(error_final, dim3(1), dim3(64), 0, h->stream, (const double *)h->dpart, nb, h->dout);
+    HIPCHK(hipGetLastError());
+    double v[DIAG_NV];
+    HIPCHK(hipMemcpyAsync(v, h->dout, sizeof v, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    for (int c = 0; c < 6; ++c) {
+        const double r = v[c] / v[12];
+        out[c] = r > 0.0 ? std::sqrt(r) : r;
+        out[6 + c] = v[6 + c];
+    }
+    return NKC_OK;
+}
+
+}  // namespace
+
+int32_t nkc_cem_error(nkc_handle *h, int32_t sol, double time, double *out) {
+    if (!h || !out) return fail(NKC_ERR_ARG, "null handle or output");
+    if (sol != SOL_BOXPER && sol != SOL_BOXPEC) return fail(NKC_ERR_ARG, "unknown solution %d", sol);
+    int32_t rc;
+    if ((rc = ensure_coords(h))) return rc;
+    return run_error(h, sol, time, nullptr, out);
+}
+
+int32_t nkc_cem_error_fields(nkc_handle *h, const double *hx, const double *ex, int64_t ld, double *out) {
+    if (!h || !hx || !ex || !out) return fail(NKC_ERR_ARG, "null argument");
+    const DevMesh &m = h->m;
+    if (ld < m.npts) return fail(NKC_ERR_ARG, "ld < npts");
+    double *tmp = nullptr;
+    int32_t rc = dev_alloc(&tmp, 6 * (size_t)m.fs);
+    if (rc) return rc;
+    if (!(rc = copy_fields_h2d(tmp, m.fs, 0, hx, ex, m.npts, ld))) rc = run_error(h, SOL_FIELDS, 0.0, tmp, out);
+    (void)hipFree(tmp);
+    return rc;
 }
 
 int32_t nkc_check_finite(nkc_handle *h, double *maxabs) {

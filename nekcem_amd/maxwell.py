@@ -227,6 +227,42 @@ class MaxwellSolver:
         _lib.check(self._L.nkc_adjoint_backward(self._h, _d(lamH), _d(lamE), n, _d(ge), _d(gm)))
         return ge, gm
 
+    # -- on-device diagnostics ------------------------------------------------
+    SOLUTIONS = {"3dboxper": 1, "3dboxpec": 2}
+
+    def _ensure_coords(self):
+        if getattr(self, "_coords_set", False) or isinstance(self.mesh, _BoxMeshInfo):
+            return
+        x, y, z = (np.ascontiguousarray(a, dtype=np.float64) for a in self.mesh.coords())
+        _lib.check(self._L.nkc_set_coords(self._h, _d(x), _d(y), _d(z)))
+        self._coords_set = True
+
+    def init_solution(self, sol, t=0.0):
+        """userini on the device: the state := analytic solution `sol`
+        ('3dboxper' or '3dboxpec') at time t, k := 0."""
+        self._ensure_coords()
+        _lib.check(self._L.nkc_init_solution(self._h, self.SOLUTIONS[sol], float(t)))
+        self.time = float(t)
+
+    def cem_error_device(self, sol, t=None):
+        """cem_error of the state against analytic solution `sol` at time t
+        (default: the solver's time), on the device: (L2[6], Linf[6]) in
+        the order Hx, Hy, Hz, Ex, Ey, Ez."""
+        self._ensure_coords()
+        out = np.empty(12)
+        _lib.check(self._L.nkc_cem_error(self._h, self.SOLUTIONS[sol], float(self.time if t is None else t),
+                                         _d(out)))
+        return out[:6], out[6:]
+
+    def cem_error_fields(self, H, E):
+        """cem_error of the state against given exact fields, on the device."""
+        n = self.npts
+        H = np.ascontiguousarray(H, dtype=np.float64).reshape(3, n)
+        E = np.ascontiguousarray(E, dtype=np.float64).reshape(3, n)
+        out = np.empty(12)
+        _lib.check(self._L.nkc_cem_error_fields(self._h, _d(H), _d(E), n, _d(out)))
+        return out[:6], out[6:]
+
     def set_layout(self, layout):
         """Force the stage-kernel layout (0..3), or -1 to re-tune."""
         _lib.check(self._L.nkc_set_layout(self._h, int(layout)))
