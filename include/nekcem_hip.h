@@ -89,6 +89,20 @@ int32_t nkc_create(const nkc_desc *d, nkc_handle **h);
 int32_t nkc_create_box(const nkc_box_desc *d, nkc_handle **h);
 
 /* Mirror of acc_copy_all_out (src/cem_drive.F:482). */
+/* A handle from the mesh arrays NekCEM's readers fill (rdmesh/rdbdry and
+ * bin_rd1_mesh/bin_rd1_bc, src/nek5_connect2.F:1774-2240): corners
+ * xc/yc/zc[8*nelt] (corners 1-4 on t = -1 counter-clockwise from
+ * (r,s) = (-1,-1), 5-8 above them), per face in Nek order (1: s=-1, 2: r=+1,
+ * 3: s=+1, 4: r=-1, 5: t=-1, 6: t=+1) cbc[(6*e + f)*3 .. +2] (3 characters)
+ * and bc[(6*e + f)*5 .. +4].  'E' and 'P' faces name their neighbour element
+ * and face in bc parameters 1-2 (1-based), or 0 to find an 'E' neighbour by
+ * position (a .re2 lists boundary faces only); 'PEC' is a perfect
+ * conductor.  Elements must be axis-aligned boxes; materials per element
+ * (NULL: 1).  Node coordinates for the on-device diagnostics are set with
+ * nkc_set_coords. */
+int32_t nkc_create_hex(int32_t N, int32_t ifupwind, int64_t nelt, const double *xc,
+                       const double *yc, const double *zc, const char *cbc, const double *bc,
+                       const double *eps_elem, const double *mu_elem, nkc_handle **h);
 int32_t nkc_destroy(nkc_handle *h);
 
 /* !$ACC UPDATE DEVICE / HOST of hn, en, khn, ken (src/EMWAVE:4-23).  Each is

@@ -84,6 +84,8 @@ SIGNATURES = [
     ("nkc_adjoint_set_store", C.c_int32, [C.c_void_p, C.c_int64]),
     ("nkc_adjoint_info", C.c_int32, [C.c_void_p, C.POINTER(C.c_int64)]),
     ("nkc_adjoint_apply", C.c_int32, [C.c_void_p, _dp, _dp, _dp, _dp, C.c_int64]),
+    ("nkc_create_hex", C.c_int32, [C.c_int32, C.c_int32, C.c_int64, _dp, _dp, _dp, C.c_char_p, _dp, _dp, _dp,
+                                   C.POINTER(C.c_void_p)]),
     ("nkc_set_coords", C.c_int32, [C.c_void_p, _dp, _dp, _dp]),
     ("nkc_init_solution", C.c_int32, [C.c_void_p, C.c_int32, C.c_double]),
     ("nkc_cem_error", C.c_int32, [C.c_void_p, C.c_int32, C.c_double, _dp]),

@@ -18,6 +18,9 @@
 #include <cstdint>
 #include <string>
 #include <vector>
+#include <cstdio>
+#include <tuple>
+#include <map>
 
 #include "../../include/nekcem_hip.h"
 
@@ -391,6 +394,158 @@ inline int32_t build_box(const nkc_box_desc *d, HostMesh &m) {
             m.nbr[e * 6 + kf] = nb;
             const double Y1 = Y0 - Ym, Z1 = Z0 - Zm;
             flux_coef(0.5 * Y0, Y1, 0.5 * Z0, Z1, C0, &m.fco[(e * 6 + kf) * 4]);
+        }
+    }
+    return NKC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// A mesh of axis-aligned box elements given the way NekCEM's readers leave
+// it (rdmesh / bin_rd1_mesh, rdbdry / bin_rd1_bc, src/nek5_connect2.F):
+// corners xc, yc, zc (8 per element, corners 1-4 on t = -1 counter-
+// clockwise from (r,s) = (-1,-1), then 5-8 on t = +1), and per face (Nek
+// face order 1: s = -1, 2: r = +1, 3: s = +1, 4: r = -1, 5: t = -1,
+// 6: t = +1) a 3-character boundary code and 5 parameters.  Codes: 'E'
+// interior and 'P' periodic, with the neighbour element and face in
+// parameters 1-2 (1-based) or 0 to find it by the face's position (a .re2
+// lists only boundary faces); 'PEC' a perfect conductor.  Each face must
+// pair with the opposite face of its neighbour over the same extent.
+// ---------------------------------------------------------------------------
+inline int32_t build_hex(int n1, int ifupwind, int64_t nelt, const double *xc, const double *yc,
+                         const double *zc, const char *cbc, const double *bc, const double *eps_elem,
+                         const double *mu_elem, HostMesh &m) {
+    static const int nek_to_kf[6] = {2, 1, 3, 0, 4, 5};   // Nek face f (0-based) -> x-,x+,y-,y+,z-,z+
+    char buf[256];
+    auto bad = [&](int32_t code, const std::string &msg) { m.err = msg; return code; };
+    const double C0 = ifupwind ? 1.0 : 0.0;
+    std::vector<double> z;
+    gll_basis(n1, z, m.w, m.D);
+    m.n1 = n1;
+    m.nelt = nelt;
+    m.mat = 0;
+    m.geo.assign(nelt * 9, 0.0);
+    m.nbr.assign(nelt * 6, HOST_FACE_NONE);
+    m.fco.assign(nelt * 6 * 4, 0.0);
+    m.ieps.assign(nelt, 0.0);
+    m.imu.assign(nelt, 0.0);
+    m.jac.assign(nelt, 0.0);
+    std::vector<double> lo(3 * nelt), hi(3 * nelt);
+    for (int64_t e = 0; e < nelt; ++e) {
+        const double *c[3] = {xc + 8 * e, yc + 8 * e, zc + 8 * e};
+        // corner q has r = -1 for q in {0,3,4,7}, s = -1 for {0,1,4,5}, t = -1 for {0..3}
+        static const int side[3][8] = {{0, 1, 1, 0, 0, 1, 1, 0}, {0, 0, 1, 1, 0, 0, 1, 1}, {0, 0, 0, 0, 1, 1, 1, 1}};
+        for (int a = 0; a < 3; ++a) {
+            double l = 0, h = 0;
+            for (int q = 0; q < 8; ++q) (side[a][q] ? h : l) += 0.25 * c[a][q];
+            const double tol = 1e-9 * std::max(1.0, std::fabs(h - l));
+            for (int q = 0; q < 8; ++q)
+                if (std::fabs(c[a][q] - (side[a][q] ? h : l)) > tol) {
+                    snprintf(buf, sizeof buf, "element %lld is not an axis-aligned box", (long long)e + 1);
+                    return bad(NKC_ERR_UNSUPPORTED, buf);
+                }
+            if (!(h > l)) {
+                snprintf(buf, sizeof buf, "element %lld has non-positive extent", (long long)e + 1);
+                return bad(NKC_ERR_ARG, buf);
+            }
+            lo[3 * e + a] = l;
+            hi[3 * e + a] = h;
+        }
+    }
+    auto eps_of = [&](int64_t e) { return eps_elem ? eps_elem[e] : 1.0; };
+    auto mu_of = [&](int64_t e) { return mu_elem ? mu_elem[e] : 1.0; };
+    for (int64_t e = 0; e < nelt; ++e)
+        if (!(eps_of(e) > 0.0) || !(mu_of(e) > 0.0)) return bad(NKC_ERR_ARG, "permittivity and permeability must be positive");
+    // faces by position for the links a .re2 leaves out: key = (axis, side, plane, centre)
+    auto face_key = [&](int64_t e, int kf) {
+        const int ax = kf / 2, sd = kf & 1;
+        const double plane = sd ? hi[3 * e + ax] : lo[3 * e + ax];
+        double cen[2];
+        int t = 0;
+        for (int a = 0; a < 3; ++a)
+            if (a != ax) cen[t++] = 0.5 * (lo[3 * e + a] + hi[3 * e + a]);
+        const double sc = 1e9;
+        return std::make_tuple(ax, sd, std::llround(plane * sc), std::llround(cen[0] * sc), std::llround(cen[1] * sc));
+    };
+    std::map<std::tuple<int, int, long long, long long, long long>, int64_t> faces;
+    for (int64_t e = 0; e < nelt; ++e)
+        for (int kf = 0; kf < 6; ++kf) faces[face_key(e, kf)] = e;
+    for (int64_t e = 0; e < nelt; ++e)
+        for (int f = 0; f < 6; ++f) {
+            const int kf = nek_to_kf[f], ax = kf / 2;
+            std::string code(cbc + (e * 6 + f) * 3, 3);
+            while (!code.empty() && code.back() == ' ') code.pop_back();
+            const double *p = bc + (e * 6 + f) * 5;
+            int32_t nb;
+            if (code == "PEC") {
+                nb = HOST_FACE_PEC;
+            } else if (code == "E" || code == "P") {
+                int64_t ne = (int64_t)std::llround(p[0]) - 1;
+                int nf = (int)std::llround(p[1]) - 1;
+                if (ne < 0) {
+                    if (code == "P") {
+                        snprintf(buf, sizeof buf, "periodic face %d of element %lld has no partner", f + 1, (long long)e + 1);
+                        return bad(NKC_ERR_ARG, buf);
+                    }
+                    auto k = face_key(e, kf);
+                    std::get<1>(k) ^= 1;   // the neighbour's opposite face at the same place
+                    auto it = faces.find(k);
+                    if (it == faces.end()) {
+                        snprintf(buf, sizeof buf, "no element meets face %d of element %lld", f + 1, (long long)e + 1);
+                        return bad(NKC_ERR_ARG, buf);
+                    }
+                    ne = it->second;
+                    nf = -1;
+                }
+                if (ne < 0 || ne >= nelt) {
+                    snprintf(buf, sizeof buf, "face %d of element %lld names element %lld", f + 1, (long long)e + 1, (long long)ne + 1);
+                    return bad(NKC_ERR_ARG, buf);
+                }
+                if (nf >= 0 && (nf > 5 || nek_to_kf[nf] != (kf ^ 1))) {
+                    snprintf(buf, sizeof buf, "face %d of element %lld pairs with face %d, not the opposite one", f + 1,
+                             (long long)e + 1, nf + 1);
+                    return bad(NKC_ERR_UNSUPPORTED, buf);
+                }
+                for (int a = 0; a < 3; ++a)
+                    if (a != ax && (std::fabs(lo[3 * e + a] - lo[3 * ne + a]) > 1e-9 * std::max(1.0, std::fabs(lo[3 * e + a])) ||
+                                    std::fabs(hi[3 * e + a] - hi[3 * ne + a]) > 1e-9 * std::max(1.0, std::fabs(hi[3 * e + a])))) {
+                        snprintf(buf, sizeof buf, "face %d of element %lld and its partner differ in extent", f + 1, (long long)e + 1);
+                        return bad(NKC_ERR_UNSUPPORTED, buf);
+                    }
+                nb = (int32_t)ne;
+            } else {
+                snprintf(buf, sizeof buf, "boundary code '%s' (element %lld face %d) is not on this path", code.c_str(),
+                         (long long)e + 1, f + 1);
+                return bad(NKC_ERR_UNSUPPORTED, buf);
+            }
+            m.nbr[e * 6 + kf] = nb;
+        }
+    const double w0 = m.w[0];
+    for (int64_t e = 0; e < nelt; ++e) {
+        const double xr = 0.5 * (hi[3 * e] - lo[3 * e]), ys = 0.5 * (hi[3 * e + 1] - lo[3 * e + 1]);
+        const double zt = 0.5 * (hi[3 * e + 2] - lo[3 * e + 2]);
+        const double J = xr * ys * zt, rx = ys * zt, sy = xr * zt, tz = xr * ys;
+        m.jac[e] = J;
+        double *g = &m.geo[e * 9];
+        g[0] = rx / J;
+        g[1] = sy / J;
+        g[2] = tz / J;
+        g[3] = g[4] = rx / (J * w0);
+        g[5] = g[6] = sy / (J * w0);
+        g[7] = g[8] = tz / (J * w0);
+        const double Zm = std::sqrt(mu_of(e) / eps_of(e)), Ym = std::sqrt(eps_of(e) / mu_of(e));
+        m.ieps[e] = 1.0 / eps_of(e);
+        m.imu[e] = 1.0 / mu_of(e);
+        for (int kf = 0; kf < 6; ++kf) {
+            const int32_t nb = m.nbr[e * 6 + kf];
+            double Y0 = Ym, Z0 = Zm;   // gs sums, as cem_maxwell_materials forms them
+            if (nb >= 0) {
+                Y0 += std::sqrt(eps_of(nb) / mu_of(nb));
+                Z0 += std::sqrt(mu_of(nb) / eps_of(nb));
+            } else if (nb == HOST_FACE_PEC) {
+                Y0 *= 2.0;
+                Z0 *= 2.0;
+            }
+            flux_coef(0.5 * Y0, Y0 - Ym, 0.5 * Z0, Z0 - Zm, C0, &m.fco[(e * 6 + kf) * 4]);
         }
     }
     return NKC_OK;

@@ -772,6 +772,21 @@ int32_t nkc_create_box(const nkc_box_desc *d, nkc_handle **out) {
     return NKC_OK;
 }
 
+int32_t nkc_create_hex(int32_t N, int32_t ifupwind, int64_t nelt, const double *xc, const double *yc,
+                       const double *zc, const char *cbc, const double *bc, const double *eps_elem,
+                       const double *mu_elem, nkc_handle **out) {
+    g_err.clear();
+    if (!out || !xc || !yc || !zc || !cbc || !bc) return fail(NKC_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (!supported_n1(N + 1)) return fail(NKC_ERR_UNSUPPORTED, "polynomial order N=%d outside 3..9", N);
+    const int64_t n1 = N + 1;
+    if (nelt <= 0 || nelt * n1 * n1 * n1 > ((int64_t)1 << 31) - 1) return fail(NKC_ERR_ARG, "bad element count");
+    HostMesh hm;
+    const int32_t rc = build_hex(N + 1, ifupwind, nelt, xc, yc, zc, cbc, bc, eps_elem, mu_elem, hm);
+    if (rc) return fail(rc, "%s", hm.err.c_str());
+    return make_handle(hm, ifupwind ? 1.0 : 0.0, out);
+}
+
 int32_t nkc_destroy(nkc_handle *h) {
     if (!h) return fail(NKC_ERR_ARG, "null handle");
     (void)hipStreamSynchronize(h->stream);

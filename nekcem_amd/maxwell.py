@@ -87,8 +87,89 @@ class _BoxMeshInfo:
         return abs(p12) if p12 < 0 else p12 * self.dxmin()
 
 
+class _HexMeshInfo:
+    """What a solver built from element corners knows of its mesh: per
+    element lower/upper corners of an axis-aligned box."""
+
+    def __init__(self, N, lo, hi):
+        from .mesh import gll
+        self.N, self.n1 = int(N), int(N) + 1
+        self.lo, self.hi = np.asarray(lo, dtype=np.float64), np.asarray(hi, dtype=np.float64)
+        self.nelt = len(self.lo)
+        self.npts = self.nelt * self.n1 ** 3
+        self._z, self._w = gll(self.n1)
+
+    def coords(self):
+        """xm1, ym1, zm1 in the reference volume order (element-major, i fastest)."""
+        n1, z = self.n1, self._z
+        t = 0.5 * (1.0 + z)
+        out = []
+        for a, shape in ((0, (1, 1, n1)), (1, (1, n1, 1)), (2, (n1, 1, 1))):
+            lo, hi = self.lo[:, a], self.hi[:, a]
+            v = lo[:, None] * (1.0 - t)[None, :] + hi[:, None] * t[None, :]
+            v = np.broadcast_to(v.reshape((self.nelt,) + shape), (self.nelt, n1, n1, n1))
+            out.append(np.ascontiguousarray(v).reshape(-1))
+        return out
+
+    @property
+    def bmn(self):
+        jac = np.prod(0.5 * (self.hi - self.lo), axis=1)
+        w3 = (self._w[None, None, :] * self._w[None, :, None] * self._w[:, None, None]).reshape(-1)
+        return (jac[:, None] * w3[None, :]).reshape(-1)
+
+    def dxmin(self):
+        """get_dxmin: half the smallest interior-node spacing over two GLL gaps."""
+        z = self._z
+        d = np.min((z[2:] - z[:-2]))
+        return float(0.5 * d * np.min(0.5 * (self.hi - self.lo)))
+
+    def dt(self, p12):
+        return abs(p12) if p12 < 0 else p12 * self.dxmin()
+
+
 class MaxwellSolver:
     """One nkc handle: the device-resident Maxwell state of a mesh."""
+
+    @classmethod
+    def from_corners(cls, N, xc, yc, zc, cbc, bc, eps_elem=None, mu_elem=None, ifupwind=True, graph=True):
+        """Handle built by nkc_create_hex from the arrays NekCEM's readers fill:
+        xc/yc/zc (nelt, 8), cbc (nelt, 6) 3-character codes, bc (nelt, 6, 5)."""
+        L = _lib.load()
+        xc, yc, zc = (np.ascontiguousarray(a, dtype=np.float64).reshape(-1, 8) for a in (xc, yc, zc))
+        nelt = xc.shape[0]
+        codes = "".join(str(c).ljust(3)[:3] for c in np.asarray(cbc).reshape(-1)).encode("ascii")
+        assert len(codes) == 18 * nelt
+        bcv = np.ascontiguousarray(bc, dtype=np.float64).reshape(nelt * 6 * 5)
+        ee = None if eps_elem is None else np.ascontiguousarray(eps_elem, dtype=np.float64)
+        mm = None if mu_elem is None else np.ascontiguousarray(mu_elem, dtype=np.float64)
+        h = C.c_void_p()
+        _lib.check(L.nkc_create_hex(int(N), 1 if ifupwind else 0, nelt, _d(xc), _d(yc), _d(zc), codes, _d(bcv),
+                                    _d(ee), _d(mm), C.byref(h)))
+        self = cls.__new__(cls)
+        lo = np.stack([xc.min(axis=1), yc.min(axis=1), zc.min(axis=1)], axis=1)
+        hi = np.stack([xc.max(axis=1), yc.max(axis=1), zc.max(axis=1)], axis=1)
+        self.mesh = _HexMeshInfo(N, lo, hi)
+        self.npts = self.mesh.npts
+        self._h, self._L = h, L
+        self.time, self.istep, self.dt, self.ndrude = 0.0, 0, None, 0
+        if not graph:
+            _lib.check(L.nkc_set_graph(h, 0))
+        return self
+
+    @classmethod
+    def from_rea(cls, rea_path, N, usrdat2=None, re2_path=None, **kw):
+        """Handle for a NekCEM case file (.rea, with its .re2 when the mesh is
+        there).  usrdat2(xc, yc, zc) -> (xc, yc, zc) may move the corners as
+        the case's usrdat2 moves the mesh.  The .rea parameters are kept in
+        ``param`` (1-based: param[11] = nsteps, param[12] = dt or CFL)."""
+        from .rea import read_case
+        r = read_case(rea_path, re2_path)
+        xc, yc, zc = r["xc"], r["yc"], r["zc"]
+        if usrdat2 is not None:
+            xc, yc, zc = usrdat2(xc, yc, zc)
+        self = cls.from_corners(N, xc, yc, zc, r["cbc"], r["bc"], **kw)
+        self.param = r["param"]
+        return self
 
     @classmethod
     def from_box(cls, nel, N, lo=(0.0, 0.0, 0.0), hi=(2 * np.pi,) * 3, bc=("P", "P", "P"),
