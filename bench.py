@@ -33,7 +33,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-B_ALG = 32.0   # bytes per GLL-pt.field.stage
+# Algorithmic HBM bytes per GLL-pt.field of each of the 5 stages of a
+# 2N-storage step: read u, read k, write k, write u (32 B), except that the
+# first stage need not read k (a_1 = 0) and the last need not write it (the
+# next step's first stage ignores it): 24 B.  Mean over a step: 28.8 B.
+B_STAGE = (24.0, 32.0, 32.0, 32.0, 24.0)
+B_ALG = sum(B_STAGE) / len(B_STAGE)
 
 
 def _hip():

@@ -196,7 +196,14 @@ StageArgs mesh_args(const DevMesh &dm) {
     a.dt_e = dm.dt_e;
     a.fs = dm.fs;
     a.nelt = (int32_t)dm.nelt;
+    a.kflags = 3;
     return a;
+}
+
+// RK-register traffic of stage s of a step: no read in the first stage
+// (a_1 = 0), no write in the last (the next step's first stage ignores it).
+int32_t stage_kflags(int s) {
+    return (s == 0 ? 0 : 1) | (s == 4 ? 0 : 2);
 }
 
 // Autotuner over the stage kernel's layouts (lane packing x register
@@ -412,6 +419,7 @@ int32_t queue_step(nkc_handle *h, int from, double dt) {
         a.uout = h->m.u[src ^ 1];
         a.kbuf = h->m.k;
         a.rka = RK4A[s];
+        a.kflags = stage_kflags(s);
         a.rkb = RK4B[s];
         a.rkc = RK4C[s];
         a.dt = dt;
@@ -624,6 +632,7 @@ int32_t queue_step_store(nkc_handle *h, double dt, int64_t slot) {
         a.uout = su + (slot * 5 + s + 1) * blk;
         a.kbuf = h->m.k;
         a.rka = RK4A[s];
+        a.kflags = stage_kflags(s);
         a.rkb = RK4B[s];
         a.rkc = RK4C[s];
         a.dt = dt;
@@ -685,6 +694,7 @@ int32_t batch_queue_step(nkc_batch *b, int from) {
             a.uout = dm.u[src ^ 1];
             a.kbuf = dm.k;
             a.rka = RK4A[s];
+            a.kflags = stage_kflags(s);
             a.rkb = RK4B[s];
             a.rkc = RK4C[s];
             a.dt = 0.0;              // per element from dt_e

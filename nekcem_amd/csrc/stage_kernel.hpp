@@ -79,6 +79,11 @@ struct StageArgs {
     int32_t nelt;
     int32_t per_xcd;                  // elements per XCD slice (grid remap)
     double rka, rkb, dt;
+    // MODE_RK: bit 0 reads the RK register, bit 1 writes it.  The first
+    // stage's a_1 = 0 makes the read moot and the next step's first stage
+    // ignores what the last stage would write, so a step's first stage
+    // skips the read and its last stage the write (10 % of a step's bytes).
+    int32_t kflags;
     const double *__restrict__ dt_e;  // per-element dt (batched cavities) or null
     // stage time = *tdev + rkc*dt (tdev advanced on the device after stage 5)
     const double *__restrict__ tdev;
@@ -231,7 +236,13 @@ stage_kernel(StageArgs a) {
     }
     // RK register of this role's fields and levels
     double kp[3][KH];
-    if (MODE == MODE_RK && act) {
+    if (MODE == MODE_RK && act && !(a.kflags & 1)) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int kk = 0; kk < KH; ++kk) kp[c][kk] = 0.0;
+    }
+    if (MODE == MODE_RK && act && (a.kflags & 1)) {
 #pragma unroll
         for (int c = 0; c < 3; ++c)
 #pragma unroll
@@ -422,7 +433,7 @@ stage_kernel(StageArgs a) {
             for (int c = 0; c < 3; ++c) {
                 const int64_t p = (co + c) * fs + node;
                 const double kk2 = a.rka * kp[c][kk] + dt * r[c];
-                st_stream(a.kbuf + p, kk2);
+                if (a.kflags & 2) st_stream(a.kbuf + p, kk2);
                 st_stream(a.uout + p, s_u[(co + c) * N3 + own] + a.rkb * kk2);
             }
             if (a.st_r) {
