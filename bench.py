@@ -293,6 +293,7 @@ def dispersive(nel=24, N=9, nsteps=2000, every=100, ndir=2, nprobe=64, seed=5, l
 
     target = forward(gam_t, wp_t)
     _, t_fwd = forward(gam, wp, timed=True)
+    s.adjoint_set_grads(False)      # the gradient asked for is w.r.t. gamma and omega_p only
     s.set_drude(dindex, gam[owner], (wp * wp)[owner])
     s.time = 0.0
     s.set_state(H0, E0)
@@ -361,7 +362,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--nel", type=int, default=32)
     ap.add_argument("--N", type=int, default=7)
-    ap.add_argument("--cpu-steps", type=int, default=1,
+    ap.add_argument("--cpu-steps", type=int, default=16,
                     help="steps of the CPU oracle for cpu_baseline (0 = skip)")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=50,

@@ -181,6 +181,12 @@ int32_t nkc_adjoint_get_drude(nkc_handle *h, double *ga, double *gb);
  * the backward sweep recomputes only the others.
  * nkc_adjoint_info: out[5] = {every, segments, S, K, device bytes held}. */
 int32_t nkc_adjoint_set_store(nkc_handle *h, int64_t max_steps);
+/* material = 1 (default): the backward sweep returns dL/deps and dL/dmu; 0:
+ * it skips them (zeros are returned), which drops the stage residual from
+ * the store (half its size: larger windows) and from the adjoint stage;
+ * the Drude gradients are unaffected.  Applies from the next
+ * nkc_adjoint_forward. */
+int32_t nkc_adjoint_set_grads(nkc_handle *h, int32_t material);
 int32_t nkc_adjoint_info(nkc_handle *h, int64_t *out);
 /* aH, aE = B^T P (kH, kE) on the current state: the transpose of the
  * operator whose residual is nkc_residual (without sources).  Test entry. */

@@ -49,6 +49,7 @@ class ProbeProblem:
 
 def _run_forward(problem: ProbeProblem, eps: np.ndarray, mu: np.ndarray):
     s = problem.solver
+    s.adjoint_set_grads(True)
     s.set_materials(eps, mu)
     s.time = problem.t0
     s.set_state(problem.H0, problem.E0)
@@ -76,6 +77,7 @@ class DrudeProbeProblem:
 
 def _run_forward_drude(problem: DrudeProbeProblem, gamma: np.ndarray, wp: np.ndarray):
     s = problem.solver
+    s.adjoint_set_grads(False)      # only the Drude gradients are asked for
     s.set_drude(problem.dindex, gamma[problem.owner], (wp * wp)[problem.owner])
     s.time = problem.t0
     s.set_state(problem.H0, problem.E0)

@@ -212,7 +212,9 @@ __device__ __forceinline__ void adj_face(double sg, int type, const double vo[6]
     gm = -dq * mo.im;
 }
 
-template <int N1, int MAT>
+// GM = 1: also the eps/mu gradients (needs the stage residual); GM = 0: the
+// adjoint state and the Drude gradients only (no residual store is read).
+template <int N1, int MAT, int GM>
 __global__ void __launch_bounds__(Shape<N1>::BS) NKC_OCC
 adjoint_kernel(AdjArgs a) {
     using S = Shape<N1>;
@@ -223,7 +225,7 @@ adjoint_kernel(AdjArgs a) {
     __shared__ double s_v[6 * N3];      // v = P kt, swizzled like s_u
     __shared__ double s_f[32 * N2];     // [face 0..3][adjH 3, adjE 3, geps, gmu][q]
     __shared__ double s_D[N2];
-    __shared__ double s_g[2 * N3];      // [grp][k N2 + q]: gradient without the face terms
+    __shared__ double s_g[GM ? 2 * N3 : 1];   // [grp][k N2 + q]: gradient without the face terms
 
     const int e = remap_element(blockIdx.x, a.nelt, a.per_xcd);
     if (e >= a.nelt) return;
@@ -259,9 +261,9 @@ adjoint_kernel(AdjArgs a) {
                 const int64_t p = (fo + c) * fs + node;
                 lm[c][kk] = ok ? a.lam_in[p] : 0.0;
                 kp[c][kk] = ok ? a.kap_in[p] : 0.0;
-                rr[c][kk] = ok ? ld_stream(a.rs + p) : 0.0;
+                rr[c][kk] = (GM && ok) ? ld_stream(a.rs + p) : 0.0;
             }
-            gg[kk] = ok ? g[node] : 0.0;
+            gg[kk] = (GM && ok) ? g[node] : 0.0;
         }
 #pragma unroll
         for (int kk = 0; kk < KH; ++kk) {
@@ -276,7 +278,7 @@ adjoint_kernel(AdjArgs a) {
                 kr += kt * rr[c][kk];
                 st_stream(a.kap_out + (fo + c) * fs + node, a.rka * kt);
             }
-            s_g[grp * N3 + k * N2 + q] = gg[kk] - a.dt * kr * pm[kk];
+            if (GM) s_g[grp * N3 + k * N2 + q] = gg[kk] - a.dt * kr * pm[kk];
         }
     }
     const double *geo = a.geo + (int64_t)e * 9;
@@ -323,8 +325,10 @@ adjoint_kernel(AdjArgs a) {
         else       adj_face<1>(sg, nb, vo, uo, vn, un, geo[3 + r], Ln, a.C0, mo, Yn, Zn, adj, ge, gm);
 #pragma unroll
         for (int c = 0; c < 6; ++c) s_f[(r * 8 + c) * N2 + q] = adj[c];
-        s_f[(r * 8 + 6) * N2 + q] = ge;
-        s_f[(r * 8 + 7) * N2 + q] = gm;
+        if (GM) {
+            s_f[(r * 8 + 6) * N2 + q] = ge;
+            s_f[(r * 8 + 7) * N2 + q] = gm;
+        }
     }
     __syncthreads();
     if (!act) return;
@@ -455,7 +459,7 @@ adjoint_kernel(AdjArgs a) {
             }
         }
         double *g = grp ? a.geps : a.gmu;
-        g[node] = s_g[grp * N3 + vloc] + a.dt * gface;
+        if (GM) g[node] = s_g[grp * N3 + vloc] + a.dt * gface;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const int64_t p = (fo + c) * fs + node;

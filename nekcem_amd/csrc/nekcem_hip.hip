@@ -341,6 +341,7 @@ struct nkc_handle {
     double C0 = 1.0;
     double *yz = nullptr;                 // (sqrt(eps/mu), sqrt(mu/eps)) per ieps entry, adjoint only
     int64_t adj_store_cap = 0;            // cap on the adjoint's stage-store window (0: from free memory)
+    int32_t adj_grads = 3;                // bit 0: eps/mu gradients, bit 1: Drude gradients
     double tune_us[4] = {NAN, NAN, NAN, NAN};   // tuner's stage times per layout
     // on-device diagnostics: node coordinates (3 x fs; from the box or
     // nkc_set_coords), element Jacobians and GLL weights (bm1 = jac w3),
@@ -585,7 +586,10 @@ int32_t copy_fields_d2h(const double *src, int64_t fs, int64_t src_off, double *
 
 template <int N1, int MAT>
 void launch_adj_t(const AdjArgs &a, int grid, hipStream_t s) {
-    hipLaunchKernelGGL((adjoint_kernel<N1, MAT>), dim3(grid), dim3(Shape<N1>::BS), 0, s, a);
+    if (a.rs)
+        hipLaunchKernelGGL((adjoint_kernel<N1, MAT, 1>), dim3(grid), dim3(Shape<N1>::BS), 0, s, a);
+    else
+        hipLaunchKernelGGL((adjoint_kernel<N1, MAT, 0>), dim3(grid), dim3(Shape<N1>::BS), 0, s, a);
 }
 
 int32_t launch_adjoint(const DevMesh &dm, const AdjArgs &a0, hipStream_t s) {
@@ -624,7 +628,7 @@ int32_t queue_step_store(nkc_handle *h, double dt, int64_t slot) {
     const int64_t blk = 6 * h->m.fs;
     auto &A = h->adj;
     double *su = A.st_u + (int64_t)A.cur_slot * (5 * A.S + 1) * blk;
-    double *sr = A.st_r + (int64_t)A.cur_slot * 5 * A.S * blk;
+    double *sr = A.st_r ? A.st_r + (int64_t)A.cur_slot * 5 * A.S * blk : nullptr;
     double *sj = h->ndrude ? A.stJ + (int64_t)A.cur_slot * 5 * A.S * (int64_t)drude_size(h) : nullptr;
     for (int s = 0; s < 5; ++s) {
         StageArgs a = base_args(h);
@@ -636,7 +640,7 @@ int32_t queue_step_store(nkc_handle *h, double dt, int64_t slot) {
         a.rkb = RK4B[s];
         a.rkc = RK4C[s];
         a.dt = dt;
-        a.st_r = sr + (slot * 5 + s) * blk;
+        a.st_r = sr ? sr + (slot * 5 + s) * blk : nullptr;
         if (h->ndrude) a.st_J = sj + (slot * 5 + s) * (int64_t)drude_size(h);
         const int32_t rc = launch_stage(h->m, a, MODE_RK, h->stream);
         if (rc) return rc;
@@ -1204,7 +1208,8 @@ int32_t alloc_adjoint(nkc_handle *h) {
         if ((rc = dev_alloc_counted(h, &A.gA, jsz / 3))) return rc;
         if ((rc = dev_alloc_counted(h, &A.gB, jsz / 3))) return rc;
     }
-    const size_t per_step = sizeof(double) * 5 * (2 * blk + jsz);
+    const bool gm = (h->adj_grads & 1) != 0;   // eps/mu gradients need the stage residuals
+    const size_t per_step = sizeof(double) * 5 * ((gm ? 2 : 1) * blk + jsz);
     int64_t S = A.every;
     if (h->adj_store_cap > 0) {
         S = std::min(S, h->adj_store_cap);
@@ -1231,7 +1236,7 @@ int32_t alloc_adjoint(nkc_handle *h) {
     A.slot_id.assign(K, -1);
     A.cur_slot = 0;
     if ((rc = dev_alloc_counted(h, &A.st_u, blk * (5 * S + 1) * K))) return rc;
-    if ((rc = dev_alloc_counted(h, &A.st_r, blk * 5 * S * K))) return rc;
+    if (gm && (rc = dev_alloc_counted(h, &A.st_r, blk * 5 * S * K))) return rc;
     if ((rc = dev_alloc_counted(h, &A.sub, blk * A.nsub_max))) return rc;
     if (h->ndrude) {
         if ((rc = dev_alloc_counted(h, &A.stJ, jsz * 5 * S * K))) return rc;
@@ -1330,7 +1335,7 @@ int32_t reverse_window(nkc_handle *h, int64_t seg, int64_t q, int &li) {
             a.lam_out = A.lam[li ^ 1];
             a.kap_out = A.kap[li ^ 1];
             a.us = slot_u(h, A.cur_slot) + (s * 5 + st) * blk;
-            a.rs = A.st_r + ((int64_t)A.cur_slot * 5 * A.S + s * 5 + st) * blk;
+            a.rs = A.st_r ? A.st_r + ((int64_t)A.cur_slot * 5 * A.S + s * 5 + st) * blk : nullptr;
             a.geps = A.geps;
             a.gmu = A.gmu;
             a.rka = RK4A[st];
@@ -1359,6 +1364,12 @@ int32_t nkc_adjoint_set_store(nkc_handle *h, int64_t max_steps) {
     if (!h) return fail(NKC_ERR_ARG, "null handle");
     if (max_steps < 0) return fail(NKC_ERR_ARG, "max_steps < 0");
     h->adj_store_cap = max_steps;
+    return NKC_OK;
+}
+
+int32_t nkc_adjoint_set_grads(nkc_handle *h, int32_t material) {
+    if (!h) return fail(NKC_ERR_ARG, "null handle");
+    h->adj_grads = (material ? 1 : 0) | 2;
     return NKC_OK;
 }
 

@@ -364,6 +364,11 @@ class MaxwellSolver:
         """Cap the stage-store window of the adjoint (0: from free memory)."""
         _lib.check(self._L.nkc_adjoint_set_store(self._h, int(max_steps)))
 
+    def adjoint_set_grads(self, material=True):
+        """Whether the backward sweep computes dL/deps, dL/dmu (default) or
+        skips them (Drude gradients only; half the stage store)."""
+        _lib.check(self._L.nkc_adjoint_set_grads(self._h, 1 if material else 0))
+
     def adjoint_info(self):
         """dict(every, segments, window, kept, bytes) of the last adjoint_forward
         (kept: segments whose stages the forward kept)."""

@@ -246,6 +246,17 @@ def test_drude_adjoint_matches_oracle_and_fd(oracle):
     ge2, _ = s.adjoint_backward(np.zeros((3, n)), lamE)
     ga2, gb2 = s.adjoint_drude_grad()
     assert np.array_equal(ge2, ge) and np.array_equal(ga2, ga) and np.array_equal(gb2, gb)
+    # without the eps/mu gradients (no residual store): the same Drude bits
+    s.adjoint_set_store(0)
+    s.adjoint_set_grads(False)
+    s.set_drude(dindex, np.repeat(gam, n3), np.repeat(wp * wp, n3))
+    s.time = 0.0
+    s.set_state(H0, E0)
+    s.adjoint_forward(dt, K, every)
+    ge3, _ = s.adjoint_backward(np.zeros((3, n)), lamE)
+    ga3, gb3 = s.adjoint_drude_grad()
+    assert np.array_equal(ga3, ga) and np.array_equal(gb3, gb) and not np.any(ge3)
+    s.adjoint_set_grads(True)
 
     def loss(g, w):
         s.set_drude(dindex, np.repeat(g, n3), np.repeat(w * w, n3))
