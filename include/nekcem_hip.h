@@ -147,6 +147,19 @@ int32_t nkc_set_drude(nkc_handle *h, int64_t ndrude, const int32_t *dindex,
  * nkc_residual) back, [3][ndrude] each; any pointer may be NULL. */
 int32_t nkc_get_drude(nkc_handle *h, double *J, double *kJ, double *resJ);
 
+/* Lorentz ADE (cem_maxwell_lorentz, src/cem_maxwell.F:3149-3211) on nlor
+ * volume nodes lindex (1-based): resE -= J*bmn, resJ = -a J - b P + c E,
+ * resP = J, J and P advanced by the same low-storage RK, i.e. eps(w) =
+ * eps_oo + c/(b - i a w - w^2).  params is (a, b, c) per listed node,
+ * interleaved [3*l + 0/1/2]; L and kL are [6][nlor] (Jx, Jy, Jz, Px, Py,
+ * Pz; component-major) or NULL for zero.  nlor = 0 removes it.  The adjoint
+ * does not cover it (nkc_adjoint_forward returns NKC_ERR_UNSUPPORTED). */
+int32_t nkc_set_lorentz(nkc_handle *h, int64_t nlor, const int32_t *lindex,
+                        const double *params, const double *L, const double *kL);
+/* Copy L, its RK register and the last residual back ([6][nlor] each;
+ * any pointer may be NULL). */
+int32_t nkc_get_lorentz(nkc_handle *h, double *L, double *kL, double *resL);
+
 /* Replace the materials by per-node permittivity and permeability (npts
  * each): 1/eps, 1/mu and every face node's flux coefficients are rebuilt as
  * cem_maxwell_materials does (src/cem_maxwell.F:262-325). */

@@ -77,6 +77,8 @@ SIGNATURES = [
     ("nkc_adjoint_forward", C.c_int32, [C.c_void_p, _dp, C.c_double, C.c_int64, C.c_int64]),
     ("nkc_adjoint_backward", C.c_int32, [C.c_void_p, _dp, _dp, C.c_int64, _dp, _dp]),
     ("nkc_adjoint_get_drude", C.c_int32, [C.c_void_p, _dp, _dp]),
+    ("nkc_set_lorentz", C.c_int32, [C.c_void_p, C.c_int64, _ip, _dp, _dp, _dp]),
+    ("nkc_get_lorentz", C.c_int32, [C.c_void_p, _dp, _dp, _dp]),
     ("nkc_set_layout", C.c_int32, [C.c_void_p, C.c_int32]),
     ("nkc_tune", C.c_int32, [C.c_void_p, _dp]),
     ("nkc_get_layout", C.c_int32, [C.c_void_p, _dp]),

@@ -131,7 +131,7 @@ constexpr int NLAYOUT = 4;
 
 template <int N1, int MAT, int MODE, int PACK, int OCC>
 void launch_t(const StageArgs &a, int grid, hipStream_t s) {
-    if (a.src_off || a.drude_off)
+    if (a.src_off || a.drude_off || a.lor_off)
         hipLaunchKernelGGL((stage_kernel<N1, MAT, MODE, PACK, OCC, 1>), dim3(grid), dim3(Shape<N1, PACK>::BS), 0, s, a);
     else
         hipLaunchKernelGGL((stage_kernel<N1, MAT, MODE, PACK, OCC, 0>), dim3(grid), dim3(Shape<N1, PACK>::BS), 0, s, a);
@@ -209,18 +209,20 @@ int32_t stage_kflags(int s) {
 // Autotuner over the stage kernel's layouts (lane packing x register
 // budget): time each on this mesh, alternating, best of three rounds of
 // four launches, and keep the fastest.  It writes only a scratch output and
-// a scratch RK register, and with the handle's source and Drude region
-// (feat, may be null) it times the kernel instance the step runs, the Drude
-// current and its register redirected to scratch: the state is untouched.
-// A packed layout whose workgroup equals the unpacked one is not timed.
-int32_t tune_stage(DevMesh &dm, const StageArgs *feat, size_t jsz, const double *uin, hipStream_t s,
-                   double us[NLAYOUT]) {
+// a scratch RK register, and with the handle's source, Drude and Lorentz
+// regions (feat, may be null) it times the kernel instance the step runs,
+// the auxiliary currents and their registers redirected to scratch: the
+// state is untouched.  A packed layout whose workgroup equals the unpacked
+// one is not timed.  jsz, lsz: doubles of the Drude / Lorentz state blocks.
+int32_t tune_stage(DevMesh &dm, const StageArgs *feat, size_t jsz, size_t lsz, const double *uin,
+                   hipStream_t s, double us[NLAYOUT]) {
     for (int l = 0; l < NLAYOUT; ++l) us[l] = NAN;
     const bool same = stage_block(dm.n1, 0) == stage_block(dm.n1, 1);
     const size_t blk = 6 * (size_t)dm.fs;
     const size_t jx = (feat && feat->drude_off) ? jsz : 0;
+    const size_t lx = (feat && feat->lor_off) ? lsz : 0;
     double *scr = nullptr;
-    int32_t rc = dev_alloc(&scr, 2 * blk + 2 * jx);
+    int32_t rc = dev_alloc(&scr, 2 * blk + 2 * jx + 2 * lx);
     if (rc) return rc;
     StageArgs a = mesh_args(dm);
     if (feat) {
@@ -235,6 +237,12 @@ int32_t tune_stage(DevMesh &dm, const StageArgs *feat, size_t jsz, const double 
             a.dpar = feat->dpar;
             a.dJ = scr + 2 * blk;
             a.dkJ = scr + 2 * blk + jx;
+        }
+        if (lx) {
+            a.lor_off = feat->lor_off;
+            a.lpar = feat->lpar;
+            a.lL = scr + 2 * blk + 2 * jx;
+            a.lkL = scr + 2 * blk + 2 * jx + lx;
         }
     }
     a.uin = uin;
@@ -335,6 +343,11 @@ struct nkc_handle {
     int64_t ndrude = 0, ndelem = 0;
     int32_t *drude_off = nullptr;
     double *dJ = nullptr, *dkJ = nullptr, *dpar = nullptr, *dresJ = nullptr;
+    // Lorentz (compact per Lorentz element: (J, P) 6 n3, register, (a,b,c) per node)
+    int64_t nlor = 0, nlelem = 0;
+    int32_t *lor_off = nullptr;
+    double *lL = nullptr, *lkL = nullptr, *lpar = nullptr, *lresL = nullptr;
+    std::vector<int64_t> lor_slot;        // listed node -> lelem 6 n3 + vloc
     std::vector<int64_t> drude_slot;      // listed node -> compact (elem*3*n3 + vloc)
     // host copy of the element tables (material updates, adjoint)
     HostMesh hm;
@@ -395,6 +408,12 @@ size_t drude_size(const nkc_handle *h) {
     return (size_t)h->ndelem * 3 * n3;
 }
 
+// doubles in one compact Lorentz state block (nlelem x 6 x n3)
+size_t lorentz_size(const nkc_handle *h) {
+    const size_t n3 = (size_t)h->m.n1 * h->m.n1 * h->m.n1;
+    return (size_t)h->nlelem * 6 * n3;
+}
+
 StageArgs base_args(nkc_handle *h) {
     StageArgs a = mesh_args(h->m);
     a.tdev = h->tdev;
@@ -408,6 +427,11 @@ StageArgs base_args(nkc_handle *h) {
     a.dkJ = h->dkJ;
     a.dpar = h->dpar;
     a.dresJ = h->dresJ;
+    a.lor_off = h->lor_off;
+    a.lL = h->lL;
+    a.lkL = h->lkL;
+    a.lpar = h->lpar;
+    a.lresL = h->lresL;
     return a;
 }
 
@@ -490,6 +514,8 @@ void free_handle(nkc_handle *h) {
     (void)hipFree(h->src_off); (void)hipFree(h->src_amp);
     (void)hipFree(h->drude_off); (void)hipFree(h->dJ); (void)hipFree(h->dkJ);
     (void)hipFree(h->dpar); (void)hipFree(h->dresJ);
+    (void)hipFree(h->lor_off); (void)hipFree(h->lL); (void)hipFree(h->lkL);
+    (void)hipFree(h->lpar); (void)hipFree(h->lresL);
     (void)hipFree(h->xyz); (void)hipFree(h->djac); (void)hipFree(h->dw);
     (void)hipFree(h->dpart); (void)hipFree(h->dout);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -855,7 +881,7 @@ int32_t nkc_step(nkc_handle *h, double *time, double dt, int64_t nsteps) {
     }
     if (h->m.layout < 0) {
         const StageArgs f = base_args(h);
-        const int32_t rc = tune_stage(h->m, &f, drude_size(h), h->m.u[h->cur], h->stream, h->tune_us);
+        const int32_t rc = tune_stage(h->m, &f, drude_size(h), lorentz_size(h), h->m.u[h->cur], h->stream, h->tune_us);
         if (rc) return rc;
     }
     if (h->use_graph) {
@@ -1074,6 +1100,72 @@ int32_t nkc_cem_error_fields(nkc_handle *h, const double *hx, const double *ex, 
     return rc;
 }
 
+int32_t nkc_set_lorentz(nkc_handle *h, int64_t nlor, const int32_t *lindex, const double *params,
+                        const double *L, const double *kL) {
+    if (!h) return fail(NKC_ERR_ARG, "null handle");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    free_adjoint(h);
+    h->m.layout = -1;                     // the step's kernel instance may change: tune again
+    (void)hipFree(h->lor_off); h->lor_off = nullptr;
+    (void)hipFree(h->lL); h->lL = nullptr;
+    (void)hipFree(h->lkL); h->lkL = nullptr;
+    (void)hipFree(h->lpar); h->lpar = nullptr;
+    (void)hipFree(h->lresL); h->lresL = nullptr;
+    h->nlor = 0;
+    h->nlelem = 0;
+    h->lor_slot.clear();
+    int32_t rc = drop_graphs(h);
+    if (rc || nlor == 0) return rc;
+    if (nlor < 0 || !lindex || !params) return fail(NKC_ERR_ARG, "bad Lorentz node list");
+    const DevMesh &m = h->m;
+    const int n3 = m.n1 * m.n1 * m.n1;
+    std::vector<int32_t> off(m.nelt, -1);
+    int32_t nle = 0;
+    for (int64_t l = 0; l < nlor; ++l) {
+        const int64_t v = (int64_t)lindex[l] - 1;
+        if (v < 0 || v >= m.npts) return fail(NKC_ERR_ARG, "lindex[%lld]=%d out of range", (long long)l, lindex[l]);
+        const int64_t e = v / n3;
+        if (off[e] < 0) off[e] = nle++;
+    }
+    std::vector<double> par((size_t)nle * n3 * 3, 0.0), Lv((size_t)nle * 6 * n3, 0.0), kv = Lv;
+    h->lor_slot.resize(nlor);
+    for (int64_t l = 0; l < nlor; ++l) {
+        const int64_t v = (int64_t)lindex[l] - 1, e = v / n3, vl = v % n3;
+        const int64_t b = (int64_t)off[e] * 6 * n3 + vl;
+        h->lor_slot[l] = b;
+        for (int q = 0; q < 3; ++q) par[((int64_t)off[e] * n3 + vl) * 3 + q] = params[3 * l + q];
+        for (int c = 0; c < 6; ++c) {
+            if (L) Lv[b + c * n3] = L[c * nlor + l];
+            if (kL) kv[b + c * n3] = kL[c * nlor + l];
+        }
+    }
+    if ((rc = upload(&h->lor_off, off))) return rc;
+    if ((rc = upload(&h->lpar, par))) return rc;
+    if ((rc = upload(&h->lL, Lv))) return rc;
+    if ((rc = upload(&h->lkL, kv))) return rc;
+    if ((rc = upload(&h->lresL, kv))) return rc;
+    h->nlor = nlor;
+    h->nlelem = nle;
+    return NKC_OK;
+}
+
+int32_t nkc_get_lorentz(nkc_handle *h, double *L, double *kL, double *resL) {
+    if (!h) return fail(NKC_ERR_ARG, "null handle");
+    if (h->nlor == 0) return fail(NKC_ERR_ARG, "no Lorentz region set");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    const int n3 = h->m.n1 * h->m.n1 * h->m.n1;
+    std::vector<double> buf(lorentz_size(h));
+    double *dst[3] = {L, kL, resL};
+    double *srcp[3] = {h->lL, h->lkL, h->lresL};
+    for (int w = 0; w < 3; ++w) {
+        if (!dst[w]) continue;
+        HIPCHK(hipMemcpy(buf.data(), srcp[w], sizeof(double) * buf.size(), hipMemcpyDeviceToHost));
+        for (int64_t l = 0; l < h->nlor; ++l)
+            for (int c = 0; c < 6; ++c) dst[w][c * h->nlor + l] = buf[h->lor_slot[l] + c * n3];
+    }
+    return NKC_OK;
+}
+
 int32_t nkc_check_finite(nkc_handle *h, double *maxabs) {
     if (!h) return fail(NKC_ERR_ARG, "null handle");
     hipLaunchKernelGGL(zero_u64, dim3(1), dim3(1), 0, h->stream, h->red);
@@ -1143,7 +1235,7 @@ int32_t nkc_tune(nkc_handle *h, double *out) {
     if (!h) return fail(NKC_ERR_ARG, "null handle");
     HIPCHK(hipStreamSynchronize(h->stream));
     const StageArgs f = base_args(h);
-    int32_t rc = tune_stage(h->m, &f, drude_size(h), h->m.u[h->cur], h->stream, h->tune_us);
+    int32_t rc = tune_stage(h->m, &f, drude_size(h), lorentz_size(h), h->m.u[h->cur], h->stream, h->tune_us);
     if (rc) return rc;
     if ((rc = drop_graphs(h))) return rc;
     if (out) {
@@ -1387,6 +1479,7 @@ int32_t nkc_adjoint_info(nkc_handle *h, int64_t *out) {
 int32_t nkc_adjoint_forward(nkc_handle *h, double *time, double dt, int64_t nsteps, int64_t every) {
     if (!h) return fail(NKC_ERR_ARG, "null handle");
     if (nsteps <= 0 || every <= 0 || !std::isfinite(dt)) return fail(NKC_ERR_ARG, "bad adjoint forward arguments");
+    if (h->nlor) return fail(NKC_ERR_UNSUPPORTED, "the adjoint of a Lorentz region is not built");
     free_adjoint(h);
     auto &A = h->adj;
     A.nsteps = nsteps;
@@ -1694,7 +1787,7 @@ int32_t nkc_batch_step(nkc_batch *b, const double *dt, double *times, int64_t ns
     for (auto &dm : b->bins)
         if (dm.layout < 0) {
             double us[NLAYOUT];
-            const int32_t rc = tune_stage(dm, nullptr, 0, dm.u[b->cur], b->main, us);
+            const int32_t rc = tune_stage(dm, nullptr, 0, 0, dm.u[b->cur], b->main, us);
             if (rc) return rc;
         }
     if (!b->graph_ok) {

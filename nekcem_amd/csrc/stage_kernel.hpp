@@ -100,6 +100,14 @@ struct StageArgs {
     double *__restrict__ dkJ;
     const double *__restrict__ dpar;
     double *__restrict__ dresJ;       // MODE_RESIDUAL: resJ out (same layout as J)
+    // Lorentz ADE (cem_maxwell_lorentz): elements with lor_off[e] >= 0 carry
+    // J then P (6 n1^3 doubles at lor_off[e] 6 n1^3) and their RK register,
+    // per-node (a, b, c) at (lor_off[e] n1^3 + vloc) 3
+    const int32_t *__restrict__ lor_off;
+    double *__restrict__ lL;
+    double *__restrict__ lkL;
+    const double *__restrict__ lpar;
+    double *__restrict__ lresL;       // MODE_RESIDUAL: (resJ, resP) out
     // MODE_RK, adjoint recompute: keep the stage residual (6 fields, stride
     // fs) and the stage-input Drude current (layout of dJ) when non-null
     double *__restrict__ st_r;
@@ -344,6 +352,7 @@ stage_kernel(StageArgs a) {
         sg = exp(-tt * tt);
     }
     const int dd = (FEAT && grp == 1 && a.drude_off) ? a.drude_off[e] : -1;
+    const int lo = (FEAT && grp == 1 && a.lor_off) ? a.lor_off[e] : -1;
 #pragma unroll
     for (int kk = 0; kk < KH; ++kk) {
         const int k = k0 + kk;
@@ -417,6 +426,30 @@ stage_kernel(StageArgs a) {
                     const double kj = a.rka * a.dkJ[jb + c * N3] + dt * rJ;
                     a.dkJ[jb + c * N3] = kj;
                     a.dJ[jb + c * N3] = Jc + a.rkb * kj;
+                }
+            }
+        }
+        if (lo >= 0) {
+            // cem_maxwell_lorentz: resE -= J bm1; resJ = -a J - b P + c E;
+            // resP = J; J and P advanced by the same RK stage
+            const int64_t lb = (int64_t)lo * 6 * N3 + vloc;
+            const double *lp = a.lpar + ((int64_t)lo * N3 + vloc) * 3;
+            const double la = lp[0], lbp = lp[1], lc = lp[2];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const double Jc = a.lL[lb + c * N3], Pc = a.lL[lb + (3 + c) * N3];
+                const double rJ = -la * Jc - lbp * Pc + lc * s_u[(3 + c) * N3 + own];
+                r[c] -= Jc;
+                if (MODE == MODE_RESIDUAL) {
+                    a.lresL[lb + c * N3] = rJ;
+                    a.lresL[lb + (3 + c) * N3] = Jc;
+                } else {
+                    const double kj = a.rka * a.lkL[lb + c * N3] + dt * rJ;
+                    const double kp2 = a.rka * a.lkL[lb + (3 + c) * N3] + dt * Jc;
+                    a.lkL[lb + c * N3] = kj;
+                    a.lkL[lb + (3 + c) * N3] = kp2;
+                    a.lL[lb + c * N3] = Jc + a.rkb * kj;
+                    a.lL[lb + (3 + c) * N3] = Pc + a.rkb * kp2;
                 }
             }
         }

@@ -151,7 +151,7 @@ class MaxwellSolver:
         self.mesh = _HexMeshInfo(N, lo, hi)
         self.npts = self.mesh.npts
         self._h, self._L = h, L
-        self.time, self.istep, self.dt, self.ndrude = 0.0, 0, None, 0
+        self.time, self.istep, self.dt, self.ndrude, self.nlor = 0.0, 0, None, 0, 0
         if not graph:
             _lib.check(L.nkc_set_graph(h, 0))
         return self
@@ -183,7 +183,7 @@ class MaxwellSolver:
         self.mesh = _BoxMeshInfo(nel, N, lo, hi, bc)
         self.npts = self.mesh.npts
         self._h, self._L = h, L
-        self.time, self.istep, self.dt, self.ndrude = 0.0, 0, None, 0
+        self.time, self.istep, self.dt, self.ndrude, self.nlor = 0.0, 0, None, 0, 0
         if not graph:
             _lib.check(L.nkc_set_graph(h, 0))
         return self
@@ -217,6 +217,7 @@ class MaxwellSolver:
         self.istep = 0
         self.dt = None
         self.ndrude = 0
+        self.nlor = 0
         if not graph:
             _lib.check(L.nkc_set_graph(h, 0))
 
@@ -283,6 +284,31 @@ class MaxwellSolver:
         J, kJ, rJ = np.empty((3, nd)), np.empty((3, nd)), np.empty((3, nd))
         _lib.check(self._L.nkc_get_drude(self._h, _d(J), _d(kJ), _d(rJ)))
         return J, kJ, rJ
+
+    def set_lorentz(self, lindex, a, b, c, L=None, kL=None):
+        """Lorentz ADE on volume nodes lindex (0-based) with per-node a, b, c
+        (eps(w) = eps_oo + c/(b - i a w - w^2)); L, kL (6, nlor) = (J, P)
+        or None (zero)."""
+        lindex = np.asarray(lindex, dtype=np.int64)
+        n = len(lindex)
+        if n == 0:
+            _lib.check(self._L.nkc_set_lorentz(self._h, 0, None, None, None, None))
+            self.nlor = 0
+            return
+        li = np.ascontiguousarray(lindex + 1, dtype=np.int32)
+        par = np.ascontiguousarray(np.stack([np.broadcast_to(a, (n,)), np.broadcast_to(b, (n,)),
+                                             np.broadcast_to(c, (n,))], axis=1), dtype=np.float64)
+        L = None if L is None else np.ascontiguousarray(L, dtype=np.float64).reshape(6, n)
+        kL = None if kL is None else np.ascontiguousarray(kL, dtype=np.float64).reshape(6, n)
+        _lib.check(self._L.nkc_set_lorentz(self._h, n, _i(li), _d(par), _d(L), _d(kL)))
+        self.nlor = n
+
+    def get_lorentz(self):
+        """(L, kL, resL), each (6, nlor): J then P."""
+        n = self.nlor
+        L, kL, rL = np.empty((6, n)), np.empty((6, n)), np.empty((6, n))
+        _lib.check(self._L.nkc_get_lorentz(self._h, _d(L), _d(kL), _d(rL)))
+        return L, kL, rL
 
     def set_materials(self, eps, mu=None):
         """Per-node permittivity/permeability (npts each)."""

@@ -32,7 +32,9 @@
  *                      :922-1002 (cem_maxwell_flux3d), :1368-1426
  *                      (cem_maxwell_flux_pec), :654-756 (add_flux_to_res),
  *                      :1833-1904 (cem_maxwell_invqmass); plus the Drude ADE
- *                      :3095-3147 and a usersrc-style source term.
+ *                      :3095-3147, a usersrc-style source term, and (in
+ *                      orc_maxwell_op_ade / orc_rk_run_ade) the Lorentz
+ *                      ADE :3149-3211.
  *   orc_rk_run         cem_maxwell.F:327-345 (cem_maxwell_op_rk), :1906-1968
  *                      (rk_maxwell_ab), cem_common.F:2-16 (rk_c), :18-76
  *                      (rk4_upd), :78-114 (rk_storage, ifrk45 constants)
@@ -463,6 +465,11 @@ typedef struct orc_mesh {
     double src_t0, src_tw;
     /* scratch (6*nxzfl) */
     double *srflx;
+    /* Lorentz ADE (cem_maxwell_lorentz, cem_maxwell.F:3149-3211) over nodes
+     * lindex[0..nlorentz): per listed node (a, b, c) at lor_par[3 l ..] */
+    int64_t nlorentz;
+    const int32_t *lindex;
+    const double *lor_par;
 } orc_mesh;
 
 /* resE = W curl H; resH = -W curl E (cem_maxwell.F:574-582 + 1428-1497) */
@@ -500,11 +507,31 @@ static void wght_curl(const orc_mesh *m, const double *u1, const double *u2, con
     }
 }
 
+static void maxwell_op_body(const orc_mesh *m, double rktime, const double *hn, const double *en,
+                            double *reshn, double *resen, const double *J, double *resJ,
+                            const double *L, double *resL);
+
 /* One residual evaluation (cem_maxwell_op, cem_maxwell.F:484-508).
  * hn, en, reshn, resen: 3 components each at stride npts.
  * J, resJ: Drude polarisation current (3*ndrude) or NULL. */
 void orc_maxwell_op(const orc_mesh *m, double rktime, const double *hn, const double *en,
                     double *reshn, double *resen, const double *J, double *resJ)
+{
+    maxwell_op_body(m, rktime, hn, en, reshn, resen, J, resJ, NULL, NULL);
+}
+
+/* The same with the Lorentz state L = (J, P) (6 blocks of nlorentz: J then
+ * P, component-major) and its residual resL, or NULL. */
+void orc_maxwell_op_ade(const orc_mesh *m, double rktime, const double *hn, const double *en,
+                        double *reshn, double *resen, const double *J, double *resJ,
+                        const double *L, double *resL)
+{
+    maxwell_op_body(m, rktime, hn, en, reshn, resen, J, resJ, L, resL);
+}
+
+static void maxwell_op_body(const orc_mesh *m, double rktime, const double *hn, const double *en,
+                            double *reshn, double *resen, const double *J, double *resJ,
+                            const double *L, double *resL)
 {
     int64_t npts = m->npts, k = m->nxzfl;
     const double *H1 = hn, *H2 = hn + npts, *H3 = hn + 2 * npts;
@@ -610,6 +637,25 @@ void orc_maxwell_op(const orc_mesh *m, double rktime, const double *hn, const do
             resJ[2 * nd + l] = -m->drude_a[l] * J[2 * nd + l] + m->drude_b[l] * E3[i];
         }
     }
+    /* Lorentz (cem_maxwell_lorentz, cem_maxwell.F:3149-3211):
+     *   resE -= J bm1 ;  resJ = -a J - b P + c E ;  resP = J */
+    if (m->nlorentz > 0 && L && resL) {
+        int64_t nl = m->nlorentz;
+        const double *E[3] = {E1, E2, E3};
+        double *rE[3] = {rE1, rE2, rE3};
+#pragma omp parallel for schedule(static)
+        for (int64_t l = 0; l < nl; ++l) {
+            int64_t i = m->lindex[l];
+            double a = m->lor_par[3 * l], b = m->lor_par[3 * l + 1], c = m->lor_par[3 * l + 2];
+            double bm = m->bm1[i];
+            for (int q = 0; q < 3; ++q) {
+                double Jq = L[q * nl + l], Pq = L[(3 + q) * nl + l];
+                rE[q][i] = rE[q][i] - Jq * bm;
+                resL[q * nl + l] = -a * Jq - b * Pq + c * E[q][i];
+                resL[(3 + q) * nl + l] = Jq;
+            }
+        }
+    }
     /* cem_maxwell_invqmass */
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < npts; ++i) {
@@ -656,6 +702,26 @@ static void rk4_upd(double *h, double *kh, const double *resh, double cb, double
     for (int64_t i = 0; i < n; ++i) {
         kh[i] = ca * kh[i] + dt * resh[i];
         h[i] = h[i] + cb * kh[i];
+    }
+}
+
+/* nsteps of cem_maxwell_op_rk with the Lorentz state L/kL/resL (6*nlorentz)
+ * as well (NULL: none); otherwise as orc_rk_run below. */
+void orc_rk_run_ade(const orc_mesh *m, double *hn, double *en, double *khn, double *ken,
+                    double *reshn, double *resen, double *J, double *kJ, double *resJ,
+                    double *L, double *kL, double *resL, double dt, int64_t nsteps, double *time)
+{
+    int64_t n3 = 3 * m->npts;
+    for (int64_t st = 0; st < nsteps; ++st) {
+        for (int r = 0; r < 5; ++r) {
+            double rktime = *time + dt * RK4C[r];
+            maxwell_op_body(m, rktime, hn, en, reshn, resen, J, resJ, L, resL);
+            rk4_upd(hn, khn, reshn, RK4B[r], RK4A[r], dt, n3);
+            rk4_upd(en, ken, resen, RK4B[r], RK4A[r], dt, n3);
+            if (m->ndrude > 0 && J) rk4_upd(J, kJ, resJ, RK4B[r], RK4A[r], dt, 3 * m->ndrude);
+            if (m->nlorentz > 0 && L) rk4_upd(L, kL, resL, RK4B[r], RK4A[r], dt, 6 * m->nlorentz);
+        }
+        *time = *time + dt;
     }
 }
 

@@ -69,6 +69,8 @@ def lib():
         L.orc_maxwell_op.argtypes = [C.c_void_p, C.c_double, _dp, _dp, _dp, _dp, _dp, _dp]
         L.orc_rk_run.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
                                  C.c_double, C.c_int64, _dp]
+        L.orc_maxwell_op_ade.argtypes = [C.c_void_p, C.c_double] + [_dp] * 8
+        L.orc_rk_run_ade.argtypes = [C.c_void_p] + [_dp] * 12 + [C.c_double, C.c_int64, _dp]
         L.orc_rk_coeffs.argtypes = [_dp, _dp, _dp]
         L.orc_cem_error.argtypes = [_dp, _dp, _dp, C.c_int64, _dp, _dp]
         L.orc_maxwell_vjp.argtypes = [C.c_void_p, C.c_double, _dp, _dp, _bp, _dp, _dp, _dp, _dp,
@@ -107,6 +109,7 @@ class _Mesh(C.Structure):
         ("src_amp", _dp), ("src_dir", C.c_double * 3), ("src_t0", C.c_double),
         ("src_tw", C.c_double),
         ("srflx", _dp),
+        ("nlorentz", C.c_int64), ("lindex", _ip), ("lor_par", _dp),
     ]
 
 
@@ -218,6 +221,8 @@ class BoxCase:
         self.dindex = None
         self.drude_a = None
         self.drude_b = None
+        self.lindex = None
+        self.lor_par = None
 
     # -- setup pieces ---------------------------------------------------
     def _face_ids(self):
@@ -285,8 +290,18 @@ class BoxCase:
         """Drude ADE over volume nodes dindex with per-node a (gamma) and
         b (omega_p^2)."""
         self.dindex = np.ascontiguousarray(dindex, dtype=np.int32)
-        self.drude_a = np.ascontiguousarray(a, dtype=np.float64)
-        self.drude_b = np.ascontiguousarray(b, dtype=np.float64)
+        n = len(self.dindex)
+        self.drude_a = np.ascontiguousarray(np.broadcast_to(a, (n,)), dtype=np.float64)
+        self.drude_b = np.ascontiguousarray(np.broadcast_to(b, (n,)), dtype=np.float64)
+
+    def set_lorentz(self, lindex, a, b, c):
+        """Lorentz ADE (cem_maxwell_lorentz) over volume nodes lindex with
+        per-node a, b, c: eps(w) = eps_oo + c/(b - i a w - w^2)."""
+        self.lindex = np.ascontiguousarray(lindex, dtype=np.int32)
+        n = len(self.lindex)
+        self.lor_par = np.ascontiguousarray(np.stack([np.broadcast_to(a, (n,)), np.broadcast_to(b, (n,)),
+                                                      np.broadcast_to(c, (n,))], axis=1).reshape(-1),
+                                            dtype=np.float64)
 
     def dt_from_cfl(self, cfl):
         """set_dt (src/cem_drive.F:351-395): dt = |p12| if p12 < 0 else p12*dxmin."""
@@ -311,6 +326,11 @@ class BoxCase:
             m.dindex, m.drude_a, m.drude_b = _p(self.dindex), _p(self.drude_a), _p(self.drude_b)
         else:
             m.ndrude = 0
+        if self.lindex is not None and len(self.lindex):
+            m.nlorentz = len(self.lindex)
+            m.lindex, m.lor_par = _p(self.lindex), _p(self.lor_par)
+        else:
+            m.nlorentz = 0
         if self.src_amp is not None:
             m.src_amp = _p(self.src_amp)
         m.src_dir = (C.c_double * 3)(*self.src_dir)
@@ -319,22 +339,32 @@ class BoxCase:
         self._keep = m
         return m
 
-    def residual(self, H, E, t=0.0, J=None):
-        """(resH, resE[, resJ]) = cem_maxwell_op(H, E) incl. inverse mass."""
+    def residual(self, H, E, t=0.0, J=None, L=None):
+        """(resH, resE[, resJ][, resL]) = cem_maxwell_op(H, E) incl. inverse
+        mass; J the Drude current (3*ndrude), L the Lorentz state (J then P,
+        6*nlorentz)."""
         H = np.ascontiguousarray(H, dtype=np.float64).reshape(3, self.npts)
         E = np.ascontiguousarray(E, dtype=np.float64).reshape(3, self.npts)
         rH, rE = np.zeros_like(H), np.zeros_like(E)
         m = self._mesh()
-        rJ = None
+        rJ = rL = None
         if J is not None:
             J = np.ascontiguousarray(J, dtype=np.float64)
             rJ = np.zeros_like(J)
-        lib().orc_maxwell_op(C.byref(m), t, _p(H), _p(E), _p(rH), _p(rE), _p(J), _p(rJ))
-        return (rH, rE) if J is None else (rH, rE, rJ)
+        if L is not None:
+            L = np.ascontiguousarray(L, dtype=np.float64)
+            rL = np.zeros_like(L)
+        lib().orc_maxwell_op_ade(C.byref(m), t, _p(H), _p(E), _p(rH), _p(rE), _p(J), _p(rJ), _p(L), _p(rL))
+        out = (rH, rE)
+        if J is not None:
+            out += (rJ,)
+        if L is not None:
+            out += (rL,)
+        return out
 
-    def run(self, H, E, dt, nsteps, t=0.0, kH=None, kE=None, J=None, kJ=None):
+    def run(self, H, E, dt, nsteps, t=0.0, kH=None, kE=None, J=None, kJ=None, L=None, kL=None):
         """nsteps of cem_maxwell_op_rk from (H, E) at time t.  Returns a dict
-        with H, E, kH, kE, time (and J, kJ with Drude)."""
+        with H, E, kH, kE, time (and J, kJ with Drude, L, kL with Lorentz)."""
         H = np.array(H, dtype=np.float64).reshape(3, self.npts)
         E = np.array(E, dtype=np.float64).reshape(3, self.npts)
         kH = np.zeros_like(H) if kH is None else np.array(kH, dtype=np.float64)
@@ -346,13 +376,21 @@ class BoxCase:
             J = np.zeros(3 * nd) if J is None else np.array(J, dtype=np.float64)
             kJ = np.zeros(3 * nd) if kJ is None else np.array(kJ, dtype=np.float64)
             rJ = np.zeros(3 * nd)
+        rL = None
+        if self.lindex is not None and len(self.lindex):
+            nl = len(self.lindex)
+            L = np.zeros(6 * nl) if L is None else np.array(L, dtype=np.float64)
+            kL = np.zeros(6 * nl) if kL is None else np.array(kL, dtype=np.float64)
+            rL = np.zeros(6 * nl)
         tt = np.array([t], dtype=np.float64)
         m = self._mesh()
-        lib().orc_rk_run(C.byref(m), _p(H), _p(E), _p(kH), _p(kE), _p(rH), _p(rE),
-                         _p(J), _p(kJ), _p(rJ), dt, nsteps, _p(tt))
+        lib().orc_rk_run_ade(C.byref(m), _p(H), _p(E), _p(kH), _p(kE), _p(rH), _p(rE),
+                             _p(J), _p(kJ), _p(rJ), _p(L), _p(kL), _p(rL), dt, nsteps, _p(tt))
         out = dict(H=H, E=E, kH=kH, kE=kE, time=float(tt[0]))
         if J is not None:
             out.update(J=J, kJ=kJ)
+        if L is not None:
+            out.update(L=L, kL=kL)
         return out
 
     def vjp(self, H, E, kH, kE, t=0.0, J=None, kJ=None):

@@ -276,6 +276,55 @@ def test_source_and_drude_steps_parity(oracle):
     gpu.close()
 
 
+def test_lorentz_residual_and_steps_parity(oracle):
+    """A Lorentz block (cem_maxwell_lorentz) beside a Drude block, with the
+    source: one residual (resH, resE and resJ/resP of a random state) within
+    1e-11 of the oracle, and 40 RK steps of fields, J and P within 1e-11
+    relative L2."""
+    print("[case] Lorentz + Drude + source, 3x3x3 N=6 PEC", flush=True)
+    rng = np.random.default_rng(7)
+    nel, N = (3, 3, 3), 6
+    mesh, ref, gpu = _pair(oracle, nel, N, bc=("PEC",) * 3, lo=(0, 0, 0), hi=(1, 1, 1))
+    x, y, z = mesh.coords()
+    n = mesh.npts
+    amp = np.exp(-((x - 0.5) ** 2 + (y - 0.5) ** 2 + (z - 0.5) ** 2) / (2 * 0.1 ** 2)) * mesh.bmn
+    lblock = np.nonzero((x > 0.34) & (y < 0.66) & (z > 0.34))[0]
+    dblock = np.nonzero((x < 0.32) & (y > 0.68))[0]
+    la, lb, lc = rng.uniform(0.05, 0.3, len(lblock)), rng.uniform(1, 3, len(lblock)), rng.uniform(0.5, 2, len(lblock))
+    for side in (ref, gpu):
+        side.set_source(amp, direction=(0, 1, 0), t0=0.05, tw=0.02)
+        side.set_drude(dblock, 0.1, 2.0)
+        side.set_lorentz(lblock, la, lb, lc)
+    # residual of a random state with a random Lorentz state
+    H, E = rng.standard_normal((3, n)), rng.standard_normal((3, n))
+    L0 = rng.standard_normal((6, len(lblock)))
+    gpu.set_lorentz(lblock, la, lb, lc, L=L0)
+    gpu.set_state(H, E)
+    rH, rE = gpu.residual(0.04)
+    rL = gpu.get_lorentz()[2]
+    oH, oE, oJ, oL = ref.residual(H, E, t=0.04, J=np.zeros(3 * len(dblock)), L=L0.reshape(-1))
+    for a_, b_ in ((rH, oH), (rE, oE), (rL, oL.reshape(6, -1))):
+        assert np.max(np.abs(a_ - b_)) <= 1e-11 * np.max(np.abs(b_))
+    # 40 steps from rest
+    gpu.set_lorentz(lblock, la, lb, lc)
+    H0, E0 = np.zeros((3, n)), np.zeros((3, n))
+    dt = mesh.dt(0.25)
+    gpu.time = 0.0
+    gpu.set_state(H0, E0)
+    gpu.step(dt, 40)
+    gH, gE = gpu.get_state()
+    gL = gpu.get_lorentz()[0]
+    out = ref.run(H0, E0, dt, 40)
+    assert np.max(np.abs(out["E"])) > 1e-3
+    assert parity(gH, gE, out["H"], out["E"], mesh.bmn) <= 1e-11
+    oLs = out["L"].reshape(6, -1)
+    assert np.max(np.abs(oLs)) > 0
+    assert np.sqrt(np.sum((gL - oLs) ** 2)) <= 1e-11 * np.sqrt(np.sum(oLs ** 2))
+    with pytest.raises(Exception):
+        gpu.adjoint_forward(dt, 4, 2)          # the Lorentz adjoint is refused
+    gpu.close()
+
+
 @pytest.mark.parametrize("N", [4, 5, 6, 8, 9])
 def test_lane_layouts_bit_identical_and_tuner(N):
     """All four stage-kernel layouts give the same bits (graph replay, 3

@@ -288,6 +288,38 @@ def test_drude_uniform_field_ode(oracle):
     assert np.max(np.abs(out["E"][0])) < 1e-12
 
 
+def test_lorentz_uniform_field_ode(oracle):
+    """Uniform E, J, P in a periodic Lorentz box: curl and jumps vanish, so
+    (cem_maxwell_lorentz) eps dE/dt = -J, dJ/dt = -a J - b P + c E,
+    dP/dt = J -- a linear 3x3 system per component whose exact solution is
+    expm(A t) y0; 300 RK45 steps of dt = 0.01 agree to 1e-9.  Distinct a, b,
+    c and eps make every matrix entry distinct, so a swapped coefficient or
+    sign fails."""
+    from scipy.linalg import expm
+    spec = oracle.BoxSpec(nel=(2, 2, 2), N=3)
+    c = oracle.BoxCase(spec)
+    eps, a, b, cc = 2.0, 0.3, 1.7, 0.9
+    c.set_materials(np.full(c.npts, eps), None)
+    n = c.npts
+    c.set_lorentz(np.arange(n), a, b, cc)
+    E = np.zeros((3, n))
+    E[1], E[2] = 1.0, -0.4
+    L0 = np.zeros(6 * n)
+    L0[1 * n:2 * n] = 0.25          # J_y
+    L0[5 * n:6 * n] = -0.6          # P_z
+    out = c.run(np.zeros_like(E), E, 0.01, 300, L=L0)
+    t = out["time"]
+    A = np.array([[0.0, -1.0 / eps, 0.0], [cc, -a, -b], [0.0, 1.0, 0.0]])
+    M = expm(A * t)
+    for q in range(3):
+        y0 = np.array([E[q, 0], L0[q * n], L0[(3 + q) * n]])
+        y = M @ y0
+        assert np.max(np.abs(out["E"][q] - y[0])) < 1e-9, q
+        assert np.max(np.abs(out["L"][q * n:(q + 1) * n] - y[1])) < 1e-9, q
+        assert np.max(np.abs(out["L"][(3 + q) * n:(4 + q) * n] - y[2])) < 1e-9, q
+    assert np.max(np.abs(out["H"])) < 1e-12
+
+
 def test_gaussian_source_uniform(oracle):
     spec = oracle.BoxSpec(nel=(2, 2, 2), N=3)
     c = oracle.BoxCase(spec)
