@@ -146,6 +146,7 @@ class BoxSpec:
     hi: tuple = (2 * np.pi, 2 * np.pi, 2 * np.pi)
     bc: tuple = ("P", "P", "P")   # per direction: 'P' periodic or 'PEC'
     upwind: bool = True
+    verts: tuple = None  # optional per-axis vertex coordinates (nel+1 each): a graded box
 
 
 class BoxCase:
@@ -172,9 +173,13 @@ class BoxCase:
         self.exyz = (ex, ey, ez)
         coords = []
         for ax, (ea, nea) in enumerate(((ex, nelx), (ey, nely), (ez, nelz))):
-            lo, hi = spec.lo[ax], spec.hi[ax]
-            v0 = lo + (hi - lo) * ea / nea
-            v1 = lo + (hi - lo) * (ea + 1) / nea
+            if spec.verts is not None:
+                vv = np.asarray(spec.verts[ax], dtype=np.float64)
+                v0, v1 = vv[ea], vv[ea + 1]
+            else:
+                lo, hi = spec.lo[ax], spec.hi[ax]
+                v0 = lo + (hi - lo) * ea / nea
+                v1 = lo + (hi - lo) * (ea + 1) / nea
             h0 = 0.5 * (1.0 - self.z)
             h1 = 0.5 * (1.0 + self.z)
             line = v0[:, None] * h0[None, :] + v1[:, None] * h1[None, :]   # (nelt, n1)
