@@ -135,6 +135,54 @@ int32_t nkc_residual(nkc_handle *h, double rktime, double *reshn, double *resen,
 int32_t nkc_set_source(nkc_handle *h, const double *amp, const double dir[3],
                        double t0, double tw);
 
+/* The source's time factor: kind 0 the Gaussian pulse of nkc_set_source
+ * (which restores it), kind 1 g(t) = sin(omega t + phase) -- the dipole of
+ * tests/3dboxpml/3dboxpml.usr:30-88 (tfac = sin(-omega tt): omega < 0,
+ * phase 0). */
+int32_t nkc_set_source_time(nkc_handle *h, int32_t kind, double omega, double phase);
+
+/* PML (src/cem_maxwell_pml.F).  The stage adds pml_step
+ * (src/cem_maxwell_pml.F:508-592) after the flux and before usersrc, and
+ * advances the auxiliary fields B = pmlbn and D = pmldn by the same RK stage
+ * (rk_maxwell_ab, src/cem_maxwell.F:1942-1954).  Faces with the boundary
+ * code 'PML' are walls (cem_maxwell_flux_pec treats them as PEC): build the
+ * mesh with them as PEC.
+ * nkc_set_pml takes the reference's own arrays: pmlptr (maxpml element
+ * numbers, 1-based, ascending), pmlsigma(lpts1,3) and the initial pmlbn,
+ * pmldn (3 components at stride ld each; NULL for zero).  maxpml = 0
+ * removes the PML.
+ * nkc_setup_pml builds them as cem_maxwell_init does (pml_fill_faceary,
+ * march_faces, pml_extent_and_tags, pml_calc_sigma; src/cem_maxwell_pml.F:
+ * 85-505; thick, order, referr = param(77..79)) from pml_face[6 nelt], 1
+ * for the faces whose code is 'PML' (reference face order 1..6 per
+ * element): the layers are the `thick` elements inward from those faces,
+ * sigma follows from the node coordinates (box extents or nkc_set_coords)
+ * and the current eps and mu; B = D = 0.  extents (12 or NULL) receives
+ * pmlinner[6] then pmlouter[6] (order x-, x+, y-, y+, z-, z+).
+ * nkc_set_pml_fields sets B and D (as userini does) and zeroes their RK
+ * register.  nkc_get_pml copies back (which = 0) B then D, (1) their RK
+ * register, (2) the last residual from nkc_residual (respmlbn, respmldn),
+ * each 6 components at stride ld, or (3) sigma (3 components), on the PML
+ * elements' nodes only.  The adjoint refuses a PML. */
+int32_t nkc_set_pml(nkc_handle *h, int64_t maxpml, const int32_t *pmlptr, const double *sigma,
+                    const double *bn, const double *dn, int64_t ld);
+int32_t nkc_setup_pml(nkc_handle *h, const int8_t *pml_face, int32_t thick, double order,
+                      double referr, double *extents);
+int32_t nkc_set_pml_fields(nkc_handle *h, const double *bn, const double *dn, int64_t ld);
+int32_t nkc_get_pml(nkc_handle *h, int32_t which, double *out, int64_t ld);
+
+/* Incident plane wave, the device form of userinc
+ * (tests/3ddielectric/3ddielectric.usr:6-47, called by cem_maxwell_op,
+ * src/cem_maxwell.F:497-498): on each listed face the face traces get
+ *   H += hamp u,  E += eamp u,  u = cos(k . x - omega t)
+ * before the flux, x the trace's node and t the stage time, and the
+ * neighbour across the face sees the same traces.  faces holds nface pairs
+ * (element 1-based, face 1..6 in the reference's face order).  Needs node
+ * coordinates (box extents or nkc_set_coords).  nface = 0 removes it.  The
+ * adjoint refuses it. */
+int32_t nkc_set_incident(nkc_handle *h, int64_t nface, const int32_t *faces, const double k[3],
+                         double omega, const double hamp[3], const double eamp[3]);
+
 /* Drude ADE of cem_maxwell_drude (src/cem_maxwell.F:3095-3147) over the
  * ndrude volume nodes dindex (1-based): resE -= J*bmn, resJ = -a J + b E,
  * J advanced by the same low-storage RK.  params is (a, b) per listed node

@@ -79,6 +79,12 @@ SIGNATURES = [
     ("nkc_adjoint_get_drude", C.c_int32, [C.c_void_p, _dp, _dp]),
     ("nkc_set_lorentz", C.c_int32, [C.c_void_p, C.c_int64, _ip, _dp, _dp, _dp]),
     ("nkc_get_lorentz", C.c_int32, [C.c_void_p, _dp, _dp, _dp]),
+    ("nkc_set_source_time", C.c_int32, [C.c_void_p, C.c_int32, C.c_double, C.c_double]),
+    ("nkc_set_pml", C.c_int32, [C.c_void_p, C.c_int64, _ip, _dp, _dp, _dp, C.c_int64]),
+    ("nkc_setup_pml", C.c_int32, [C.c_void_p, C.POINTER(C.c_int8), C.c_int32, C.c_double, C.c_double, _dp]),
+    ("nkc_set_pml_fields", C.c_int32, [C.c_void_p, _dp, _dp, C.c_int64]),
+    ("nkc_get_pml", C.c_int32, [C.c_void_p, C.c_int32, _dp, C.c_int64]),
+    ("nkc_set_incident", C.c_int32, [C.c_void_p, C.c_int64, _ip, _dp, C.c_double, _dp, _dp]),
     ("nkc_set_layout", C.c_int32, [C.c_void_p, C.c_int32]),
     ("nkc_tune", C.c_int32, [C.c_void_p, _dp]),
     ("nkc_get_layout", C.c_int32, [C.c_void_p, _dp]),

@@ -408,7 +408,8 @@ inline int32_t build_box(const nkc_box_desc *d, HostMesh &m) {
 // 6: t = +1) a 3-character boundary code and 5 parameters.  Codes: 'E'
 // interior and 'P' periodic, with the neighbour element and face in
 // parameters 1-2 (1-based) or 0 to find it by the face's position (a .re2
-// lists only boundary faces); 'PEC' a perfect conductor.  Each face must
+// lists only boundary faces); 'PEC' a perfect conductor, and so 'PML' (the
+// wall behind an absorbing layer).  Each face must
 // pair with the opposite face of its neighbour over the same extent.
 // ---------------------------------------------------------------------------
 inline int32_t build_hex(int n1, int ifupwind, int64_t nelt, const double *xc, const double *yc,
@@ -476,7 +477,9 @@ inline int32_t build_hex(int n1, int ifupwind, int64_t nelt, const double *xc, c
             while (!code.empty() && code.back() == ' ') code.pop_back();
             const double *p = bc + (e * 6 + f) * 5;
             int32_t nb;
-            if (code == "PEC") {
+            if (code == "PEC" || code == "PML") {
+                // cem_maxwell_pec_init (src/cem_maxwell.F:1340-1366) lists
+                // both; the PML layers themselves come from nkc_setup_pml
                 nb = HOST_FACE_PEC;
             } else if (code == "E" || code == "P") {
                 int64_t ne = (int64_t)std::llround(p[0]) - 1;

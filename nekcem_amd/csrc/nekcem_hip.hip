@@ -131,7 +131,7 @@ constexpr int NLAYOUT = 4;
 
 template <int N1, int MAT, int MODE, int PACK, int OCC>
 void launch_t(const StageArgs &a, int grid, hipStream_t s) {
-    if (a.src_off || a.drude_off || a.lor_off)
+    if (a.src_off || a.drude_off || a.lor_off || a.pml_off || a.inc_mask)
         hipLaunchKernelGGL((stage_kernel<N1, MAT, MODE, PACK, OCC, 1>), dim3(grid), dim3(Shape<N1, PACK>::BS), 0, s, a);
     else
         hipLaunchKernelGGL((stage_kernel<N1, MAT, MODE, PACK, OCC, 0>), dim3(grid), dim3(Shape<N1, PACK>::BS), 0, s, a);
@@ -209,20 +209,22 @@ int32_t stage_kflags(int s) {
 // Autotuner over the stage kernel's layouts (lane packing x register
 // budget): time each on this mesh, alternating, best of three rounds of
 // four launches, and keep the fastest.  It writes only a scratch output and
-// a scratch RK register, and with the handle's source, Drude and Lorentz
-// regions (feat, may be null) it times the kernel instance the step runs,
-// the auxiliary currents and their registers redirected to scratch: the
-// state is untouched.  A packed layout whose workgroup equals the unpacked
-// one is not timed.  jsz, lsz: doubles of the Drude / Lorentz state blocks.
-int32_t tune_stage(DevMesh &dm, const StageArgs *feat, size_t jsz, size_t lsz, const double *uin,
+// a scratch RK register, and with the handle's source, Drude, Lorentz, PML
+// and incident-field set-up (feat, may be null) it times the kernel
+// instance the step runs, the auxiliary fields and their registers
+// redirected to scratch: the state is untouched.  A packed layout whose
+// workgroup equals the unpacked one is not timed.  jsz, lsz, psz: doubles of
+// the Drude / Lorentz / PML state blocks.
+int32_t tune_stage(DevMesh &dm, const StageArgs *feat, size_t jsz, size_t lsz, size_t psz, const double *uin,
                    hipStream_t s, double us[NLAYOUT]) {
     for (int l = 0; l < NLAYOUT; ++l) us[l] = NAN;
     const bool same = stage_block(dm.n1, 0) == stage_block(dm.n1, 1);
     const size_t blk = 6 * (size_t)dm.fs;
     const size_t jx = (feat && feat->drude_off) ? jsz : 0;
     const size_t lx = (feat && feat->lor_off) ? lsz : 0;
+    const size_t px = (feat && feat->pml_off) ? psz : 0;
     double *scr = nullptr;
-    int32_t rc = dev_alloc(&scr, 2 * blk + 2 * jx + 2 * lx);
+    int32_t rc = dev_alloc(&scr, 2 * blk + 2 * jx + 2 * lx + 2 * px);
     if (rc) return rc;
     StageArgs a = mesh_args(dm);
     if (feat) {
@@ -232,6 +234,23 @@ int32_t tune_stage(DevMesh &dm, const StageArgs *feat, size_t jsz, size_t lsz, c
         for (int c = 0; c < 3; ++c) a.src_dir[c] = feat->src_dir[c];
         a.src_t0 = feat->src_t0;
         a.src_tw = feat->src_tw;
+        a.src_kind = feat->src_kind;
+        a.src_om = feat->src_om;
+        a.src_ph = feat->src_ph;
+        a.inc_mask = feat->inc_mask;
+        a.xyz = feat->xyz;
+        for (int c = 0; c < 3; ++c) {
+            a.inc_k[c] = feat->inc_k[c];
+            a.inc_h[c] = feat->inc_h[c];
+            a.inc_e[c] = feat->inc_e[c];
+        }
+        a.inc_om = feat->inc_om;
+        if (px) {
+            a.pml_off = feat->pml_off;
+            a.psig = feat->psig;
+            a.pP = scr + 2 * blk + 2 * jx + 2 * lx;
+            a.pkP = scr + 2 * blk + 2 * jx + 2 * lx + px;
+        }
         if (jx) {
             a.drude_off = feat->drude_off;
             a.dpar = feat->dpar;
@@ -339,6 +358,17 @@ struct nkc_handle {
     int32_t *src_off = nullptr;
     double *src_amp = nullptr;
     double src_dir[3] = {0, 0, 0}, src_t0 = 0, src_tw = 1;
+    int32_t src_kind = 0;                 // 0 Gaussian pulse, 1 sin(src_om t + src_ph)
+    double src_om = 0, src_ph = 0;
+    // PML (compact per PML element: sigma 3 n3; (B, D) 6 n3, register, residual)
+    int64_t npml = 0;
+    std::vector<int64_t> pml_elems;       // PML element -> mesh element
+    int32_t *pml_off = nullptr;
+    double *psig = nullptr, *pP = nullptr, *pkP = nullptr, *presP = nullptr;
+    // incident plane wave: per element face bits, wave parameters
+    int64_t ninc = 0;
+    int32_t *inc_mask = nullptr;
+    double inc_k[3] = {0, 0, 0}, inc_om = 0, inc_h[3] = {0, 0, 0}, inc_e[3] = {0, 0, 0};
     // Drude
     int64_t ndrude = 0, ndelem = 0;
     int32_t *drude_off = nullptr;
@@ -363,6 +393,7 @@ struct nkc_handle {
     int64_t box_nel[3] = {0, 0, 0};
     double box_lo[3] = {0, 0, 0}, box_hi[3] = {0, 0, 0};
     double *xyz = nullptr, *djac = nullptr, *dw = nullptr, *dpart = nullptr, *dout = nullptr;
+    std::vector<double> hxyz;             // host copy of xyz
     // adjoint: checkpoints of the state every `every` steps of the last
     // nkc_adjoint_forward, the stage store of one segment, adjoint buffers
     struct Adj {
@@ -414,6 +445,12 @@ size_t lorentz_size(const nkc_handle *h) {
     return (size_t)h->nlelem * 6 * n3;
 }
 
+// doubles in one compact PML state block (npml x 6 x n3)
+size_t pml_size(const nkc_handle *h) {
+    const size_t n3 = (size_t)h->m.n1 * h->m.n1 * h->m.n1;
+    return (size_t)h->npml * 6 * n3;
+}
+
 StageArgs base_args(nkc_handle *h) {
     StageArgs a = mesh_args(h->m);
     a.tdev = h->tdev;
@@ -432,6 +469,22 @@ StageArgs base_args(nkc_handle *h) {
     a.lkL = h->lkL;
     a.lpar = h->lpar;
     a.lresL = h->lresL;
+    a.src_kind = h->src_kind;
+    a.src_om = h->src_om;
+    a.src_ph = h->src_ph;
+    a.pml_off = h->pml_off;
+    a.psig = h->psig;
+    a.pP = h->pP;
+    a.pkP = h->pkP;
+    a.presP = h->presP;
+    a.inc_mask = h->inc_mask;
+    a.xyz = h->xyz;
+    for (int c = 0; c < 3; ++c) {
+        a.inc_k[c] = h->inc_k[c];
+        a.inc_h[c] = h->inc_h[c];
+        a.inc_e[c] = h->inc_e[c];
+    }
+    a.inc_om = h->inc_om;
     return a;
 }
 
@@ -516,6 +569,8 @@ void free_handle(nkc_handle *h) {
     (void)hipFree(h->dpar); (void)hipFree(h->dresJ);
     (void)hipFree(h->lor_off); (void)hipFree(h->lL); (void)hipFree(h->lkL);
     (void)hipFree(h->lpar); (void)hipFree(h->lresL);
+    (void)hipFree(h->pml_off); (void)hipFree(h->psig); (void)hipFree(h->pP); (void)hipFree(h->pkP);
+    (void)hipFree(h->presP); (void)hipFree(h->inc_mask);
     (void)hipFree(h->xyz); (void)hipFree(h->djac); (void)hipFree(h->dw);
     (void)hipFree(h->dpart); (void)hipFree(h->dout);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -532,9 +587,9 @@ int32_t ensure_diag(nkc_handle *h) {
     return NKC_OK;
 }
 
-// Node coordinates on the device: a box handle builds them from its extents
-// (the GLL lines of genbox, element-major, i fastest); others need
-// nkc_set_coords.
+// Node coordinates on the host and the device: a box handle builds them
+// from its extents (the GLL lines of genbox, element-major, i fastest);
+// others need nkc_set_coords.
 int32_t ensure_coords(nkc_handle *h) {
     if (h->xyz) return NKC_OK;
     if (!h->is_box) return fail(NKC_ERR_ARG, "no node coordinates: call nkc_set_coords first");
@@ -560,6 +615,7 @@ int32_t ensure_coords(nkc_handle *h) {
         xyz[fs + p] = line[1][ey * n1 + (r / n1) % n1];
         xyz[2 * fs + p] = line[2][ez * n1 + r / n2];
     }
+    h->hxyz = xyz;
     return upload(&h->xyz, xyz);
 }
 
@@ -881,7 +937,7 @@ int32_t nkc_step(nkc_handle *h, double *time, double dt, int64_t nsteps) {
     }
     if (h->m.layout < 0) {
         const StageArgs f = base_args(h);
-        const int32_t rc = tune_stage(h->m, &f, drude_size(h), lorentz_size(h), h->m.u[h->cur], h->stream, h->tune_us);
+        const int32_t rc = tune_stage(h->m, &f, drude_size(h), lorentz_size(h), pml_size(h), h->m.u[h->cur], h->stream, h->tune_us);
         if (rc) return rc;
     }
     if (h->use_graph) {
@@ -957,7 +1013,20 @@ int32_t nkc_set_source(nkc_handle *h, const double *amp, const double dir[3], do
     for (int c = 0; c < 3; ++c) h->src_dir[c] = dir[c];
     h->src_t0 = t0;
     h->src_tw = tw;
+    h->src_kind = 0;
     return NKC_OK;
+}
+
+int32_t nkc_set_source_time(nkc_handle *h, int32_t kind, double omega, double phase) {
+    if (!h) return fail(NKC_ERR_ARG, "null handle");
+    if (kind != 0 && kind != 1) return fail(NKC_ERR_ARG, "source time kind %d not 0 or 1", kind);
+    if (!std::isfinite(omega) || !std::isfinite(phase)) return fail(NKC_ERR_ARG, "source omega/phase not finite");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    free_adjoint(h);                      // the stored trajectory has the old source
+    h->src_kind = kind;
+    h->src_om = omega;
+    h->src_ph = phase;
+    return drop_graphs(h);                // the captured stages carry the old factor
 }
 
 int32_t nkc_set_drude(nkc_handle *h, int64_t ndrude, const int32_t *dindex, const double *params,
@@ -1036,8 +1105,12 @@ int32_t nkc_set_coords(nkc_handle *h, const double *xm1, const double *ym1, cons
     std::copy(ym1, ym1 + npts, xyz.begin() + fs);
     std::copy(zm1, zm1 + npts, xyz.begin() + 2 * fs);
     HIPCHK(hipStreamSynchronize(h->stream));
-    (void)hipFree(h->xyz);
-    h->xyz = nullptr;
+    h->hxyz = xyz;
+    if (h->xyz) {
+        // in place: captured steps that read the coordinates stay valid
+        HIPCHK(hipMemcpy(h->xyz, xyz.data(), sizeof(double) * xyz.size(), hipMemcpyHostToDevice));
+        return NKC_OK;
+    }
     return upload(&h->xyz, xyz);
 }
 
@@ -1166,6 +1239,231 @@ int32_t nkc_get_lorentz(nkc_handle *h, double *L, double *kL, double *resL) {
     return NKC_OK;
 }
 
+namespace {
+
+void free_pml(nkc_handle *h) {
+    (void)hipFree(h->pml_off); h->pml_off = nullptr;
+    (void)hipFree(h->psig); h->psig = nullptr;
+    (void)hipFree(h->pP); h->pP = nullptr;
+    (void)hipFree(h->pkP); h->pkP = nullptr;
+    (void)hipFree(h->presP); h->presP = nullptr;
+    h->npml = 0;
+    h->pml_elems.clear();
+}
+
+// Install a PML on the mesh elements `elems` (0-based, ascending): sig and
+// P are compact (3 n3 and 6 n3 doubles per PML element).
+int32_t install_pml(nkc_handle *h, const std::vector<int64_t> &elems, const std::vector<double> &sig,
+                    const std::vector<double> &P) {
+    std::vector<int32_t> off(h->m.nelt, -1);
+    for (size_t l = 0; l < elems.size(); ++l) off[elems[l]] = (int32_t)l;
+    std::vector<double> zero(P.size(), 0.0);
+    int32_t rc;
+    if ((rc = upload(&h->pml_off, off))) return rc;
+    if ((rc = upload(&h->psig, sig))) return rc;
+    if ((rc = upload(&h->pP, P))) return rc;
+    if ((rc = upload(&h->pkP, zero))) return rc;
+    if ((rc = upload(&h->presP, zero))) return rc;
+    h->npml = (int64_t)elems.size();
+    h->pml_elems = elems;
+    return NKC_OK;
+}
+
+// Common start of the PML set-up entry points: nothing stored survives.
+int32_t reset_pml(nkc_handle *h) {
+    HIPCHK(hipStreamSynchronize(h->stream));
+    free_adjoint(h);
+    h->m.layout = -1;                     // the step's kernel instance may change: tune again
+    free_pml(h);
+    return drop_graphs(h);
+}
+
+}  // namespace
+
+int32_t nkc_set_pml(nkc_handle *h, int64_t maxpml, const int32_t *pmlptr, const double *sigma, const double *bn,
+                    const double *dn, int64_t ld) {
+    if (!h) return fail(NKC_ERR_ARG, "null handle");
+    int32_t rc = reset_pml(h);
+    if (rc || maxpml == 0) return rc;
+    const DevMesh &m = h->m;
+    if (maxpml < 0 || maxpml > m.nelt || !pmlptr || !sigma) return fail(NKC_ERR_ARG, "bad PML element list");
+    if (ld < m.npts) return fail(NKC_ERR_ARG, "ld < npts");
+    const int64_t n3 = (int64_t)m.n1 * m.n1 * m.n1;
+    std::vector<int64_t> elems(maxpml);
+    for (int64_t l = 0; l < maxpml; ++l) {
+        elems[l] = (int64_t)pmlptr[l] - 1;
+        if (elems[l] < 0 || elems[l] >= m.nelt) return fail(NKC_ERR_ARG, "pmlptr[%lld]=%d out of range", (long long)l, pmlptr[l]);
+        if (l > 0 && elems[l] <= elems[l - 1]) return fail(NKC_ERR_ARG, "pmlptr must be ascending");
+    }
+    std::vector<double> sig((size_t)maxpml * 3 * n3), P((size_t)maxpml * 6 * n3, 0.0);
+    for (int64_t l = 0; l < maxpml; ++l)
+        for (int64_t v = 0; v < n3; ++v) {
+            const int64_t g = elems[l] * n3 + v;
+            for (int c = 0; c < 3; ++c) {
+                sig[(l * 3 + c) * n3 + v] = sigma[c * ld + g];
+                if (bn) P[(l * 6 + c) * n3 + v] = bn[c * ld + g];
+                if (dn) P[(l * 6 + 3 + c) * n3 + v] = dn[c * ld + g];
+            }
+        }
+    return install_pml(h, elems, sig, P);
+}
+
+int32_t nkc_setup_pml(nkc_handle *h, const int8_t *pml_face, int32_t thick, double order, double referr,
+                      double *extents) {
+    if (!h) return fail(NKC_ERR_ARG, "null handle");
+    int32_t rc = reset_pml(h);
+    if (rc || !pml_face) return rc;
+    // pml_errchk (src/cem_maxwell_pml.F:2-29)
+    if (thick < 1 || thick > 10) return fail(NKC_ERR_ARG, "PML thickness %d not in 1..10", thick);
+    if (!(order >= 1.0 && order <= 10.0)) return fail(NKC_ERR_ARG, "PML order %g not in 1..10", order);
+    if (!(referr > 0.0 && referr <= 1.0)) return fail(NKC_ERR_ARG, "PML reflection error %g not in (0, 1]", referr);
+    if ((rc = ensure_coords(h))) return rc;
+    const DevMesh &m = h->m;
+    const HostMesh &hm = h->hm;
+    const int64_t nelt = m.nelt, n3 = (int64_t)m.n1 * m.n1 * m.n1, fs = m.fs;
+    const int *kf_ed = kf_of_ed();
+    // the layers: `thick` elements inward from every PML face, along the
+    // face's axis (march_faces on a conforming axis-aligned mesh)
+    std::vector<int32_t> tag(nelt, 0);
+    for (int64_t e = 0; e < nelt; ++e)
+        for (int f = 1; f <= 6; ++f) {
+            if (!pml_face[e * 6 + f - 1]) continue;
+            const int kf = kf_ed[f];
+            if (hm.nbr[e * 6 + kf] >= 0) return fail(NKC_ERR_ARG, "PML face %d of element %lld is not a wall", f, (long long)e + 1);
+            int64_t cur = e;
+            for (int l = 0; l < thick; ++l) {
+                tag[cur] |= 1 << kf;
+                if ((tag[cur] >> (kf ^ 1)) & 1) return fail(NKC_ERR_ARG, "PML layers collide in element %lld", (long long)cur + 1);
+                if (l + 1 == thick) break;
+                const int32_t nb = hm.nbr[cur * 6 + (kf ^ 1)];
+                if (nb < 0) return fail(NKC_ERR_ARG, "the PML is thicker than the mesh at element %lld", (long long)cur + 1);
+                cur = nb;
+            }
+        }
+    // pml_extent_and_tags (src/cem_maxwell_pml.F:261-431): inner / outer
+    // coordinate of each of the six layers
+    const double inf = 1e20;
+    double inner[6] = {-inf, inf, -inf, inf, -inf, inf}, outer[6] = {inf, -inf, inf, -inf, inf, -inf};
+    std::vector<int64_t> elems;
+    for (int64_t e = 0; e < nelt; ++e) {
+        if (!tag[e]) continue;
+        elems.push_back(e);
+        for (int f = 0; f < 6; ++f) {
+            if (!((tag[e] >> f) & 1)) continue;
+            const int ax = f / 2;
+            double lo = inf, hi = -inf;
+            for (int64_t v = 0; v < n3; ++v) {
+                const double x = h->hxyz[ax * fs + e * n3 + v];
+                lo = std::min(lo, x);
+                hi = std::max(hi, x);
+            }
+            if (f & 1) { inner[f] = std::min(inner[f], lo); outer[f] = std::max(outer[f], hi); }
+            else       { inner[f] = std::max(inner[f], hi); outer[f] = std::min(outer[f], lo); }
+        }
+    }
+    // pml_calc_sigma (src/cem_maxwell_pml.F:433-505): sigma_axis =
+    // sigmamax ((x - inner) / (outer - inner))^order, sigmamax =
+    // -(order + 1) log(referr) / (2 eta width), eta = sqrt(mu / eps)
+    std::vector<double> sig(elems.size() * 3 * n3, 0.0), P(elems.size() * 6 * n3, 0.0);
+    for (size_t l = 0; l < elems.size(); ++l) {
+        const int64_t e = elems[l];
+        for (int f = 0; f < 6; ++f) {
+            if (!((tag[e] >> f) & 1)) continue;
+            const int ax = f / 2;
+            const double width = std::fabs(outer[f] - inner[f]);
+            for (int64_t v = 0; v < n3; ++v) {
+                const int64_t g = e * n3 + v;
+                const size_t mi = hm.mat ? (size_t)g : (size_t)e;
+                const double eta = std::sqrt((1.0 / hm.imu[mi]) / (1.0 / hm.ieps[mi]));
+                const double smax = -(order + 1.0) * std::log(referr) / (2.0 * eta * width);
+                const double z01 = (h->hxyz[ax * fs + g] - inner[f]) / (outer[f] - inner[f]);
+                sig[(l * 3 + ax) * n3 + v] = smax * std::pow(z01, order);
+            }
+        }
+    }
+    if (extents)
+        for (int f = 0; f < 6; ++f) { extents[f] = inner[f]; extents[6 + f] = outer[f]; }
+    if (elems.empty()) return NKC_OK;
+    return install_pml(h, elems, sig, P);
+}
+
+int32_t nkc_set_pml_fields(nkc_handle *h, const double *bn, const double *dn, int64_t ld) {
+    if (!h) return fail(NKC_ERR_ARG, "null handle");
+    if (h->npml == 0) return fail(NKC_ERR_ARG, "no PML set");
+    if (ld < h->m.npts) return fail(NKC_ERR_ARG, "ld < npts");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    free_adjoint(h);
+    const int64_t n3 = (int64_t)h->m.n1 * h->m.n1 * h->m.n1;
+    std::vector<double> P(pml_size(h), 0.0);
+    for (size_t l = 0; l < h->pml_elems.size(); ++l)
+        for (int64_t v = 0; v < n3; ++v) {
+            const int64_t g = h->pml_elems[l] * n3 + v;
+            for (int c = 0; c < 3; ++c) {
+                if (bn) P[(l * 6 + c) * n3 + v] = bn[c * ld + g];
+                if (dn) P[(l * 6 + 3 + c) * n3 + v] = dn[c * ld + g];
+            }
+        }
+    HIPCHK(hipMemcpy(h->pP, P.data(), sizeof(double) * P.size(), hipMemcpyHostToDevice));
+    std::fill(P.begin(), P.end(), 0.0);
+    HIPCHK(hipMemcpy(h->pkP, P.data(), sizeof(double) * P.size(), hipMemcpyHostToDevice));
+    return NKC_OK;
+}
+
+int32_t nkc_get_pml(nkc_handle *h, int32_t which, double *out, int64_t ld) {
+    if (!h || !out) return fail(NKC_ERR_ARG, "null handle or output");
+    if (h->npml == 0) return fail(NKC_ERR_ARG, "no PML set");
+    if (which < 0 || which > 3) return fail(NKC_ERR_ARG, "which %d not in 0..3", which);
+    if (ld < h->m.npts) return fail(NKC_ERR_ARG, "ld < npts");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    const int64_t n3 = (int64_t)h->m.n1 * h->m.n1 * h->m.n1;
+    const int nc = which == 3 ? 3 : 6;
+    std::vector<double> buf(h->npml * nc * n3);
+    const double *src = which == 0 ? h->pP : which == 1 ? h->pkP : which == 2 ? h->presP : h->psig;
+    HIPCHK(hipMemcpy(buf.data(), src, sizeof(double) * buf.size(), hipMemcpyDeviceToHost));
+    for (size_t l = 0; l < h->pml_elems.size(); ++l)
+        for (int c = 0; c < nc; ++c)
+            for (int64_t v = 0; v < n3; ++v) out[c * ld + h->pml_elems[l] * n3 + v] = buf[(l * nc + c) * n3 + v];
+    return NKC_OK;
+}
+
+int32_t nkc_set_incident(nkc_handle *h, int64_t nface, const int32_t *faces, const double k[3], double omega,
+                         const double hamp[3], const double eamp[3]) {
+    if (!h) return fail(NKC_ERR_ARG, "null handle");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    free_adjoint(h);
+    h->m.layout = -1;                     // the step's kernel instance may change: tune again
+    (void)hipFree(h->inc_mask); h->inc_mask = nullptr;
+    h->ninc = 0;
+    int32_t rc = drop_graphs(h);
+    if (rc || nface == 0) return rc;
+    if (nface < 0 || !faces || !k || !hamp || !eamp) return fail(NKC_ERR_ARG, "bad incident-field face list");
+    for (int c = 0; c < 3; ++c)
+        if (!std::isfinite(k[c]) || !std::isfinite(hamp[c]) || !std::isfinite(eamp[c]))
+            return fail(NKC_ERR_ARG, "incident-field parameters not finite");
+    if ((rc = ensure_coords(h))) return rc;
+    const int64_t nelt = h->m.nelt;
+    const int *kf_ed = kf_of_ed();
+    std::vector<int32_t> mask(nelt, 0);
+    for (int64_t l = 0; l < nface; ++l) {
+        const int64_t e = (int64_t)faces[2 * l] - 1;
+        const int f = faces[2 * l + 1];
+        if (e < 0 || e >= nelt || f < 1 || f > 6) return fail(NKC_ERR_ARG, "incident face %lld: (%d, %d) out of range", (long long)l, faces[2 * l], f);
+        const int kf = kf_ed[f];
+        mask[e] |= 1 << kf;
+        const int32_t nb = h->hm.nbr[e * 6 + kf];
+        if (nb >= 0) mask[nb] |= 1 << (8 + (kf ^ 1));   // the neighbour sees this trace across its opposite face
+    }
+    if ((rc = upload(&h->inc_mask, mask))) return rc;
+    for (int c = 0; c < 3; ++c) {
+        h->inc_k[c] = k[c];
+        h->inc_h[c] = hamp[c];
+        h->inc_e[c] = eamp[c];
+    }
+    h->inc_om = omega;
+    h->ninc = nface;
+    return NKC_OK;
+}
+
 int32_t nkc_check_finite(nkc_handle *h, double *maxabs) {
     if (!h) return fail(NKC_ERR_ARG, "null handle");
     hipLaunchKernelGGL(zero_u64, dim3(1), dim3(1), 0, h->stream, h->red);
@@ -1235,7 +1533,7 @@ int32_t nkc_tune(nkc_handle *h, double *out) {
     if (!h) return fail(NKC_ERR_ARG, "null handle");
     HIPCHK(hipStreamSynchronize(h->stream));
     const StageArgs f = base_args(h);
-    int32_t rc = tune_stage(h->m, &f, drude_size(h), lorentz_size(h), h->m.u[h->cur], h->stream, h->tune_us);
+    int32_t rc = tune_stage(h->m, &f, drude_size(h), lorentz_size(h), pml_size(h), h->m.u[h->cur], h->stream, h->tune_us);
     if (rc) return rc;
     if ((rc = drop_graphs(h))) return rc;
     if (out) {
@@ -1480,6 +1778,8 @@ int32_t nkc_adjoint_forward(nkc_handle *h, double *time, double dt, int64_t nste
     if (!h) return fail(NKC_ERR_ARG, "null handle");
     if (nsteps <= 0 || every <= 0 || !std::isfinite(dt)) return fail(NKC_ERR_ARG, "bad adjoint forward arguments");
     if (h->nlor) return fail(NKC_ERR_UNSUPPORTED, "the adjoint of a Lorentz region is not built");
+    if (h->npml) return fail(NKC_ERR_UNSUPPORTED, "the adjoint of a PML is not built");
+    if (h->ninc) return fail(NKC_ERR_UNSUPPORTED, "the adjoint with an incident field is not built");
     free_adjoint(h);
     auto &A = h->adj;
     A.nsteps = nsteps;
@@ -1787,7 +2087,7 @@ int32_t nkc_batch_step(nkc_batch *b, const double *dt, double *times, int64_t ns
     for (auto &dm : b->bins)
         if (dm.layout < 0) {
             double us[NLAYOUT];
-            const int32_t rc = tune_stage(dm, nullptr, 0, 0, dm.u[b->cur], b->main, us);
+            const int32_t rc = tune_stage(dm, nullptr, 0, 0, 0, dm.u[b->cur], b->main, us);
             if (rc) return rc;
         }
     if (!b->graph_ok) {

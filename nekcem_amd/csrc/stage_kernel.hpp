@@ -89,10 +89,13 @@ struct StageArgs {
     const double *__restrict__ tdev;
     double rkc;
     // usersrc-style source on E: src_amp (un-weighted, per node of the
-    // elements with src_off[e] >= 0) * dir * exp(-((t-t0)/tw)^2)
+    // elements with src_off[e] >= 0) * dir * g(t); g(t) = exp(-((t-t0)/tw)^2)
+    // (src_kind 0) or sin(src_om t + src_ph) (src_kind 1)
     const int32_t *__restrict__ src_off;
     const double *__restrict__ src_amp;
     double src_dir[3], src_t0, src_tw;
+    int32_t src_kind;
+    double src_om, src_ph;
     // Drude ADE (cem_maxwell_drude): elements with drude_off[e] >= 0 carry
     // J, kJ ([3][n1^3] each) and (a, b) per node
     const int32_t *__restrict__ drude_off;
@@ -112,7 +115,34 @@ struct StageArgs {
     // fs) and the stage-input Drude current (layout of dJ) when non-null
     double *__restrict__ st_r;
     double *__restrict__ st_J;
+    // PML (pml_step): elements with pml_off[e] >= 0 carry sigma x, y, z
+    // (3 n1^3 doubles at pml_off[e] 3 n1^3) and the auxiliary fields B then
+    // D (6 n1^3 at pml_off[e] 6 n1^3) with their RK register
+    const int32_t *__restrict__ pml_off;
+    const double *__restrict__ psig;
+    double *__restrict__ pP;
+    double *__restrict__ pkP;
+    double *__restrict__ presP;       // MODE_RESIDUAL: (resB, resD) out
+    // userinc-style incident plane wave: inc_mask[e] bit f (kernel face
+    // order) adds it to the element's own trace on face f, bit 8 + f to the
+    // neighbour's trace seen across face f; u = cos(inc_k . x - inc_om t)
+    // at the trace's node (coordinates xyz, 3 x fs)
+    const int32_t *__restrict__ inc_mask;
+    const double *__restrict__ xyz;
+    double inc_k[3], inc_om, inc_h[3], inc_e[3];
 };
+
+// Incident plane wave at volume node p (global index), time t, added to a
+// face trace (H then E).
+__device__ __forceinline__ void add_incident(const StageArgs &a, int64_t p, double t, double tr[6]) {
+    const double u = cos(a.inc_k[0] * a.xyz[p] + a.inc_k[1] * a.xyz[a.fs + p] + a.inc_k[2] * a.xyz[2 * a.fs + p] -
+                         a.inc_om * t);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        tr[c] = tr[c] + a.inc_h[c] * u;
+        tr[3 + c] = tr[3 + c] + a.inc_e[c] * u;
+    }
+}
 
 // n x v for n = sg * e_AX.
 template <int AX>
@@ -259,15 +289,10 @@ stage_kernel(StageArgs a) {
                 kp[c][kk] = (k < N1) ? ld_stream(a.kbuf + (3 * grp + c) * fs + eo + k * N2 + q) : 0.0;
             }
     }
-    // source amplitude of this role's levels (E roles), with the HBM
-    // round trip of the other streams
+    // source region of this element (E roles); its amplitudes are read
+    // level by level in phase 2, so elements without a source hold no
+    // registers for them
     const int so = (FEAT && grp == 1 && a.src_off) ? a.src_off[e] : -1;
-    double samp[KH];
-#pragma unroll
-    for (int kk = 0; kk < KH; ++kk) {
-        const int k = k0 + kk;
-        samp[kk] = (so >= 0 && act && k < N1) ? a.src_amp[(int64_t)so * N3 + k * N2 + q] : 0.0;
-    }
     // neighbour trace for this role's x/y face (node (a,b) = (i,j))
     double nxy[6];
     int64_t nbn_xy = 0;
@@ -298,6 +323,10 @@ stage_kernel(StageArgs a) {
         }
     }
     const double *geo = a.geo + (int64_t)e * 9;
+    const int imask = (FEAT && a.inc_mask) ? a.inc_mask[e] : 0;
+    // stage time (usersrc, userinc)
+    const double dt = a.dt_e ? a.dt_e[e] : a.dt;
+    const double tstage = (FEAT && (a.src_off || imask)) ? *a.tdev + a.rkc * dt : 0.0;
     __syncthreads();
 
     // ---- phase 1: role r computes the flux of x/y face r at node (i,j) ----
@@ -313,6 +342,8 @@ stage_kernel(StageArgs a) {
         for (int c = 0; c < 6; ++c) um[c] = s_u[c * N3 + own];
         load_coef<N1, MAT>(a, e, role, q, cf);
         const double sg = (role & 1) ? 1.0 : -1.0;
+        if (FEAT && (imask >> role) & 1) add_incident(a, eo + own, tstage, um);
+        if (FEAT && (imask >> (8 + role)) & 1) add_incident(a, (int64_t)nbr_xy * N3 + nbn_xy, tstage, nxy);
         if (role < 2) face_flux<0>(sg, nbr_xy, um, nxy, cf, geo[3 + role], FH, FE);
         else          face_flux<1>(sg, nbr_xy, um, nxy, cf, geo[3 + role], FH, FE);
 #pragma unroll
@@ -342,17 +373,21 @@ stage_kernel(StageArgs a) {
     }
     double minv = 0.0;
     if (MAT == MAT_ELEM) minv = grp ? a.ieps[e] : a.imu[e];
-    const double dt = a.dt_e ? a.dt_e[e] : a.dt;
     double cfz[4];
     load_coef<N1, MAT>(a, e, zface, q, cfz);
     // usersrc time factor: one value per stage (and element, with per-element dt)
     double sg = 0.0;
     if (so >= 0) {
-        const double tt = (*a.tdev + a.rkc * dt - a.src_t0) / a.src_tw;
-        sg = exp(-tt * tt);
+        if (a.src_kind == 1) {
+            sg = sin(a.src_om * tstage + a.src_ph);
+        } else {
+            const double tt = (tstage - a.src_t0) / a.src_tw;
+            sg = exp(-tt * tt);
+        }
     }
     const int dd = (FEAT && grp == 1 && a.drude_off) ? a.drude_off[e] : -1;
     const int lo = (FEAT && grp == 1 && a.lor_off) ? a.lor_off[e] : -1;
+    const int po = (FEAT && a.pml_off) ? a.pml_off[e] : -1;
 #pragma unroll
     for (int kk = 0; kk < KH; ++kk) {
         const int k = k0 + kk;
@@ -381,6 +416,8 @@ stage_kernel(StageArgs a) {
             double um[6], FH[3], FE[3];
 #pragma unroll
             for (int c = 0; c < 6; ++c) um[c] = s_u[c * N3 + own];
+            if (FEAT && (imask >> zface) & 1) add_incident(a, eo + own, tstage, um);
+            if (FEAT && (imask >> (8 + zface)) & 1) add_incident(a, (int64_t)nbr_z * N3 + nbn_z, tstage, nz);
             face_flux<2>(half ? 1.0 : -1.0, nbr_z, um, nz, cfz, geo[3 + zface], FH, FE);
 #pragma unroll
             for (int c = 0; c < 3; ++c) r[c] += grp ? FE[c] : FH[c];
@@ -402,9 +439,36 @@ stage_kernel(StageArgs a) {
 #pragma unroll
             for (int c = 0; c < 3; ++c) r[c] += s_f[(3 * 6 + co + c) * N2 + i + N1 * k];
         }
+        const int64_t node = eo + k * N2 + q;
+        if (po >= 0) {
+            // pml_step: with sa, sb, sd = sigma/eps of the component's own,
+            // next and last axis, the auxiliary field P (B for H, D for E)
+            //   resP = res - sb P,  res += (sa - sb) P - sd mu H  (H)
+            //                       res += (sa - sb) P - sigma_d E (E)
+            const int64_t sb = (int64_t)po * 3 * N3 + vloc, pb = (int64_t)po * 6 * N3 + 3 * grp * N3 + vloc;
+            const double sig[3] = {a.psig[sb], a.psig[sb + N3], a.psig[sb + 2 * N3]};
+            const double iep = (MAT == MAT_NODE) ? a.ieps[node] : a.ieps[e];
+            const double mu = grp ? 0.0 : 1.0 / ((MAT == MAT_NODE) ? a.imu[node] : a.imu[e]);
+            const double sp[3] = {sig[0] * iep, sig[1] * iep, sig[2] * iep};
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const int cb = (c + 1) % 3, cd = (c + 2) % 3;
+                const double P = a.pP[pb + c * N3];
+                const double rP = r[c] - sp[cb] * P;
+                const double drive = grp ? sig[cd] * s_u[(3 + c) * N3 + own] : sp[cd] * mu * s_u[c * N3 + own];
+                r[c] += -sp[cb] * P + sp[c] * P - drive;
+                if (MODE == MODE_RESIDUAL) {
+                    a.presP[pb + c * N3] = rP;
+                } else {
+                    const double kq = a.rka * a.pkP[pb + c * N3] + dt * rP;
+                    a.pkP[pb + c * N3] = kq;
+                    a.pP[pb + c * N3] = P + a.rkb * kq;
+                }
+            }
+        }
         if (so >= 0) {
             // usersrc: resE += amp * bm1 * dir * g(t) before the inverse mass
-            const double am = samp[kk] * sg;
+            const double am = a.src_amp[(int64_t)so * N3 + vloc] * sg;
 #pragma unroll
             for (int c = 0; c < 3; ++c) r[c] += am * a.src_dir[c];
         }
@@ -454,7 +518,6 @@ stage_kernel(StageArgs a) {
             }
         }
         // inverse mass (the J w3 of hbm1/ebm1 is already divided out)
-        const int64_t node = eo + k * N2 + q;
         if (MAT == MAT_NODE) minv = grp ? a.ieps[node] : a.imu[node];
 #pragma unroll
         for (int c = 0; c < 3; ++c) r[c] *= minv;

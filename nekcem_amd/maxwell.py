@@ -52,7 +52,7 @@ def box_desc(nel, N, lo=(0.0, 0.0, 0.0), hi=(2 * np.pi,) * 3, bc=("P", "P", "P")
     d = _lib.NkcBoxDesc()
     d.N, d.ifupwind = int(N), 1 if ifupwind else 0
     d.nel = (C.c_int32 * 3)(*[int(x) for x in nel])
-    d.pec = (C.c_int32 * 3)(*[1 if b == "PEC" else 0 for b in bc])
+    d.pec = (C.c_int32 * 3)(*[1 if b in ("PEC", "PML") else 0 for b in bc])
     d.lo = (C.c_double * 3)(*[float(x) for x in lo])
     d.hi = (C.c_double * 3)(*[float(x) for x in hi])
     keep = []
@@ -64,6 +64,23 @@ def box_desc(nel, N, lo=(0.0, 0.0, 0.0), hi=(2 * np.pi,) * 3, bc=("P", "P", "P")
             setattr(d, name, _d(arr))
     d._keep = keep
     return d
+
+
+# reference face order 1..6 -> (axis, side): y-, x+, y+, x-, z-, z+
+_ED_FACES = ((1, 0), (0, 1), (1, 1), (0, 0), (2, 0), (2, 1))
+
+
+def box_pml_faces(nel, bc):
+    """(nelt, 6) int8: 1 on the faces of a box (elements x fastest, faces in
+    the reference's order) whose direction's bc is 'PML'."""
+    nel = tuple(int(n) for n in nel)
+    e = np.arange(int(np.prod(nel)))
+    ex = (e % nel[0], (e // nel[0]) % nel[1], e // (nel[0] * nel[1]))
+    out = np.zeros((len(e), 6), dtype=np.int8)
+    for f, (ax, side) in enumerate(_ED_FACES):
+        if bc[ax] == "PML":
+            out[:, f] = (ex[ax] == 0) if side == 0 else (ex[ax] == nel[ax] - 1)
+    return out
 
 
 class _BoxMeshInfo:
@@ -80,6 +97,9 @@ class _BoxMeshInfo:
     def coords(self):
         return self._coords(self.nel, self.N, self.lo, self.hi)
 
+    def pml_faces(self):
+        return box_pml_faces(self.nel, self.bc)
+
     def dxmin(self):
         return self._dxmin(self.nel, self.N, self.lo, self.hi)
 
@@ -91,10 +111,11 @@ class _HexMeshInfo:
     """What a solver built from element corners knows of its mesh: per
     element lower/upper corners of an axis-aligned box."""
 
-    def __init__(self, N, lo, hi):
+    def __init__(self, N, lo, hi, cbc=None):
         from .mesh import gll
         self.N, self.n1 = int(N), int(N) + 1
         self.lo, self.hi = np.asarray(lo, dtype=np.float64), np.asarray(hi, dtype=np.float64)
+        self.cbc = None if cbc is None else np.asarray(cbc).reshape(len(self.lo), 6)
         self.nelt = len(self.lo)
         self.npts = self.nelt * self.n1 ** 3
         self._z, self._w = gll(self.n1)
@@ -110,6 +131,12 @@ class _HexMeshInfo:
             v = np.broadcast_to(v.reshape((self.nelt,) + shape), (self.nelt, n1, n1, n1))
             out.append(np.ascontiguousarray(v).reshape(-1))
         return out
+
+    def pml_faces(self):
+        """(nelt, 6) int8: 1 where the boundary code is 'PML'."""
+        if self.cbc is None:
+            return np.zeros((len(self.lo), 6), dtype=np.int8)
+        return np.vectorize(lambda c: str(c).strip().upper() == "PML")(self.cbc).astype(np.int8)
 
     @property
     def bmn(self):
@@ -148,7 +175,7 @@ class MaxwellSolver:
         self = cls.__new__(cls)
         lo = np.stack([xc.min(axis=1), yc.min(axis=1), zc.min(axis=1)], axis=1)
         hi = np.stack([xc.max(axis=1), yc.max(axis=1), zc.max(axis=1)], axis=1)
-        self.mesh = _HexMeshInfo(N, lo, hi)
+        self.mesh = _HexMeshInfo(N, lo, hi, cbc)
         self.npts = self.mesh.npts
         self._h, self._L = h, L
         self.time, self.istep, self.dt, self.ndrude, self.nlor = 0.0, 0, None, 0, 0
@@ -251,9 +278,10 @@ class MaxwellSolver:
         self.time = t.value
         self.istep += int(nsteps)
 
-    def set_source(self, amp, direction=(0.0, 0.0, 1.0), t0=0.0, tw=1.0):
-        """usersrc-style E source: resE += amp * dir * exp(-((t-t0)/tw)^2)
-        with amp (npts) bm1-weighted; amp=None removes it."""
+    def set_source(self, amp, direction=(0.0, 0.0, 1.0), t0=0.0, tw=1.0, omega=None, phase=0.0):
+        """usersrc-style E source: resE += amp * dir * g(t) with amp (npts)
+        bm1-weighted, g(t) = exp(-((t-t0)/tw)^2), or sin(omega t + phase)
+        when omega is given; amp=None removes it."""
         if amp is None:
             _lib.check(self._L.nkc_set_source(self._h, None, None, 0.0, 1.0))
             return
@@ -261,6 +289,66 @@ class MaxwellSolver:
         d = np.ascontiguousarray(direction, dtype=np.float64)
         self._src_keep = (amp, d)
         _lib.check(self._L.nkc_set_source(self._h, _d(amp), _d(d), float(t0), float(tw)))
+        if omega is not None:
+            _lib.check(self._L.nkc_set_source_time(self._h, 1, float(omega), float(phase)))
+
+    # -- PML and incident field --------------------------------------------
+    def _pml_faces(self):
+        if hasattr(self.mesh, "pml_faces"):
+            return self.mesh.pml_faces()
+        return box_pml_faces(self.mesh.nel, self.mesh.bc)
+
+    def setup_pml(self, thick, order, referr, pml_faces=None):
+        """PML layers on the 'PML' walls as cem_maxwell_init sets them up
+        (thick, order, referr = param(77..79)); B = D = 0.  Returns
+        dict(inner, outer): pmlinner / pmlouter in the order x-, x+, y-, y+,
+        z-, z+."""
+        self._ensure_coords()
+        pf = self._pml_faces() if pml_faces is None else pml_faces
+        pf = np.ascontiguousarray(pf, dtype=np.int8).reshape(-1)
+        ext = np.empty(12)
+        _lib.check(self._L.nkc_setup_pml(self._h, pf.ctypes.data_as(C.POINTER(C.c_int8)), int(thick), float(order),
+                                         float(referr), _d(ext)))
+        self.pml = dict(inner=ext[:6].copy(), outer=ext[6:].copy(), thick=int(thick), order=float(order),
+                        referr=float(referr))
+        return self.pml
+
+    def set_pml(self, pmlptr, sigma, B=None, D=None):
+        """The reference's PML arrays: pmlptr (0-based element numbers),
+        sigma (3, npts), initial B = pmlbn, D = pmldn (3, npts) or None."""
+        n = self.npts
+        ptr = np.ascontiguousarray(np.asarray(pmlptr) + 1, dtype=np.int32)
+        sig = np.ascontiguousarray(sigma, dtype=np.float64).reshape(3, n)
+        b = None if B is None else np.ascontiguousarray(B, dtype=np.float64).reshape(3, n)
+        d = None if D is None else np.ascontiguousarray(D, dtype=np.float64).reshape(3, n)
+        _lib.check(self._L.nkc_set_pml(self._h, len(ptr), _i(ptr), _d(sig), _d(b), _d(d), n))
+
+    def set_pml_fields(self, B, D):
+        """pmlbn, pmldn (3, npts) as userini sets them; their RK register := 0."""
+        n = self.npts
+        b = np.ascontiguousarray(B, dtype=np.float64).reshape(3, n)
+        d = np.ascontiguousarray(D, dtype=np.float64).reshape(3, n)
+        _lib.check(self._L.nkc_set_pml_fields(self._h, _d(b), _d(d), n))
+
+    _PML_WHICH = dict(state=0, register=1, residual=2, sigma=3)
+
+    def get_pml(self, which="state"):
+        """(6, npts) B then D (state, register or last residual), or (3, npts)
+        sigma; zero outside the PML elements."""
+        w = self._PML_WHICH[which]
+        out = np.zeros((3 if w == 3 else 6, self.npts))
+        _lib.check(self._L.nkc_get_pml(self._h, w, _d(out), self.npts))
+        return out
+
+    def set_incident(self, faces, k, omega, hamp, eamp):
+        """Incident plane wave u = cos(k . x - omega t) on the listed faces
+        ((element 0-based, face 1..6 in the reference's order) pairs): the
+        traces get H += hamp u, E += eamp u before the flux.  faces = [] removes it."""
+        self._ensure_coords()
+        f = np.ascontiguousarray(np.asarray(faces, dtype=np.int64).reshape(-1, 2) + [1, 0], dtype=np.int32)
+        kk, hh, ee = (np.ascontiguousarray(v, dtype=np.float64) for v in (k, hamp, eamp))
+        _lib.check(self._L.nkc_set_incident(self._h, len(f), _i(f) if len(f) else None, _d(kk), float(omega),
+                                            _d(hh), _d(ee)))
 
     def set_drude(self, dindex, a, b, J=None, kJ=None):
         """Drude ADE on volume nodes dindex (0-based): a = gamma, b =

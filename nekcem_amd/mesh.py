@@ -120,7 +120,8 @@ def box_dxmin(nel, N, lo=(0.0, 0.0, 0.0), hi=(2 * np.pi,) * 3):
 class BoxMesh:
     """Reference-layout arrays for a box mesh.
 
-    bc: per direction 'P' (periodic) or 'PEC'.
+    bc: per direction 'P' (periodic), 'PEC', or 'PML' (a PEC wall behind an
+    absorbing layer).
     """
 
     def __init__(self, nel, N, lo=(0.0, 0.0, 0.0), hi=(2 * np.pi,) * 3, bc=("P", "P", "P")):
@@ -255,7 +256,7 @@ class BoxMesh:
                 valid = np.ones(nelt, dtype=bool)
             else:
                 valid = (nb_idx[ax] >= 0) & (nb_idx[ax] < nea)
-                pec[~valid, f] = self.bc[ax] == "PEC"
+                pec[~valid, f] = self.bc[ax] in ("PEC", "PML")   # cem_maxwell_pec_init lists both
             ne = nb_idx[0] + self.nel[0] * (nb_idx[1] + self.nel[1] * nb_idx[2])
             opp = ED_FACES.index((ax, 1 - side))
             slot = (ne[:, None] * 6 + opp) * n2 + q[None, :] + 1

@@ -12,6 +12,14 @@ Formats:
   5-8); "CURVED SIDE DATA" (must be empty here); "FLUID BOUNDARY
   CONDITIONS" with one line per element face: code (columns 2-4), element,
   face, 5 parameters.
+* .box (genbox2 / getbox / makebox, src/nek5_genbox.F:2-555; used when
+  the .rea's NDIM is negative): lines starting with '#' are comments; a
+  dummy line, NDIM, the number of fields, then a box: a line starting
+  'Box', "nelx nely nelz" (negative: x0 x1 ratio on the next line, a
+  geometric spacing; positive: the nel+1 vertex coordinates), the same for
+  y and z, and a line of six comma-separated 3-character codes for the x-,
+  x+, y-, y+, z-, z+ walls.  Elements are numbered x fastest; interior
+  faces are 'E' with their neighbour, periodic walls 'P' with theirs.
 * .re2 (v002): an 80-byte header "#v002 nelgt ndim nelgv", a float32
   6.54321 byte-order mark, per element 25 float64 (group, x[8], y[8],
   z[8]), then ncurve (float64) and its 64-byte records, then nbc (float64)
@@ -30,7 +38,8 @@ def _floats(line):
 
 
 def read_rea(path):
-    """dict(param (1-based list), ndim, nel, xc, yc, zc, cbc, bc, in_re2)."""
+    """dict(param (1-based list), ndim, nel, xc, yc, zc, cbc, bc, in_re2,
+    in_box)."""
     with open(path) as f:
         lines = f.read().splitlines()
     out = {}
@@ -43,6 +52,10 @@ def read_rea(path):
     while "MESH DATA" not in lines[i]:
         i += 1
     nel, ndim, _ = (int(v) for v in lines[i + 1].split()[:3])
+    out["in_box"] = ndim < 0          # ifgtp: the mesh is the .box beside it
+    if out["in_box"]:
+        out["ndim"], out["nel"], out["in_re2"] = None, None, False
+        return out
     if ndim != 3:
         raise ValueError(f"{path}: {ndim}-D mesh; the HIP path is 3-D")
     out["ndim"], out["nel"], out["in_re2"] = ndim, abs(nel), nel < 0
@@ -110,10 +123,84 @@ def read_re2(path):
     return dict(nel=nel, xc=xc, yc=yc, zc=zc, cbc=cbc, bc=bc)
 
 
+def _box_vertices(tokens, n):
+    """One axis of getbox: n < 0 -> x0 x1 ratio (geometric spacing scaled
+    onto [x0, x1]); n > 0 -> the n + 1 vertices."""
+    if n < 0:
+        n = -n
+        x0, x1, ratio = (float(tokens.pop(0)) for _ in range(3))
+        if ratio <= 0:
+            ratio = 1.0
+        x = np.zeros(n + 1)
+        dx = (x1 - x0) / n
+        for k in range(1, n + 1):
+            x[k] = x[k - 1] + dx
+            dx = dx * ratio
+        return x0 + (x1 - x0) / x[n] * x
+    return np.array([float(tokens.pop(0)) for _ in range(n + 1)])
+
+
+def read_box(path):
+    """dict(ndim, nel, nelxyz, xc, yc, zc, cbc, bc) of a one-box .box file,
+    elements and faces as makebox lays them out."""
+    with open(path) as f:
+        lines = [ln for ln in f.read().splitlines() if ln.strip() and not ln.lstrip().startswith("#")]
+    ndim = abs(int(lines[1].split()[0]))
+    if ndim != 3:
+        raise ValueError(f"{path}: {ndim}-D box; the HIP path is 3-D")
+    k = 3
+    if not lines[k].lstrip()[:1] in ("B", "b"):
+        raise ValueError(f"{path}: only a plain 'Box' is read (got {lines[k]!r})")
+    k += 1
+    nels = [int(v) for v in lines[k].split()[:3]]
+    k += 1
+    tokens = []
+    verts = []
+    for n in nels:
+        need = 3 if n < 0 else abs(n) + 1
+        while len(tokens) < need:
+            tokens += [t.replace("D", "E").replace("d", "e") for t in lines[k].replace(",", " ").split()
+                       if not t.startswith("(")][:need - len(tokens)]
+            k += 1
+        verts.append(_box_vertices(tokens, n))
+        tokens = []
+    codes = [c.strip()[:3] for c in lines[k].split(",")[:6]]
+    codes = [(c + "  ")[:3] for c in codes]
+    nx, ny, nz = (abs(n) for n in nels)
+    nelt = nx * ny * nz
+    e = np.arange(nelt)
+    ex, ey, ez = e % nx, (e // nx) % ny, e // (nx * ny)
+    X, Y, Z = verts
+    xc = np.stack([X[ex], X[ex + 1], X[ex + 1], X[ex]] * 2, axis=1)
+    yc = np.stack([Y[ey], Y[ey], Y[ey + 1], Y[ey + 1]] * 2, axis=1)
+    zc = np.stack([Z[ez]] * 4 + [Z[ez + 1]] * 4, axis=1)
+    cbc = np.full((nelt, 6), "E  ", dtype="<U3")
+    bc = np.zeros((nelt, 6, 5))
+    gid = lambda a, b, c: 1 + a + nx * (b + ny * c)
+    # (wall code index, reference face, axis, side, neighbour's face)
+    for wall, f, ax, side, nf in ((0, 4, 0, 0, 2), (1, 2, 0, 1, 4), (2, 1, 1, 0, 3), (3, 3, 1, 1, 1),
+                                  (4, 5, 2, 0, 6), (5, 6, 2, 1, 5)):
+        idx = [ex, ey, ez]
+        nn = (nx, ny, nz)[ax]
+        nb = [ex.copy(), ey.copy(), ez.copy()]
+        nb[ax] = (idx[ax] + (1 if side else -1)) % nn
+        on_wall = idx[ax] == (nn - 1 if side else 0)
+        cbc[:, f - 1] = np.where(on_wall, codes[wall], "E  ")
+        link = (~on_wall) | (codes[wall].strip() == "P")
+        bc[:, f - 1, 0] = np.where(link, gid(*nb), 0.0)
+        bc[:, f - 1, 1] = np.where(link, nf, 0.0)
+    return dict(ndim=3, nel=nelt, nelxyz=(nx, ny, nz), xc=xc, yc=yc, zc=zc, cbc=cbc, bc=bc)
+
+
 def read_case(rea_path, re2_path=None):
-    """The mesh of a case: from the .rea, or from the .re2 beside it when the
-    .rea says so (NEL < 0); plus the .rea parameters."""
+    """The mesh of a case: from the .rea, from the .re2 beside it when the
+    .rea says so (NEL < 0), or from the .box beside it (NDIM < 0); plus the
+    .rea parameters."""
     r = read_rea(rea_path)
+    if r["in_box"]:
+        m = read_box(rea_path[:-4] + ".box")
+        r.update(m)
+        return r
     if r["in_re2"]:
         re2 = re2_path or rea_path[:-4] + ".re2"
         m = read_re2(re2)

@@ -103,3 +103,117 @@ def gaussian_source(x, y, z, bm1, center=(0.4, 0.55, 0.45), width=0.1):
     """bm1-weighted spatial amplitude of a Gaussian pulse source on Ez."""
     r2 = (x - center[0]) ** 2 + (y - center[1]) ** 2 + (z - center[2]) ** 2
     return np.exp(-r2 / (2.0 * width * width)) * bm1
+
+
+# ---------------------------------------------------------------------------
+# 3ddielectric (tests/3ddielectric/3ddielectric.usr): a plane wave of
+# frequency omega = 2 comes from y > 0 onto the interface y = 0 between
+# (eps1, mu1) = (1, 1) above and (eps2, mu2) = (1 or 2, 1) below, with PML
+# walls in y.  uservp (lines 171-296) sets the materials from each element's
+# probe node ym1(nx1/2, nx1/2, nx1/2, e) and lists the upper elements' faces
+# that lie on y = 0 for userinc (6-47); usrdat2 (300-330) maps the box onto
+# [-2.5, 2.5] x [-5, 5] x [-2.5, 2.5]; usersol (83-147) is the exact
+# reflected / transmitted wave, damped inside the PML; userchk (332-392)
+# holds the tolerances.
+# ---------------------------------------------------------------------------
+
+DIELECTRIC_L2TOL = (5e-4, 5e-14, 5e-4, 5e-4, 5e-14, 5e-4)
+DIELECTRIC_LINFTOL = {False: (5e-3, 5e-12, 5e-3, 5e-3, 5e-12, 5e-3),
+                      True: (5e-3, 1e-12, 5e-3, 5e-3, 1e-12, 5e-3)}
+
+
+def dielectric_params(twomat):
+    """The /userparam/ block of uservp: omega, eps1, eps2, mu1, mu2 and the
+    TE / TM reflection and transmission coefficients."""
+    omega, eps1, eps2, mu1, mu2 = 2.0, 1.0, (2.0 if twomat else 1.0), 1.0, 1.0
+    z1, z2 = np.sqrt(mu1 / eps1), np.sqrt(mu2 / eps2)
+    return dict(omega=omega, eps1=eps1, eps2=eps2, mu1=mu1, mu2=mu2, reflte=(z1 - z2) / (z1 + z2),
+                trante=2 * z1 / (z1 + z2), refltm=(z2 - z1) / (z1 + z2), trantm=2 * z2 / (z1 + z2))
+
+
+def rescale_usrdat2(sizes):
+    """usrdat2 of the box cases: each axis mapped onto [-s/2, s/2]."""
+    def f(xc, yc, zc):
+        out = []
+        for c, sz in zip((xc, yc, zc), sizes):
+            a, b = c.min(), c.max()
+            out.append(sz * (c - a) / (b - a) - sz / 2.0)
+        return out
+    return f
+
+
+def probe_upper(lo_y, hi_y, n1):
+    """uservp's region test per element: ym1 at the probe node (1-based
+    nx1/2 along each direction) > 0."""
+    from .mesh import gll
+    z, _ = gll(n1)
+    zp = z[n1 // 2 - 1]
+    return lo_y + (hi_y - lo_y) * 0.5 * (1.0 + zp) > 0.0
+
+
+def dielectric_materials(upper, n1, twomat):
+    """Per-node permittivity and permeability (npts each) of uservp."""
+    p = dielectric_params(twomat)
+    per = np.where(upper, p["eps1"], p["eps2"])
+    eps = np.repeat(per, n1 ** 3)
+    mu = np.repeat(np.where(upper, p["mu1"], p["mu2"]), n1 ** 3)
+    return eps, mu
+
+
+def dielectric_incident(upper, lo_y, hi_y, twomat):
+    """uservp's incident faces and userinc's wave: (faces, k, omega, hamp,
+    eamp) for MaxwellSolver.set_incident.  Faces are the upper elements'
+    faces whose every point has |y| <= 1e-8: on an axis-aligned box only a
+    y face can, face 1 (s = -1, y = lo) or 3 (s = +1, y = hi)."""
+    p = dielectric_params(twomat)
+    faces = []
+    for e in np.nonzero(upper)[0]:
+        if abs(lo_y[e]) <= 1e-8:
+            faces.append((e, 1))
+        if abs(hi_y[e]) <= 1e-8:
+            faces.append((e, 3))
+    eta = np.sqrt(p["mu1"] / p["eps1"])
+    ky = p["omega"] * np.sqrt(p["mu1"] * p["eps1"])
+    # uinc = cos(-ky y - omega t); hz += u, ex += eta u, ez += u, hx -= u / eta
+    return faces, (0.0, -ky, 0.0), p["omega"], (-1.0 / eta, 0.0, 1.0), (eta, 0.0, 1.0)
+
+
+def dielectric_solution(t, y, eps, mu, upper_nodes, pml_nodes, inner, outer, order, referr, twomat):
+    """usersol: (H, E), (3, npts) each; y, eps, mu per node; upper_nodes /
+    pml_nodes: the node's element is in the upper region / a PML element;
+    inner, outer: pmlinner / pmlouter (x-, x+, y-, y+, z-, z+)."""
+    p = dielectric_params(twomat)
+    om = p["omega"]
+    eta = np.sqrt(mu / eps)
+    ky = om * np.sqrt(eps * mu)
+    H = np.zeros((3, len(y)))
+    E = np.zeros((3, len(y)))
+    # upper region: the reflected wave, damped in the +y PML
+    d = outer[3] - inner[3]
+    smax = -(order + 1) * np.log(referr) / (2 * eta * d)
+    fac = np.where(pml_nodes, smax * d / (order + 1) * np.abs((y - inner[3]) / d) ** (order + 1), 0.0)
+    uu = np.exp(-eta * fac) * np.cos(ky * y - om * t)
+    up = upper_nodes
+    H[2] = np.where(up, p["reflte"] * uu, H[2])
+    E[0] = np.where(up, -eta * p["reflte"] * uu, E[0])
+    E[2] = np.where(up, p["refltm"] * uu, E[2])
+    H[0] = np.where(up, p["refltm"] * uu / eta, H[0])
+    # lower region: the transmitted wave, damped in the -y PML
+    d = inner[2] - outer[2]
+    smax = -(order + 1) * np.log(referr) / (2 * eta * d)
+    fac = np.where(pml_nodes, smax * d / (order + 1) * np.abs((inner[2] - y) / d) ** (order + 1), 0.0)
+    uu = np.exp(-eta * fac) * np.cos(-ky * y - om * t)
+    lo = ~upper_nodes
+    H[2] = np.where(lo, p["trante"] * uu, H[2])
+    E[0] = np.where(lo, eta * p["trante"] * uu, E[0])
+    E[2] = np.where(lo, p["trantm"] * uu, E[2])
+    H[0] = np.where(lo, -p["trantm"] * uu / eta, H[0])
+    return H, E
+
+
+def dipole_amplitude(x, y, z, bm1, width=0.1):
+    """usersrc of 3dboxpml (tests/3dboxpml/3dboxpml.usr:30-88): the Gaussian
+    Hertzian dipole at the origin on Ez, srcez -= i0 norm exp(-r^2 / (2 w^2))
+    sin(-omega t) bm1 with i0 norm = 1: the bm1-weighted amplitude for
+    set_source(..., direction=(0, 0, 1), omega=-2.0)."""
+    return -np.exp(-0.5 * (x / width) ** 2 - 0.5 * (y / width) ** 2 - 0.5 * (z / width) ** 2) * bm1

@@ -470,6 +470,22 @@ typedef struct orc_mesh {
     int64_t nlorentz;
     const int32_t *lindex;
     const double *lor_par;
+    /* usersrc time factor: kind 0 the Gaussian pulse above, kind 1
+     * sin(src_omega t + src_phase) (the dipole of tests/3dboxpml/3dboxpml.usr:30-88) */
+    int32_t src_kind;
+    double src_omega, src_phase;
+    /* PML (src/cem_maxwell_pml.F): elements pmlptr[0..npml) (0-based),
+     * pmlsigma(npts,3), permittivity and permeability per node */
+    int64_t npml;
+    const int32_t *pmlptr;
+    const double *pmlsigma;
+    const double *eps, *mu;
+    /* userinc-style incident plane wave on the face points with inc_on set:
+     * fH += inc_h u, fE += inc_e u, u = cos(inc_k . x - inc_omega t) at the
+     * face point's volume node (tests/3ddielectric/3ddielectric.usr:6-47) */
+    const int8_t *inc_on;                 /* nxzfl or NULL */
+    const double *xm1, *ym1, *zm1;        /* npts */
+    double inc_k[3], inc_omega, inc_h[3], inc_e[3];
 } orc_mesh;
 
 /* resE = W curl H; resH = -W curl E (cem_maxwell.F:574-582 + 1428-1497) */
@@ -509,7 +525,7 @@ static void wght_curl(const orc_mesh *m, const double *u1, const double *u2, con
 
 static void maxwell_op_body(const orc_mesh *m, double rktime, const double *hn, const double *en,
                             double *reshn, double *resen, const double *J, double *resJ,
-                            const double *L, double *resL);
+                            const double *L, double *resL, const double *P, double *resP);
 
 /* One residual evaluation (cem_maxwell_op, cem_maxwell.F:484-508).
  * hn, en, reshn, resen: 3 components each at stride npts.
@@ -517,7 +533,7 @@ static void maxwell_op_body(const orc_mesh *m, double rktime, const double *hn, 
 void orc_maxwell_op(const orc_mesh *m, double rktime, const double *hn, const double *en,
                     double *reshn, double *resen, const double *J, double *resJ)
 {
-    maxwell_op_body(m, rktime, hn, en, reshn, resen, J, resJ, NULL, NULL);
+    maxwell_op_body(m, rktime, hn, en, reshn, resen, J, resJ, NULL, NULL, NULL, NULL);
 }
 
 /* The same with the Lorentz state L = (J, P) (6 blocks of nlorentz: J then
@@ -526,12 +542,21 @@ void orc_maxwell_op_ade(const orc_mesh *m, double rktime, const double *hn, cons
                         double *reshn, double *resen, const double *J, double *resJ,
                         const double *L, double *resL)
 {
-    maxwell_op_body(m, rktime, hn, en, reshn, resen, J, resJ, L, resL);
+    maxwell_op_body(m, rktime, hn, en, reshn, resen, J, resJ, L, resL, NULL, NULL);
+}
+
+/* The same with the PML state P = (pmlbn, pmldn): 6 blocks of npts, and
+ * its residual resP = (respmlbn, respmldn), or NULL. */
+void orc_maxwell_op_full(const orc_mesh *m, double rktime, const double *hn, const double *en,
+                         double *reshn, double *resen, const double *J, double *resJ,
+                         const double *L, double *resL, const double *P, double *resP)
+{
+    maxwell_op_body(m, rktime, hn, en, reshn, resen, J, resJ, L, resL, P, resP);
 }
 
 static void maxwell_op_body(const orc_mesh *m, double rktime, const double *hn, const double *en,
                             double *reshn, double *resen, const double *J, double *resJ,
-                            const double *L, double *resL)
+                            const double *L, double *resL, const double *P, double *resP)
 {
     int64_t npts = m->npts, k = m->nxzfl;
     const double *H1 = hn, *H2 = hn + npts, *H3 = hn + 2 * npts;
@@ -552,6 +577,18 @@ static void maxwell_op_body(const orc_mesh *m, double rktime, const double *hn, 
         int64_t v = m->cemface[i];
         double fh1 = H1[v], fh2 = H2[v], fh3 = H3[v];
         double fe1 = E1[v], fe2 = E2[v], fe3 = E3[v];
+        if (m->inc_on && m->inc_on[i]) {
+            /* userinc (cem_maxwell.F:497-498): added to fhn/fen after the
+             * restriction, before the flux */
+            double u = cos(m->inc_k[0] * m->xm1[v] + m->inc_k[1] * m->ym1[v] + m->inc_k[2] * m->zm1[v] -
+                           m->inc_omega * rktime);
+            fh1 = fh1 + m->inc_h[0] * u;
+            fh2 = fh2 + m->inc_h[1] * u;
+            fh3 = fh3 + m->inc_h[2] * u;
+            fe1 = fe1 + m->inc_e[0] * u;
+            fe2 = fe2 + m->inc_e[1] * u;
+            fe3 = fe3 + m->inc_e[2] * u;
+        }
         double nx = m->unx[i], ny = m->uny[i], nz = m->unz[i];
         s[0 * k + i] = -ny * fe3 + nz * fe2;
         s[1 * k + i] = -nz * fe1 + nx * fe3;
@@ -610,11 +647,52 @@ static void maxwell_op_body(const orc_mesh *m, double rktime, const double *hn, 
             rE2[i] = rE2[i] + a * s[4 * k + j];
             rE3[i] = rE3[i] + a * s[5 * k + j];
         }
-    /* usersrc: Gaussian-pulse source on E (bm1-weighted amplitude), and the
-     * Drude current (cem_maxwell_drude, cem_maxwell.F:3095-3147):
+    /* pml_step (src/cem_maxwell_pml.F:508-592), after the flux and before
+     * usersrc (cem_maxwell.F:502) */
+    if (m->npml > 0 && P && resP) {
+        int64_t n3 = (int64_t)m->n1 * m->n1 * m->n1;
+        const double *B1 = P, *B2 = P + npts, *B3 = P + 2 * npts;
+        const double *D1 = P + 3 * npts, *D2 = P + 4 * npts, *D3 = P + 5 * npts;
+        double *rB1 = resP, *rB2 = resP + npts, *rB3 = resP + 2 * npts;
+        double *rD1 = resP + 3 * npts, *rD2 = resP + 4 * npts, *rD3 = resP + 5 * npts;
+#pragma omp parallel for schedule(static)
+        for (int64_t l = 0; l < m->npml; ++l) {
+            int64_t e = m->pmlptr[l];
+            for (int64_t i = 0; i < n3; ++i) {
+                int64_t j = e * n3 + i;
+                double bm = m->bm1[j], bminv = 1.0 / bm;
+                double sigx = m->pmlsigma[j], sigy = m->pmlsigma[npts + j], sigz = m->pmlsigma[2 * npts + j];
+                double sxp = sigx / m->eps[j], syp = sigy / m->eps[j], szp = sigz / m->eps[j];
+                double mu = m->mu[j];
+                rB1[j] = rH1[j] * bminv - syp * B1[j];
+                rB2[j] = rH2[j] * bminv - szp * B2[j];
+                rB3[j] = rH3[j] * bminv - sxp * B3[j];
+                rD1[j] = rE1[j] * bminv - syp * D1[j];
+                rD2[j] = rE2[j] * bminv - szp * D2[j];
+                rD3[j] = rE3[j] * bminv - sxp * D3[j];
+                double ph1 = -syp * B1[j] + sxp * B1[j] - szp * mu * H1[j];
+                double ph2 = -szp * B2[j] + syp * B2[j] - sxp * mu * H2[j];
+                double ph3 = -sxp * B3[j] + szp * B3[j] - syp * mu * H3[j];
+                double pe1 = -syp * D1[j] + sxp * D1[j] - sigz * E1[j];
+                double pe2 = -szp * D2[j] + syp * D2[j] - sigx * E2[j];
+                double pe3 = -sxp * D3[j] + szp * D3[j] - sigy * E3[j];
+                rH1[j] = rH1[j] + ph1 * bm;
+                rH2[j] = rH2[j] + ph2 * bm;
+                rH3[j] = rH3[j] + ph3 * bm;
+                rE1[j] = rE1[j] + pe1 * bm;
+                rE2[j] = rE2[j] + pe2 * bm;
+                rE3[j] = rE3[j] + pe3 * bm;
+            }
+        }
+    }
+    /* usersrc: source on E (bm1-weighted amplitude) with a Gaussian-pulse
+     * or sinusoidal time factor, and the Drude current (cem_maxwell_drude,
+     * cem_maxwell.F:3095-3147):
      *   resE -= J * bm1 ;  resJ = -a J + b E */
     if (m->src_amp) {
-        double g = exp(-((rktime - m->src_t0) / m->src_tw) * ((rktime - m->src_t0) / m->src_tw));
+        double g = m->src_kind == 1
+                       ? sin(m->src_omega * rktime + m->src_phase)
+                       : exp(-((rktime - m->src_t0) / m->src_tw) * ((rktime - m->src_t0) / m->src_tw));
 #pragma omp parallel for schedule(static)
         for (int64_t i = 0; i < npts; ++i) {
             double a = m->src_amp[i] * g;
@@ -706,23 +784,34 @@ static void rk4_upd(double *h, double *kh, const double *resh, double cb, double
 }
 
 /* nsteps of cem_maxwell_op_rk with the Lorentz state L/kL/resL (6*nlorentz)
- * as well (NULL: none); otherwise as orc_rk_run below. */
-void orc_rk_run_ade(const orc_mesh *m, double *hn, double *en, double *khn, double *ken,
-                    double *reshn, double *resen, double *J, double *kJ, double *resJ,
-                    double *L, double *kL, double *resL, double dt, int64_t nsteps, double *time)
+ * and the PML state P/kP/resP (6*npts; rk_maxwell_ab's PML updates,
+ * cem_maxwell.F:1942-1954) as well (NULL: none); otherwise as orc_rk_run. */
+void orc_rk_run_full(const orc_mesh *m, double *hn, double *en, double *khn, double *ken,
+                     double *reshn, double *resen, double *J, double *kJ, double *resJ,
+                     double *L, double *kL, double *resL, double *P, double *kP, double *resP,
+                     double dt, int64_t nsteps, double *time)
 {
     int64_t n3 = 3 * m->npts;
     for (int64_t st = 0; st < nsteps; ++st) {
         for (int r = 0; r < 5; ++r) {
             double rktime = *time + dt * RK4C[r];
-            maxwell_op_body(m, rktime, hn, en, reshn, resen, J, resJ, L, resL);
+            maxwell_op_body(m, rktime, hn, en, reshn, resen, J, resJ, L, resL, P, resP);
             rk4_upd(hn, khn, reshn, RK4B[r], RK4A[r], dt, n3);
             rk4_upd(en, ken, resen, RK4B[r], RK4A[r], dt, n3);
             if (m->ndrude > 0 && J) rk4_upd(J, kJ, resJ, RK4B[r], RK4A[r], dt, 3 * m->ndrude);
             if (m->nlorentz > 0 && L) rk4_upd(L, kL, resL, RK4B[r], RK4A[r], dt, 6 * m->nlorentz);
+            if (m->npml > 0 && P) rk4_upd(P, kP, resP, RK4B[r], RK4A[r], dt, 6 * m->npts);
         }
         *time = *time + dt;
     }
+}
+
+void orc_rk_run_ade(const orc_mesh *m, double *hn, double *en, double *khn, double *ken,
+                    double *reshn, double *resen, double *J, double *kJ, double *resJ,
+                    double *L, double *kL, double *resL, double dt, int64_t nsteps, double *time)
+{
+    orc_rk_run_full(m, hn, en, khn, ken, reshn, resen, J, kJ, resJ, L, kL, resL, NULL, NULL, NULL, dt, nsteps,
+                    time);
 }
 
 /* nsteps of cem_maxwell_op_rk.  u arrays hold 3 components at stride npts.

@@ -71,6 +71,8 @@ def lib():
                                  C.c_double, C.c_int64, _dp]
         L.orc_maxwell_op_ade.argtypes = [C.c_void_p, C.c_double] + [_dp] * 8
         L.orc_rk_run_ade.argtypes = [C.c_void_p] + [_dp] * 12 + [C.c_double, C.c_int64, _dp]
+        L.orc_maxwell_op_full.argtypes = [C.c_void_p, C.c_double] + [_dp] * 10
+        L.orc_rk_run_full.argtypes = [C.c_void_p] + [_dp] * 15 + [C.c_double, C.c_int64, _dp]
         L.orc_rk_coeffs.argtypes = [_dp, _dp, _dp]
         L.orc_cem_error.argtypes = [_dp, _dp, _dp, C.c_int64, _dp, _dp]
         L.orc_maxwell_vjp.argtypes = [C.c_void_p, C.c_double, _dp, _dp, _bp, _dp, _dp, _dp, _dp,
@@ -110,6 +112,11 @@ class _Mesh(C.Structure):
         ("src_tw", C.c_double),
         ("srflx", _dp),
         ("nlorentz", C.c_int64), ("lindex", _ip), ("lor_par", _dp),
+        ("src_kind", C.c_int32), ("src_omega", C.c_double), ("src_phase", C.c_double),
+        ("npml", C.c_int64), ("pmlptr", _ip), ("pmlsigma", _dp), ("eps", _dp), ("mu", _dp),
+        ("inc_on", _bp), ("xm1", _dp), ("ym1", _dp), ("zm1", _dp),
+        ("inc_k", C.c_double * 3), ("inc_omega", C.c_double), ("inc_h", C.c_double * 3),
+        ("inc_e", C.c_double * 3),
     ]
 
 
@@ -144,7 +151,7 @@ class BoxSpec:
     N: int              # polynomial order
     lo: tuple = (0.0, 0.0, 0.0)
     hi: tuple = (2 * np.pi, 2 * np.pi, 2 * np.pi)
-    bc: tuple = ("P", "P", "P")   # per direction: 'P' periodic or 'PEC'
+    bc: tuple = ("P", "P", "P")   # per direction: 'P' periodic, 'PEC' or 'PML' walls
     upwind: bool = True
     verts: tuple = None  # optional per-axis vertex coordinates (nel+1 each): a graded box
 
@@ -209,7 +216,7 @@ class BoxCase:
 
         self.pec_face = np.zeros(6 * nelt, dtype=np.int8)
         for f, (ax, side) in enumerate(_ED_FACES):
-            if spec.bc[ax] == "PEC":
+            if spec.bc[ax] in ("PEC", "PML"):   # cem_maxwell_pec_init lists both
                 ea, nea = self.exyz[ax], spec.nel[ax]
                 on = (ea == 0) if side == 0 else (ea == nea - 1)
                 self.pec_face[6 * e[on] + f] = 1
@@ -228,6 +235,12 @@ class BoxCase:
         self.drude_b = None
         self.lindex = None
         self.lor_par = None
+        self.src_kind, self.src_omega, self.src_phase = 0, 0.0, 0.0
+        self.pmlptr = None
+        self.pmlsigma = None
+        self.pmltag = np.zeros(nelt, dtype=np.int64)
+        self.inc_on = None
+        self.inc = None
 
     # -- setup pieces ---------------------------------------------------
     def _face_ids(self):
@@ -284,12 +297,111 @@ class BoxCase:
                             _p(self.Z0), _p(self.Z1), _p(self.Y0), _p(self.Y1),
                             _p(self.ebm1), _p(self.hbm1))
 
-    def set_source(self, amp, direction=(0.0, 0.0, 1.0), t0=0.0, tw=1.0):
-        """usersrc-style source: resE += amp(x) * dir * exp(-((t-t0)/tw)^2),
-        amp already bm1-weighted (as in src/cem_maxwell.F:3130-3132)."""
+    def set_source(self, amp, direction=(0.0, 0.0, 1.0), t0=0.0, tw=1.0, omega=None, phase=0.0):
+        """usersrc-style source: resE += amp(x) * dir * g(t), amp already
+        bm1-weighted (as in src/cem_maxwell.F:3130-3132); g(t) =
+        exp(-((t-t0)/tw)^2), or sin(omega t + phase) when omega is given."""
         self.src_amp = None if amp is None else np.ascontiguousarray(amp, dtype=np.float64)
         self.src_dir = tuple(float(x) for x in direction)
         self.src_t0, self.src_tw = float(t0), float(tw)
+        self.src_kind = 0 if omega is None else 1
+        self.src_omega, self.src_phase = (0.0 if omega is None else float(omega)), float(phase)
+
+    # -- PML set-up (src/cem_maxwell_pml.F), restated on the face arrays ---
+    def _pml_faceary(self, thick):
+        """pml_fill_faceary + thick x march_faces (src/cem_maxwell_pml.F:85-190):
+        1 on the interior points of every PML face, then per element and axis
+        both opposite faces += 1 where their sum is nonzero, and the gs max
+        over face partners."""
+        n1, nelt = self.n1, self.nelt
+        n2 = n1 * n1
+        fa = np.zeros((nelt, 6, n2))
+        a = np.arange(n1)
+        A, B = np.meshgrid(a, a, indexing="xy")
+        interior = ((A >= 1) & (A <= n1 - 2) & (B >= 1) & (B <= n1 - 2)).reshape(-1)
+        e = np.arange(nelt)
+        for f, (ax, side) in enumerate(_ED_FACES):
+            if self.spec.bc[ax] != "PML":
+                continue
+            ea, nea = self.exyz[ax], self.spec.nel[ax]
+            on = (ea == 0) if side == 0 else (ea == nea - 1)
+            fa[np.ix_(e[on], [f], np.nonzero(interior)[0])] = 1.0
+        eface = [4, 2, 1, 3, 5, 6]   # pff/sym face -> Ed face (1-based)
+        flat = fa.reshape(-1)
+        has = self.partner >= 0
+        for _ in range(thick):
+            fa = flat.reshape(nelt, 6, n2)
+            for axis in range(3):
+                pos, neg = eface[1 + 2 * axis] - 1, eface[2 * axis] - 1
+                ssum = fa[:, pos, :] + fa[:, neg, :]
+                nz = ssum != 0
+                fa[:, pos, :][nz] += 1
+                fa[:, neg, :][nz] += 1
+            flat = fa.reshape(-1)
+            other = flat.copy()
+            other[has] = flat[self.partner[has]]
+            flat = np.maximum(flat, other)
+        return flat.reshape(nelt, 6, n2)
+
+    def set_pml(self, thick, order, referr):
+        """PML of `thick` layers on the 'PML' walls: pml_extent_and_tags and
+        pml_calc_sigma (src/cem_maxwell_pml.F:261-505) for axis-aligned
+        elements (dir_local_to_global is the identity).  Sets pmltag,
+        pmlinner/pmlouter (pff order x-, x+, y-, y+, z-, z+), pmlptr and
+        pmlsigma (npts, 3)."""
+        n1, nelt, npts = self.n1, self.nelt, self.npts
+        fa = self._pml_faceary(int(thick))
+        ind = 1 + 1 * n1                       # ind_ptidx = 1 + 1*lx1 + 1 (1-based)
+        eface = [4, 2, 1, 3, 5, 6]
+        inf = 1e20
+        inner = np.array([-inf, inf] * 3)
+        outer = np.array([inf, -inf] * 3)
+        tag = np.zeros(nelt, dtype=np.int64)
+        X = (self.xm1, self.ym1, self.zm1)
+        n3 = n1 ** 3
+        for e in range(nelt):
+            for axis in range(3):
+                face = 2 * axis                 # pff negative face (0-based)
+                here = fa[e, eface[face] - 1, ind]
+                opp = fa[e, eface[face + 1] - 1, ind]
+                if here == 0 or opp == 0:
+                    continue
+                if here == opp:
+                    raise ValueError("no gradient in the PML indicators: PMLs collide")
+                c = X[axis][e * n3:(e + 1) * n3]
+                lo, hi = c.min(), c.max()
+                if -(here - opp) > 0:           # globsign = -1 for the negative face
+                    inner[face + 1] = min(inner[face + 1], lo)
+                    outer[face + 1] = max(outer[face + 1], hi)
+                    tag[e] |= 1 << (face + 1)
+                else:
+                    inner[face] = max(inner[face], hi)
+                    outer[face] = min(outer[face], lo)
+                    tag[e] |= 1 << face
+        sigma = np.zeros((3, npts))
+        ptr = np.nonzero(tag)[0]
+        for e in ptr:
+            sl = slice(e * n3, (e + 1) * n3)
+            for face in range(6):
+                if not (tag[e] >> face) & 1:
+                    continue
+                axis = face // 2
+                width = abs(outer[face] - inner[face])
+                eta = np.sqrt(self.mu[sl] / self.eps[sl])
+                smax = -(order + 1) * np.log(referr) / (2 * eta * width)
+                z01 = (X[axis][sl] - inner[face]) / (outer[face] - inner[face])
+                sigma[axis, sl] = smax * z01 ** order
+        self.pmltag, self.pmlinner, self.pmlouter = tag, inner, outer
+        self.pmlptr = np.ascontiguousarray(ptr, dtype=np.int32)
+        self.pmlsigma = np.ascontiguousarray(sigma.reshape(-1))
+        self.pml_order, self.pml_referr = float(order), float(referr)
+
+    def set_incident(self, on, k, omega, h_amp, e_amp):
+        """userinc-style plane wave on the face points where `on` (nxzfl) is
+        set: fH += h_amp u, fE += e_amp u, u = cos(k . x - omega t)."""
+        self.inc_on = None if on is None else np.ascontiguousarray(on, dtype=np.int8)
+        self.inc = (tuple(float(x) for x in k), float(omega), tuple(float(x) for x in h_amp),
+                    tuple(float(x) for x in e_amp))
 
     def set_drude(self, dindex, a, b):
         """Drude ADE over volume nodes dindex with per-node a (gamma) and
@@ -340,14 +452,28 @@ class BoxCase:
             m.src_amp = _p(self.src_amp)
         m.src_dir = (C.c_double * 3)(*self.src_dir)
         m.src_t0, m.src_tw = self.src_t0, self.src_tw
+        m.src_kind, m.src_omega, m.src_phase = self.src_kind, self.src_omega, self.src_phase
         m.srflx = _p(self.srflx)
+        m.eps, m.mu = _p(self.eps), _p(self.mu)
+        m.xm1, m.ym1, m.zm1 = _p(self.xm1), _p(self.ym1), _p(self.zm1)
+        if self.pmlptr is not None and len(self.pmlptr):
+            m.npml, m.pmlptr, m.pmlsigma = len(self.pmlptr), _p(self.pmlptr), _p(self.pmlsigma)
+        else:
+            m.npml = 0
+        if self.inc_on is not None:
+            k, om, hh, ee = self.inc
+            m.inc_on = _p(self.inc_on)
+            m.inc_k = (C.c_double * 3)(*k)
+            m.inc_omega = om
+            m.inc_h = (C.c_double * 3)(*hh)
+            m.inc_e = (C.c_double * 3)(*ee)
         self._keep = m
         return m
 
-    def residual(self, H, E, t=0.0, J=None, L=None):
-        """(resH, resE[, resJ][, resL]) = cem_maxwell_op(H, E) incl. inverse
-        mass; J the Drude current (3*ndrude), L the Lorentz state (J then P,
-        6*nlorentz)."""
+    def residual(self, H, E, t=0.0, J=None, L=None, P=None):
+        """(resH, resE[, resJ][, resL][, resP]) = cem_maxwell_op(H, E) incl.
+        inverse mass; J the Drude current (3*ndrude), L the Lorentz state (J
+        then P, 6*nlorentz), P the PML state (pmlbn, pmldn: (6, npts))."""
         H = np.ascontiguousarray(H, dtype=np.float64).reshape(3, self.npts)
         E = np.ascontiguousarray(E, dtype=np.float64).reshape(3, self.npts)
         rH, rE = np.zeros_like(H), np.zeros_like(E)
@@ -359,17 +485,25 @@ class BoxCase:
         if L is not None:
             L = np.ascontiguousarray(L, dtype=np.float64)
             rL = np.zeros_like(L)
-        lib().orc_maxwell_op_ade(C.byref(m), t, _p(H), _p(E), _p(rH), _p(rE), _p(J), _p(rJ), _p(L), _p(rL))
+        rP = None
+        if P is not None:
+            P = np.ascontiguousarray(P, dtype=np.float64).reshape(6, self.npts)
+            rP = np.zeros_like(P)
+        lib().orc_maxwell_op_full(C.byref(m), t, _p(H), _p(E), _p(rH), _p(rE), _p(J), _p(rJ), _p(L), _p(rL),
+                                  _p(P), _p(rP))
         out = (rH, rE)
         if J is not None:
             out += (rJ,)
         if L is not None:
             out += (rL,)
+        if P is not None:
+            out += (rP,)
         return out
 
-    def run(self, H, E, dt, nsteps, t=0.0, kH=None, kE=None, J=None, kJ=None, L=None, kL=None):
+    def run(self, H, E, dt, nsteps, t=0.0, kH=None, kE=None, J=None, kJ=None, L=None, kL=None, P=None, kP=None):
         """nsteps of cem_maxwell_op_rk from (H, E) at time t.  Returns a dict
-        with H, E, kH, kE, time (and J, kJ with Drude, L, kL with Lorentz)."""
+        with H, E, kH, kE, time (and J, kJ with Drude, L, kL with Lorentz,
+        P, kP with a PML)."""
         H = np.array(H, dtype=np.float64).reshape(3, self.npts)
         E = np.array(E, dtype=np.float64).reshape(3, self.npts)
         kH = np.zeros_like(H) if kH is None else np.array(kH, dtype=np.float64)
@@ -387,15 +521,23 @@ class BoxCase:
             L = np.zeros(6 * nl) if L is None else np.array(L, dtype=np.float64)
             kL = np.zeros(6 * nl) if kL is None else np.array(kL, dtype=np.float64)
             rL = np.zeros(6 * nl)
+        rP = None
+        if self.pmlptr is not None and len(self.pmlptr):
+            P = np.zeros((6, self.npts)) if P is None else np.array(P, dtype=np.float64).reshape(6, self.npts)
+            kP = np.zeros((6, self.npts)) if kP is None else np.array(kP, dtype=np.float64).reshape(6, self.npts)
+            rP = np.zeros((6, self.npts))
         tt = np.array([t], dtype=np.float64)
         m = self._mesh()
-        lib().orc_rk_run_ade(C.byref(m), _p(H), _p(E), _p(kH), _p(kE), _p(rH), _p(rE),
-                             _p(J), _p(kJ), _p(rJ), _p(L), _p(kL), _p(rL), dt, nsteps, _p(tt))
+        lib().orc_rk_run_full(C.byref(m), _p(H), _p(E), _p(kH), _p(kE), _p(rH), _p(rE),
+                              _p(J), _p(kJ), _p(rJ), _p(L), _p(kL), _p(rL), _p(P), _p(kP), _p(rP),
+                              dt, nsteps, _p(tt))
         out = dict(H=H, E=E, kH=kH, kE=kE, time=float(tt[0]))
         if J is not None:
             out.update(J=J, kJ=kJ)
         if L is not None:
             out.update(L=L, kL=kL)
+        if P is not None:
+            out.update(P=P, kP=kP)
         return out
 
     def vjp(self, H, E, kH, kE, t=0.0, J=None, kJ=None):

@@ -22,16 +22,16 @@ def _norm(u, bm1):
     return float(np.sqrt(np.sum(bm1 * u * u)))
 
 
-def parity(gH, gE, oH, oE, bm1):
+def parity(gH, gE, oH, oE, bm1, floor=1e-6):
     """Largest per-field relative L2 (bm1-weighted) difference.  A field whose
-    own norm is below 1e-6 of the largest field's norm (e.g. a component that
-    is zero in the exact solution and holds only truncation error) is
+    own norm is below `floor` of the largest field's norm (e.g. a component
+    that is zero in the exact solution and holds only truncation error) is
     measured against the largest field's norm instead."""
     g = np.concatenate([gH, gE])
     o = np.concatenate([oH, oE])
     norms = [_norm(o[c], bm1) for c in range(6)]
     big = max(norms)
-    return max(_norm(g[c] - o[c], bm1) / (norms[c] if norms[c] >= 1e-6 * big else big)
+    return max(_norm(g[c] - o[c], bm1) / (norms[c] if norms[c] >= floor * big else big)
                for c in range(6))
 
 

@@ -331,6 +331,114 @@ def test_gaussian_source_uniform(oracle):
     assert np.max(np.abs(out["E"][2] - exact)) < 1e-10
 
 
+# ------------------------------------------- PML and incident field (oracle) --
+def test_pml_layers_from_marching(oracle):
+    """pml_fill_faceary + march_faces + pml_extent_and_tags on boxes: the
+    PML elements are exactly those within `thick` layers of a PML wall, each
+    tagged with the walls it is near (corners with three), and the inner /
+    outer planes are the layers' element boundaries."""
+    c = oracle.BoxCase(oracle.BoxSpec(nel=(6, 6, 6), N=3, lo=(-1,) * 3, hi=(1,) * 3, bc=("PML",) * 3))
+    c.set_pml(1, 3.0, 1e-8)
+    ex = np.stack(c.exyz)
+    want = np.zeros(c.nelt, dtype=np.int64)
+    for ax in range(3):
+        want |= (ex[ax] == 0) << (2 * ax)
+        want |= (ex[ax] == 5) << (2 * ax + 1)
+    assert np.array_equal(c.pmltag, want)
+    assert np.sum(c.pmltag != 0) == 6 ** 3 - 4 ** 3
+    assert np.allclose(c.pmlinner, [-2 / 3, 2 / 3] * 3) and np.allclose(c.pmlouter, [-1, 1] * 3)
+    c = oracle.BoxCase(oracle.BoxSpec(nel=(2, 8, 3), N=4, lo=(0, -5, 0), hi=(1, 5, 1), bc=("P", "PML", "PEC")))
+    c.set_pml(2, 2.0, 1e-6)
+    ey = c.exyz[1]
+    assert np.array_equal(c.pmltag, np.where(ey <= 1, 1 << 2, 0) | np.where(ey >= 6, 1 << 3, 0))
+    assert np.allclose(c.pmlinner[2:4], [-2.5, 2.5]) and np.allclose(c.pmlouter[2:4], [-5, 5])
+    # sigma: zero on the inner plane, sigma_max at the wall, graded (x)^order between
+    n3 = 125
+    for e in np.nonzero(c.pmltag)[0]:
+        y, sg = c.ym1[e * n3:(e + 1) * n3], c.pmlsigma.reshape(3, -1)[1, e * n3:(e + 1) * n3]
+        z01 = np.abs(y) / 2.5 - 1.0
+        smax = -3.0 * np.log(1e-6) / (2 * 2.5)
+        assert np.allclose(sg, smax * z01 ** 2, rtol=1e-13, atol=1e-13)
+
+
+def _dielectric_oracle(oracle, twomat, n1=9):
+    from nekcem_amd import usr
+    c = oracle.BoxCase(oracle.BoxSpec(nel=(4, 8, 4), N=n1 - 1, lo=(-2.5, -5.0, -2.5), hi=(2.5, 5.0, 2.5),
+                                      bc=("P", "PML", "P")))
+    ey = c.exyz[1]
+    lo_y, hi_y = -5.0 + 10.0 * ey / 8, -5.0 + 10.0 * (ey + 1) / 8
+    upper = usr.probe_upper(lo_y, hi_y, n1)
+    eps, mu = usr.dielectric_materials(upper, n1, twomat)
+    c.set_materials(eps, mu)
+    c.set_pml(2, 3.0, 1e-10)
+    faces, k, om, ha, ea = usr.dielectric_incident(upper, lo_y, hi_y, twomat)
+    assert len(faces) == 16    # the 4 x 4 upper elements on y = 0, face 1 (s = -1)
+    on = np.zeros(c.nxzfl, dtype=np.int8)
+    n2 = n1 * n1
+    for e, f in faces:
+        on[(e * 6 + f - 1) * n2:(e * 6 + f) * n2] = 1
+    c.set_incident(on, k, om, ha, ea)
+    upn, pmn = np.repeat(upper, n1 ** 3), np.repeat(c.pmltag != 0, n1 ** 3)
+
+    def sol(t):
+        return usr.dielectric_solution(t, c.ym1, eps, mu, upn, pmn, c.pmlinner, c.pmlouter, 3.0, 1e-10, twomat)
+    return c, sol, eps, mu
+
+
+@pytest.mark.parametrize("twomat,nsteps", [(True, 1000), (False, 300)], ids=["2mat", "1mat"])
+def test_3ddielectric_reference_tolerance(oracle, twomat, nsteps):
+    """The reference's 3ddielectric (tests/3ddielectric, N = 8, dt = 0.005,
+    PML thick 2 / order 3 / referr 1e-10, the incident wave on y = 0) meets
+    its userchk tolerances every 50 steps: the exact reflected / transmitted
+    wave including its decay inside the PML pins pml_step, pml_calc_sigma,
+    the incident-field hook and the two-material flux."""
+    from nekcem_amd import usr
+    c, sol, eps, mu = _dielectric_oracle(oracle, twomat)
+    H, E = sol(0.0)
+    P, kP = np.concatenate([mu * H, eps * E]), None
+    kH = kE = None
+    t = 0.0
+    l2tol, litol = np.array(usr.DIELECTRIC_L2TOL), np.array(usr.DIELECTRIC_LINFTOL[twomat])
+    for _ in range(nsteps // 50):
+        out = c.run(H, E, 0.005, 50, t=t, kH=kH, kE=kE, P=P, kP=kP)
+        H, E, kH, kE, P, kP, t = out["H"], out["E"], out["kH"], out["kE"], out["P"], out["kP"], out["time"]
+        Hs, Es = sol(t)
+        err = [c.cem_error(H[i], Hs[i]) for i in range(3)] + [c.cem_error(E[i], Es[i]) for i in range(3)]
+        l2, li = np.array([a for a, _ in err]), np.array([b for _, b in err])
+        assert np.all(l2 <= l2tol) and np.all(li <= litol), (t, l2, li)
+    assert l2[0] > 1e-6
+
+
+def test_3ddielectric_fails_without_pml_or_incident(oracle):
+    """The same check is sensitive to the pieces it pins: with sigma = 0 (no
+    absorption) or without the incident wave, 100 steps already miss the
+    tolerance."""
+    from nekcem_amd import usr
+    for broken in ("sigma", "incident"):
+        c, sol, eps, mu = _dielectric_oracle(oracle, True)
+        if broken == "sigma":
+            c.pmlsigma = np.zeros_like(c.pmlsigma)
+        else:
+            c.inc_on = None
+        H, E = sol(0.0)
+        out = c.run(H, E, 0.005, 100, P=np.concatenate([mu * H, eps * E]))
+        Hs, _ = sol(out["time"])
+        assert c.cem_error(out["H"][2], Hs[2])[0] > usr.DIELECTRIC_L2TOL[2], broken
+
+
+def test_sinusoidal_source_uniform(oracle):
+    """usersrc with g(t) = sin(omega t + phase) on a uniform amplitude:
+    E_z(t) = a (cos(phase) - cos(omega t + phase)) / omega (periodic box,
+    no curl)."""
+    c = oracle.BoxCase(oracle.BoxSpec(nel=(2, 2, 2), N=3))
+    a, om, ph = 0.7, -2.0, 0.3
+    c.set_source(a * c.bm1, direction=(0, 0, 1), omega=om, phase=ph)
+    out = c.run(np.zeros((3, c.npts)), np.zeros((3, c.npts)), 0.005, 300)
+    t = out["time"]
+    exact = a * (math.cos(ph) - math.cos(om * t + ph)) / om
+    assert np.max(np.abs(out["E"][2] - exact)) < 1e-10
+
+
 # --------------------------------------------- adjoint checker (oracle side) --
 @pytest.mark.parametrize("bc,upwind", [(("P", "P", "P"), True), (("PEC", "P", "PEC"), True),
                                        (("P", "PEC", "P"), False)])

@@ -50,3 +50,40 @@ def test_rea_re2_3dboxper():
     for e, f in zip(*np.nonzero(per)):
         ne, nf = int(r["bc"][e, f, 0]), int(r["bc"][e, f, 1])
         assert r["cbc"][ne - 1, nf - 1] == "P  " and int(r["bc"][ne - 1, nf - 1, 0]) == e + 1
+
+
+def test_box_matches_genbox_output(tmp_path):
+    """read_box follows makebox: the .box of a 4x4x8 periodic box on
+    [-1, 1]^3 gives the corners, codes and periodic links of 3dboxper.re2,
+    which the reference's genbox wrote for that box."""
+    p = tmp_path / "per.box"
+    p.write_text("base.rea\n3\n1\n# comment\nBox\n-4 -4 -8   (nel)\n-1 1 1   (x0,x1,gain)\n"
+                 "-1 1 1   (y)\n-1 1 1   (z)\nP  ,P  ,P  ,P  ,P  ,P    (bc's)\n")
+    b = rea.read_box(str(p))
+    r = rea.read_case(os.path.join(CASES, "3dboxper.rea"))
+    assert b["nel"] == r["nel"] == 128 and b["nelxyz"] == (4, 4, 8)
+    for c in ("xc", "yc", "zc"):
+        assert np.array_equal(b[c], r[c]), c
+    assert np.array_equal(b["cbc"], r["cbc"])
+    per = r["cbc"] == "P  "
+    assert np.array_equal(b["bc"][per][:, :2], r["bc"][per][:, :2])
+
+
+def test_box_dielectric_and_boxpml():
+    """3ddielectric.rea (NDIM < 0) reads its mesh from 3ddielectric.box:
+    4x8x4 on [-1, 1]^3, P in x and z, PML walls in y, PML parameters 77-79;
+    3dboxpml: 6^3 with PML on all six walls."""
+    r = rea.read_case(os.path.join(CASES, "3ddielectric.rea"))
+    assert r["in_box"] and r["nel"] == 128 and r["nelxyz"] == (4, 8, 4)
+    assert r["param"][77:80] == [2.0, 3.0, 1e-10] and r["param"][11] == 1000 and r["param"][12] == -0.005
+    _check_boxes(r)
+    assert (r["yc"].min(), r["yc"].max()) == (-1.0, 1.0)
+    assert np.sum(r["cbc"] == "PML") == 32 and np.all(r["cbc"][:, [0, 2]][r["cbc"][:, [0, 2]] != "E  "] == "PML")
+    assert np.sum(r["cbc"] == "P  ") == 2 * (8 * 4) * 2
+    opp = {1: 3, 3: 1, 2: 4, 4: 2, 5: 6, 6: 5}
+    for e, f in zip(*np.nonzero((r["cbc"] == "E  ") | (r["cbc"] == "P  "))):
+        ne, nf = int(r["bc"][e, f, 0]), int(r["bc"][e, f, 1])
+        assert nf == opp[f + 1] and int(r["bc"][ne - 1, nf - 1, 0]) == e + 1
+    r = rea.read_case(os.path.join(CASES, "3dboxpml.rea"))
+    assert r["nel"] == 216 and np.sum(r["cbc"] == "PML") == 6 * 36
+    assert r["param"][77:80] == [1.0, 3.0, 1e-8] and r["param"][12] == 0.1
