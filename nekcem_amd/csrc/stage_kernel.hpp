@@ -289,10 +289,15 @@ stage_kernel(StageArgs a) {
                 kp[c][kk] = (k < N1) ? ld_stream(a.kbuf + (3 * grp + c) * fs + eo + k * N2 + q) : 0.0;
             }
     }
-    // source region of this element (E roles); its amplitudes are read
-    // level by level in phase 2, so elements without a source hold no
-    // registers for them
+    // source amplitude of this role's levels (E roles), with the HBM
+    // round trip of the other streams
     const int so = (FEAT && grp == 1 && a.src_off) ? a.src_off[e] : -1;
+    double samp[KH];
+#pragma unroll
+    for (int kk = 0; kk < KH; ++kk) {
+        const int k = k0 + kk;
+        samp[kk] = (so >= 0 && act && k < N1) ? a.src_amp[(int64_t)so * N3 + k * N2 + q] : 0.0;
+    }
     // neighbour trace for this role's x/y face (node (a,b) = (i,j))
     double nxy[6];
     int64_t nbn_xy = 0;
@@ -468,7 +473,7 @@ stage_kernel(StageArgs a) {
         }
         if (so >= 0) {
             // usersrc: resE += amp * bm1 * dir * g(t) before the inverse mass
-            const double am = a.src_amp[(int64_t)so * N3 + vloc] * sg;
+            const double am = samp[kk] * sg;
 #pragma unroll
             for (int c = 0; c < 3; ++c) r[c] += am * a.src_dir[c];
         }

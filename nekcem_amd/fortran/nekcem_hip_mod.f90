@@ -33,7 +33,10 @@ module nekcem_hip
   end type nkc_desc
 
   public :: nkc_create, nkc_destroy, nkc_set_state, nkc_get_state, nkc_step, &
-       nkc_residual, nkc_set_source, nkc_set_drude, nkc_get_drude, &
+       nkc_residual, nkc_set_source, nkc_set_source_time, nkc_set_drude, nkc_get_drude, &
+       nkc_set_lorentz, nkc_get_lorentz, nkc_set_materials, &
+       nkc_set_pml, nkc_set_pml_fields, nkc_get_pml, nkc_set_incident, &
+       nkc_adjoint_forward, nkc_adjoint_backward, nkc_adjoint_get_drude, &
        nkc_check_finite, nkc_sync, nkc_stream, nkc_info, &
        nkc_set_graph, nkc_last_error, nkc_check, nkc_error_message
 
@@ -109,6 +112,71 @@ module nekcem_hip
        import :: c_int32_t, c_ptr
        type(c_ptr), value :: h, J, kJ, resJ
      end function nkc_get_drude
+
+     ! source time factor: kind 0 Gaussian pulse, 1 sin(omega t + phase)
+     integer(c_int32_t) function nkc_set_source_time(h, kind, omega, phase) &
+          bind(C, name='nkc_set_source_time')
+       import :: c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: kind
+       real(c_double), value :: omega, phase
+     end function nkc_set_source_time
+
+     ! Lorentz ADE on lindex (1-based); params (a, b, c) per node; L, kL:
+     ! c_loc of [6][nlor] arrays or c_null_ptr
+     integer(c_int32_t) function nkc_set_lorentz(h, nlor, lindex, params, L, kL) &
+          bind(C, name='nkc_set_lorentz')
+       import :: c_int32_t, c_int64_t, c_ptr, c_double
+       type(c_ptr), value :: h
+       integer(c_int64_t), value :: nlor
+       integer(c_int32_t), intent(in) :: lindex(*)
+       real(c_double), intent(in) :: params(*)
+       type(c_ptr), value :: L, kL
+     end function nkc_set_lorentz
+
+     integer(c_int32_t) function nkc_get_lorentz(h, L, kL, resL) bind(C, name='nkc_get_lorentz')
+       import :: c_int32_t, c_ptr
+       type(c_ptr), value :: h, L, kL, resL
+     end function nkc_get_lorentz
+
+     ! PML from the reference's arrays: pmlptr(maxpml), pmlsigma(lpts1,3),
+     ! pmlbn / pmldn (lpts1,3) as c_loc or c_null_ptr, ld = lpts1
+     integer(c_int32_t) function nkc_set_pml(h, maxpml, pmlptr, sigma, bn, dn, ld) &
+          bind(C, name='nkc_set_pml')
+       import :: c_int32_t, c_int64_t, c_ptr, c_double
+       type(c_ptr), value :: h
+       integer(c_int64_t), value :: maxpml, ld
+       integer(c_int32_t), intent(in) :: pmlptr(*)
+       real(c_double), intent(in) :: sigma(*)
+       type(c_ptr), value :: bn, dn
+     end function nkc_set_pml
+
+     integer(c_int32_t) function nkc_set_pml_fields(h, bn, dn, ld) bind(C, name='nkc_set_pml_fields')
+       import :: c_int32_t, c_int64_t, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(in) :: bn(*), dn(*)
+       integer(c_int64_t), value :: ld
+     end function nkc_set_pml_fields
+
+     ! which: 0 (pmlbn, pmldn), 1 their RK register, 2 residual, 3 sigma
+     integer(c_int32_t) function nkc_get_pml(h, which, out, ld) bind(C, name='nkc_get_pml')
+       import :: c_int32_t, c_int64_t, c_ptr, c_double
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: which
+       real(c_double), intent(out) :: out(*)
+       integer(c_int64_t), value :: ld
+     end function nkc_get_pml
+
+     ! incident plane wave on faces (element, face) pairs, 1-based
+     integer(c_int32_t) function nkc_set_incident(h, nface, faces, k, omega, hamp, eamp) &
+          bind(C, name='nkc_set_incident')
+       import :: c_int32_t, c_int64_t, c_ptr, c_double
+       type(c_ptr), value :: h
+       integer(c_int64_t), value :: nface
+       integer(c_int32_t), intent(in) :: faces(*)
+       real(c_double), intent(in) :: k(3), hamp(3), eamp(3)
+       real(c_double), value :: omega
+     end function nkc_set_incident
 
      ! per-node permittivity and permeability, npts each
      integer(c_int32_t) function nkc_set_materials(h, eps, mu) bind(C, name='nkc_set_materials')
