@@ -131,7 +131,9 @@ constexpr int NLAYOUT = 4;
 
 template <int N1, int MAT, int MODE, int PACK, int OCC>
 void launch_t(const StageArgs &a, int grid, hipStream_t s) {
-    if (a.src_off || a.drude_off || a.lor_off || a.pml_off || a.inc_mask)
+    if (a.pml_off || a.inc_mask)
+        hipLaunchKernelGGL((stage_kernel<N1, MAT, MODE, PACK, OCC, 2>), dim3(grid), dim3(Shape<N1, PACK>::BS), 0, s, a);
+    else if (a.src_off || a.drude_off || a.lor_off)
         hipLaunchKernelGGL((stage_kernel<N1, MAT, MODE, PACK, OCC, 1>), dim3(grid), dim3(Shape<N1, PACK>::BS), 0, s, a);
     else
         hipLaunchKernelGGL((stage_kernel<N1, MAT, MODE, PACK, OCC, 0>), dim3(grid), dim3(Shape<N1, PACK>::BS), 0, s, a);

@@ -237,8 +237,9 @@ struct Shape {
 
 // OCC = 1 asks the register allocator for 4 waves per SIMD (128 VGPRs,
 // spilling if it must); OCC = 0 leaves it free.  The autotuner picks.
-// FEAT = 1 compiles the in-stage source and Drude terms; FEAT = 0 leaves
-// them out, so the plain step carries none of their registers.
+// FEAT = 1 compiles the in-stage source, Drude and Lorentz terms, FEAT = 2
+// the PML and the incident field as well; FEAT = 0 leaves them all out, so
+// the plain step carries none of their registers or instructions.
 template <int N1, int MAT, int MODE, int PACK, int OCC, int FEAT>
 __global__ void __launch_bounds__((Shape<N1, PACK>::BS)) __attribute__((amdgpu_waves_per_eu(OCC ? 4 : 1)))
 stage_kernel(StageArgs a) {
@@ -328,10 +329,9 @@ stage_kernel(StageArgs a) {
         }
     }
     const double *geo = a.geo + (int64_t)e * 9;
-    const int imask = (FEAT && a.inc_mask) ? a.inc_mask[e] : 0;
-    // stage time (usersrc, userinc)
-    const double dt = a.dt_e ? a.dt_e[e] : a.dt;
-    const double tstage = (FEAT && (a.src_off || imask)) ? *a.tdev + a.rkc * dt : 0.0;
+    const int imask = (FEAT == 2 && a.inc_mask) ? a.inc_mask[e] : 0;
+    // stage time for userinc (phase 1); usersrc takes it in phase 2
+    const double tinc = (FEAT == 2 && imask) ? *a.tdev + a.rkc * (a.dt_e ? a.dt_e[e] : a.dt) : 0.0;
     __syncthreads();
 
     // ---- phase 1: role r computes the flux of x/y face r at node (i,j) ----
@@ -347,8 +347,8 @@ stage_kernel(StageArgs a) {
         for (int c = 0; c < 6; ++c) um[c] = s_u[c * N3 + own];
         load_coef<N1, MAT>(a, e, role, q, cf);
         const double sg = (role & 1) ? 1.0 : -1.0;
-        if (FEAT && (imask >> role) & 1) add_incident(a, eo + own, tstage, um);
-        if (FEAT && (imask >> (8 + role)) & 1) add_incident(a, (int64_t)nbr_xy * N3 + nbn_xy, tstage, nxy);
+        if (FEAT == 2 && (imask >> role) & 1) add_incident(a, eo + own, tinc, um);
+        if (FEAT == 2 && (imask >> (8 + role)) & 1) add_incident(a, (int64_t)nbr_xy * N3 + nbn_xy, tinc, nxy);
         if (role < 2) face_flux<0>(sg, nbr_xy, um, nxy, cf, geo[3 + role], FH, FE);
         else          face_flux<1>(sg, nbr_xy, um, nxy, cf, geo[3 + role], FH, FE);
 #pragma unroll
@@ -378,11 +378,13 @@ stage_kernel(StageArgs a) {
     }
     double minv = 0.0;
     if (MAT == MAT_ELEM) minv = grp ? a.ieps[e] : a.imu[e];
+    const double dt = a.dt_e ? a.dt_e[e] : a.dt;
     double cfz[4];
     load_coef<N1, MAT>(a, e, zface, q, cfz);
     // usersrc time factor: one value per stage (and element, with per-element dt)
     double sg = 0.0;
     if (so >= 0) {
+        const double tstage = *a.tdev + a.rkc * dt;
         if (a.src_kind == 1) {
             sg = sin(a.src_om * tstage + a.src_ph);
         } else {
@@ -392,7 +394,7 @@ stage_kernel(StageArgs a) {
     }
     const int dd = (FEAT && grp == 1 && a.drude_off) ? a.drude_off[e] : -1;
     const int lo = (FEAT && grp == 1 && a.lor_off) ? a.lor_off[e] : -1;
-    const int po = (FEAT && a.pml_off) ? a.pml_off[e] : -1;
+    const int po = (FEAT == 2 && a.pml_off) ? a.pml_off[e] : -1;
 #pragma unroll
     for (int kk = 0; kk < KH; ++kk) {
         const int k = k0 + kk;
@@ -421,8 +423,8 @@ stage_kernel(StageArgs a) {
             double um[6], FH[3], FE[3];
 #pragma unroll
             for (int c = 0; c < 6; ++c) um[c] = s_u[c * N3 + own];
-            if (FEAT && (imask >> zface) & 1) add_incident(a, eo + own, tstage, um);
-            if (FEAT && (imask >> (8 + zface)) & 1) add_incident(a, (int64_t)nbr_z * N3 + nbn_z, tstage, nz);
+            if (FEAT == 2 && (imask >> zface) & 1) add_incident(a, eo + own, tinc, um);
+            if (FEAT == 2 && (imask >> (8 + zface)) & 1) add_incident(a, (int64_t)nbr_z * N3 + nbn_z, tinc, nz);
             face_flux<2>(half ? 1.0 : -1.0, nbr_z, um, nz, cfz, geo[3 + zface], FH, FE);
 #pragma unroll
             for (int c = 0; c < 3; ++c) r[c] += grp ? FE[c] : FH[c];
@@ -445,7 +447,7 @@ stage_kernel(StageArgs a) {
             for (int c = 0; c < 3; ++c) r[c] += s_f[(3 * 6 + co + c) * N2 + i + N1 * k];
         }
         const int64_t node = eo + k * N2 + q;
-        if (po >= 0) {
+        if (FEAT == 2 && po >= 0) {
             // pml_step: with sa, sb, sd = sigma/eps of the component's own,
             // next and last axis, the auxiliary field P (B for H, D for E)
             //   resP = res - sb P,  res += (sa - sb) P - sd mu H  (H)
