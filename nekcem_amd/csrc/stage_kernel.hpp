@@ -236,12 +236,15 @@ struct Shape {
 #endif
 
 // OCC = 1 asks the register allocator for 4 waves per SIMD (128 VGPRs,
-// spilling if it must); OCC = 0 leaves it free.  The autotuner picks.
+// spilling if it must); OCC = 0 leaves it free, except that the FEAT = 1
+// instance of a 4-wave workgroup is held to 3 waves per SIMD (168 VGPRs:
+// three elements per CU instead of two; a few spills, 10 % faster at N = 7
+// with a source).  The autotuner picks.
 // FEAT = 1 compiles the in-stage source, Drude and Lorentz terms, FEAT = 2
 // the PML and the incident field as well; FEAT = 0 leaves them all out, so
 // the plain step carries none of their registers or instructions.
 template <int N1, int MAT, int MODE, int PACK, int OCC, int FEAT>
-__global__ void __launch_bounds__((Shape<N1, PACK>::BS)) __attribute__((amdgpu_waves_per_eu(OCC ? 4 : 1)))
+__global__ void __launch_bounds__((Shape<N1, PACK>::BS)) __attribute__((amdgpu_waves_per_eu(OCC ? 4 : (FEAT == 1 && Shape<N1, PACK>::BS <= 256 ? 3 : 1))))
 stage_kernel(StageArgs a) {
     using S = Shape<N1, PACK>;
     constexpr int N2 = S::N2, N3 = S::N3, KH = S::KH, BS = S::BS, LANES = S::LANES;
