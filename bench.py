@@ -11,9 +11,12 @@ state resident in HBM for the whole timed region.
 
 Also reported:
   roofline      the fused stage kernel (5 launches per step): algorithmic
-                32 B per GLL-pt.field.stage (read u, read k, write k, write
-                u) x 6 x npts per launch, over its average launch duration
-                from HIP events recorded on the library's stream;
+                bytes per launch = B_ALG x 6 x npts, B_ALG = 28.8 B per
+                GLL-pt.field.stage (read u, read k, write k, write u = 32 B,
+                less the k read of stage 1 and the k write of stage 5), over
+                its average launch duration from HIP events recorded on the
+                library's stream; frac_at_32B is the same rate priced at the
+                plain 32 B floor (G/s x 32 B / peak);
   cpu_baseline  the CPU oracle (a loop-for-loop C restatement of the
                 reference path, OpenMP) on the same workload for a bounded
                 number of steps on this host's cores;
@@ -494,7 +497,8 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "nkc::stage_kernel (fused stage)",
                      "avg_launch_us": avg_launch_s * 1e6,
-                     "alg_bytes_per_launch": alg_bytes},
+                     "alg_bytes_per_launch": alg_bytes, "alg_bytes_per_unit": B_ALG,
+                     "frac_at_32B": value * 32.0 / HBM_PEAK_GBS},
         "cpu_baseline": cpu,
         "parity": parity,
         "batched": bat,

@@ -276,21 +276,24 @@ stage_kernel(StageArgs a) {
         const int v = t + BS * n;
         ld[n] = (v < 6 * N3) ? a.uin[(v / N3) * fs + eo + (v % N3)] : 0.0;
     }
-    // RK register of this role's fields and levels
-    double kp[3][KH];
-    if (MODE == MODE_RK && act && !(a.kflags & 1)) {
+    // RK register of this role's fields and levels, up front (KPRE) or, in
+    // the 4-waves/SIMD layouts, level by level in phase 2 (3 KH fewer live
+    // doubles, which the 128-VGPR budget would otherwise spill)
+    constexpr bool KPRE = !OCC;
+    double kp[3][KPRE ? KH : 1];
+    if (MODE == MODE_RK && act && !(KPRE && (a.kflags & 1))) {
 #pragma unroll
         for (int c = 0; c < 3; ++c)
 #pragma unroll
-            for (int kk = 0; kk < KH; ++kk) kp[c][kk] = 0.0;
+            for (int kk = 0; kk < (KPRE ? KH : 1); ++kk) kp[c][kk] = 0.0;
     }
-    if (MODE == MODE_RK && act && (a.kflags & 1)) {
+    if (KPRE && MODE == MODE_RK && act && (a.kflags & 1)) {
 #pragma unroll
         for (int c = 0; c < 3; ++c)
 #pragma unroll
             for (int kk = 0; kk < KH; ++kk) {
                 const int k = k0 + kk;
-                kp[c][kk] = (k < N1) ? ld_stream(a.kbuf + (3 * grp + c) * fs + eo + k * N2 + q) : 0.0;
+                kp[c][KPRE ? kk : 0] = (k < N1) ? ld_stream(a.kbuf + (3 * grp + c) * fs + eo + k * N2 + q) : 0.0;
             }
     }
     // source amplitude of this role's levels (E roles), with the HBM
@@ -403,6 +406,12 @@ stage_kernel(StageArgs a) {
         const int k = k0 + kk;
         if (k >= N1) break;
         const int vloc = k * N2 + q;
+        double kold[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            kold[c] = (KPRE || MODE != MODE_RK || !(a.kflags & 1))
+                          ? kp[c][KPRE ? kk : 0]
+                          : ld_stream(a.kbuf + (3 * grp + c) * fs + eo + vloc);
         double S1_r = 0, S2_r = 0, S0_s = 0, S2_s = 0, S0_t = 0, S1_t = 0;
 #pragma unroll
         for (int l = 0; l < N1; ++l) {
@@ -538,7 +547,7 @@ stage_kernel(StageArgs a) {
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 const int64_t p = (co + c) * fs + node;
-                const double kk2 = a.rka * kp[c][kk] + dt * r[c];
+                const double kk2 = a.rka * kold[c] + dt * r[c];
                 if (a.kflags & 2) st_stream(a.kbuf + p, kk2);
                 st_stream(a.uout + p, s_u[(co + c) * N3 + own] + a.rkb * kk2);
             }
