@@ -376,7 +376,7 @@ stage_kernel(StageArgs a) {
     const double *S2 = s_u + (src + 2) * N3;
     // D rows of this lane's column cached in registers, except at high
     // orders where 2 (N+1) doubles would cost occupancy: read from LDS
-    constexpr bool DREG = N1 <= 8;
+    constexpr bool DREG = N1 <= 8 && !OCC;
     double Dr[DREG ? N1 : 1], Ds[DREG ? N1 : 1];
     if (DREG) {
 #pragma unroll
