@@ -127,6 +127,12 @@ int32_t nkc_step(nkc_handle *h, double *time, double dt, int64_t nsteps);
 int32_t nkc_residual(nkc_handle *h, double rktime, double *reshn, double *resen,
                      int64_t ld);
 
+/* The volume term alone on the current state: cem_maxwell's 3-D branch
+ * (src/cem_maxwell.F:574-582, maxwell_wght_curl :1428-1497, then chsign):
+ * resen = W curl H, reshn = -W curl E with W = bm1, before the flux and the
+ * inverse mass.  Test entry point for the weighted curl. */
+int32_t nkc_curl(nkc_handle *h, double *reshn, double *resen, int64_t ld);
+
 /* usersrc-style source on E (the Gaussian pulse of the configs): every
  * stage adds amp(x) * dir * exp(-((t - t0)/tw)^2) to resE before the inverse
  * mass, with amp (npts) bm1-weighted as a usersrc hook would add it

@@ -70,6 +70,7 @@ SIGNATURES = [
     ("nkc_get_state", C.c_int32, [C.c_void_p, _dp, _dp, _dp, _dp, C.c_int64]),
     ("nkc_step", C.c_int32, [C.c_void_p, _dp, C.c_double, C.c_int64]),
     ("nkc_residual", C.c_int32, [C.c_void_p, C.c_double, _dp, _dp, C.c_int64]),
+    ("nkc_curl", C.c_int32, [C.c_void_p, _dp, _dp, C.c_int64]),
     ("nkc_set_source", C.c_int32, [C.c_void_p, _dp, _dp, C.c_double, C.c_double]),
     ("nkc_set_drude", C.c_int32, [C.c_void_p, C.c_int64, _ip, _dp, _dp, _dp]),
     ("nkc_get_drude", C.c_int32, [C.c_void_p, _dp, _dp, _dp]),

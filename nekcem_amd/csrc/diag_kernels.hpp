@@ -116,4 +116,51 @@ init_solution(int sol, double t, double *__restrict__ u, double *__restrict__ kr
     }
 }
 
+// The volume term alone (cem_maxwell 3-D branch, src/cem_maxwell.F:574-582,
+// with maxwell_wght_curl :1428-1497): resE = W curl H, resH = -W curl E,
+// W = bm1 = J w3, no flux and no inverse mass; a test entry.  One
+// workgroup per element, one lane per node; the element's six fields and D
+// staged in LDS.  geo[9 e + 0..2] = (dr/dx, ds/dy, dt/dz) of the affine
+// element.
+template <int N1>
+__global__ void __launch_bounds__(((N1 * N1 * N1 + 63) / 64) * 64)
+curl_kernel(const double *__restrict__ u, double *__restrict__ res, int64_t fs, const double *__restrict__ D,
+            const double *__restrict__ geo, const double *__restrict__ jac, const double *__restrict__ w) {
+    constexpr int N2 = N1 * N1, N3 = N2 * N1;
+    __shared__ double s_u[6 * N3];
+    __shared__ double s_D[N2];
+    const int e = blockIdx.x;
+    const int t = threadIdx.x;
+    const int64_t eo = (int64_t)e * N3;
+    for (int v = t; v < 6 * N3; v += blockDim.x) s_u[v] = u[(v / N3) * fs + eo + v % N3];
+    if (t < N2) s_D[t] = D[t];
+    __syncthreads();
+    if (t >= N3) return;
+    const int i = t % N1, j = (t / N1) % N1, k = t / N2;
+    const double gx = geo[(int64_t)e * 9 + 0], gy = geo[(int64_t)e * 9 + 1], gz = geo[(int64_t)e * 9 + 2];
+    // d/dr, d/ds, d/dt of the six fields at (i,j,k)
+    double dr[6], ds[6], dt[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) { dr[c] = 0.0; ds[c] = 0.0; dt[c] = 0.0; }
+    for (int l = 0; l < N1; ++l) {
+        const double di = s_D[i + N1 * l], dj = s_D[j + N1 * l], dk = s_D[k + N1 * l];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            const double *f = s_u + c * N3;
+            dr[c] += di * f[l + N1 * j + N2 * k];
+            ds[c] += dj * f[i + N1 * l + N2 * k];
+            dt[c] += dk * f[i + N1 * j + N2 * l];
+        }
+    }
+    const double bm = jac[e] * (w[i] * w[j] * w[k]);
+    // curl of (F0, F1, F2) = (dF2/dy - dF1/dz, dF0/dz - dF2/dx, dF1/dx - dF0/dy)
+    const double cH[3] = {gy * ds[2] - gz * dt[1], gz * dt[0] - gx * dr[2], gx * dr[1] - gy * ds[0]};
+    const double cE[3] = {gy * ds[5] - gz * dt[4], gz * dt[3] - gx * dr[5], gx * dr[4] - gy * ds[3]};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        res[c * fs + eo + t] = -bm * cE[c];
+        res[(3 + c) * fs + eo + t] = bm * cH[c];
+    }
+}
+
 }  // namespace nkc

@@ -981,6 +981,32 @@ int32_t nkc_residual(nkc_handle *h, double rktime, double *reshn, double *resen,
     return copy_fields_d2h(h->res, m.fs, 0, reshn, resen, m.npts, ld);
 }
 
+int32_t nkc_curl(nkc_handle *h, double *reshn, double *resen, int64_t ld) {
+    if (!h || !reshn || !resen) return fail(NKC_ERR_ARG, "null handle or output");
+    const DevMesh &m = h->m;
+    if (ld < m.npts) return fail(NKC_ERR_ARG, "ld < npts");
+    int32_t rc;
+    if ((rc = ensure_diag(h))) return rc;
+    if (!h->res) HIPCHK(hipMalloc((void **)&h->res, sizeof(double) * 6 * m.fs));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    const int bs = ((m.n1 * m.n1 * m.n1 + 63) / 64) * 64;
+    const dim3 grid((unsigned)m.nelt), block((unsigned)bs);
+    const double *u = m.u[h->cur];
+    switch (m.n1) {
+    case 4: hipLaunchKernelGGL(curl_kernel<4>, grid, block, 0, h->stream, u, h->res, m.fs, m.D, m.geo, h->djac, h->dw); break;
+    case 5: hipLaunchKernelGGL(curl_kernel<5>, grid, block, 0, h->stream, u, h->res, m.fs, m.D, m.geo, h->djac, h->dw); break;
+    case 6: hipLaunchKernelGGL(curl_kernel<6>, grid, block, 0, h->stream, u, h->res, m.fs, m.D, m.geo, h->djac, h->dw); break;
+    case 7: hipLaunchKernelGGL(curl_kernel<7>, grid, block, 0, h->stream, u, h->res, m.fs, m.D, m.geo, h->djac, h->dw); break;
+    case 8: hipLaunchKernelGGL(curl_kernel<8>, grid, block, 0, h->stream, u, h->res, m.fs, m.D, m.geo, h->djac, h->dw); break;
+    case 9: hipLaunchKernelGGL(curl_kernel<9>, grid, block, 0, h->stream, u, h->res, m.fs, m.D, m.geo, h->djac, h->dw); break;
+    case 10: hipLaunchKernelGGL(curl_kernel<10>, grid, block, 0, h->stream, u, h->res, m.fs, m.D, m.geo, h->djac, h->dw); break;
+    default: return fail(NKC_ERR_UNSUPPORTED, "polynomial order N=%d not built", m.n1 - 1);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return copy_fields_d2h(h->res, m.fs, 0, reshn, resen, m.npts, ld);
+}
+
 int32_t nkc_set_source(nkc_handle *h, const double *amp, const double dir[3], double t0, double tw) {
     if (!h) return fail(NKC_ERR_ARG, "null handle");
     HIPCHK(hipStreamSynchronize(h->stream));

@@ -272,6 +272,14 @@ class MaxwellSolver:
         _lib.check(self._L.nkc_residual(self._h, rktime, _d(rH), _d(rE), n))
         return rH, rE
 
+    def curl(self):
+        """(resH, resE) = (-W curl E, W curl H), W = bm1: cem_maxwell's volume
+        term alone on the current state (no flux, no inverse mass)."""
+        n = self.npts
+        rH, rE = np.empty((3, n)), np.empty((3, n))
+        _lib.check(self._L.nkc_curl(self._h, _d(rH), _d(rE), n))
+        return rH, rE
+
     def step(self, dt, nsteps=1):
         t = C.c_double(self.time)
         _lib.check(self._L.nkc_step(self._h, C.byref(t), float(dt), int(nsteps)))

@@ -527,6 +527,17 @@ static void maxwell_op_body(const orc_mesh *m, double rktime, const double *hn, 
                             double *reshn, double *resen, const double *J, double *resJ,
                             const double *L, double *resL, const double *P, double *resP);
 
+/* cem_maxwell's 3-D branch alone (cem_maxwell.F:574-582 + chsign):
+ * resE = W curl H, resH = -W curl E (maxwell_wght_curl :1428-1497). */
+void orc_curl(const orc_mesh *m, const double *hn, const double *en, double *reshn, double *resen)
+{
+    int64_t npts = m->npts;
+    wght_curl(m, hn, hn + npts, hn + 2 * npts, resen, resen + npts, resen + 2 * npts);
+    wght_curl(m, en, en + npts, en + 2 * npts, reshn, reshn + npts, reshn + 2 * npts);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < 3 * npts; ++i) reshn[i] = -reshn[i];
+}
+
 /* One residual evaluation (cem_maxwell_op, cem_maxwell.F:484-508).
  * hn, en, reshn, resen: 3 components each at stride npts.
  * J, resJ: Drude polarisation current (3*ndrude) or NULL. */

@@ -72,6 +72,7 @@ def lib():
         L.orc_maxwell_op_ade.argtypes = [C.c_void_p, C.c_double] + [_dp] * 8
         L.orc_rk_run_ade.argtypes = [C.c_void_p] + [_dp] * 12 + [C.c_double, C.c_int64, _dp]
         L.orc_maxwell_op_full.argtypes = [C.c_void_p, C.c_double] + [_dp] * 10
+        L.orc_curl.argtypes = [C.c_void_p] + [_dp] * 4
         L.orc_rk_run_full.argtypes = [C.c_void_p] + [_dp] * 15 + [C.c_double, C.c_int64, _dp]
         L.orc_rk_coeffs.argtypes = [_dp, _dp, _dp]
         L.orc_cem_error.argtypes = [_dp, _dp, _dp, C.c_int64, _dp, _dp]
@@ -499,6 +500,16 @@ class BoxCase:
         if P is not None:
             out += (rP,)
         return out
+
+    def curl(self, H, E):
+        """(resH, resE) = (-W curl E, W curl H): cem_maxwell's volume term
+        (maxwell_wght_curl with chsign), W = bm1."""
+        H = np.ascontiguousarray(H, dtype=np.float64).reshape(3, self.npts)
+        E = np.ascontiguousarray(E, dtype=np.float64).reshape(3, self.npts)
+        rH, rE = np.zeros_like(H), np.zeros_like(E)
+        m = self._mesh()
+        lib().orc_curl(C.byref(m), _p(H), _p(E), _p(rH), _p(rE))
+        return rH, rE
 
     def run(self, H, E, dt, nsteps, t=0.0, kH=None, kE=None, J=None, kJ=None, L=None, kL=None, P=None, kP=None):
         """nsteps of cem_maxwell_op_rk from (H, E) at time t.  Returns a dict

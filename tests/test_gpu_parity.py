@@ -354,3 +354,24 @@ def test_lane_layouts_bit_identical_and_tuner(N):
     for layout, o in enumerate(out[1:], start=1):
         for a, b in zip(out[0], o):
             assert np.array_equal(a, b), (layout, float(np.max(np.abs(a - b))))
+
+
+@pytest.mark.parametrize("N", [4, 7, 9])
+def test_curl_parity(oracle, N):
+    """nkc_curl (the weighted volume curl alone) against the oracle's
+    cem_maxwell volume term on a random state, PEC/periodic box with
+    per-element materials (which the curl does not see): <= 1e-11 of the
+    field's max."""
+    rng = np.random.default_rng(N)
+    nel = (2, 3, 2)
+    n1 = N + 1
+    nelt = int(np.prod(nel))
+    eps = np.repeat(rng.uniform(1, 4, nelt), n1 ** 3)
+    mesh, ref, gpu = _pair(oracle, nel, N, bc=("P", "PEC", "P"), lo=(0, -1, 0), hi=(1, 1, 2), eps=eps)
+    H = rng.standard_normal((3, mesh.npts))
+    E = rng.standard_normal((3, mesh.npts))
+    gpu.set_state(H, E)
+    rH, rE = gpu.curl()
+    oH, oE = ref.curl(H, E)
+    for a, b in ((rH, oH), (rE, oE)):
+        assert np.max(np.abs(a - b)) <= 1e-11 * np.max(np.abs(b))

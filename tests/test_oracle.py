@@ -331,6 +331,22 @@ def test_gaussian_source_uniform(oracle):
     assert np.max(np.abs(out["E"][2] - exact)) < 1e-10
 
 
+def test_weighted_curl_exact_for_polynomials(oracle):
+    """orc_curl (cem_maxwell's volume term): for fields of degree <= N the
+    GLL derivative is exact, so resE = bm1 curl H and resH = -bm1 curl E to
+    round-off, on a box with unequal element sizes."""
+    c = oracle.BoxCase(oracle.BoxSpec(nel=(2, 3, 2), N=5, lo=(0, -1, 0), hi=(1, 1, 2), bc=("P", "PEC", "P")))
+    x, y, z = c.xm1, c.ym1, c.zm1
+    E = np.stack([y ** 2 * z, z ** 3, x ** 2 * y])
+    H = np.stack([x * y * z, y ** 2, z ** 2 * x])
+    rH, rE = c.curl(H, E)
+    curlE = np.stack([x ** 2 - 3 * z ** 2, y ** 2 - 2 * x * y, -2 * y * z])
+    curlH = np.stack([0 * x, x * y - z ** 2, -x * z])
+    assert np.max(np.abs(rH + c.bm1 * curlE)) < 1e-14
+    assert np.max(np.abs(rE - c.bm1 * curlH)) < 1e-14
+    assert np.max(np.abs(rE)) > 1e-2
+
+
 # ------------------------------------------- PML and incident field (oracle) --
 def test_pml_layers_from_marching(oracle):
     """pml_fill_faceary + march_faces + pml_extent_and_tags on boxes: the
