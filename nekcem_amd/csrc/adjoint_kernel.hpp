@@ -214,8 +214,11 @@ __device__ __forceinline__ void adj_face(double sg, int type, const double vo[6]
 
 // GM = 1: also the eps/mu gradients (needs the stage residual); GM = 0: the
 // adjoint state and the Drude gradients only (no residual store is read).
+// A 4-wave workgroup is held to 3 waves per SIMD (168 VGPRs, a few spills:
+// three elements per CU, as its LDS allows), so that lambda can stay in
+// registers from its load to the update instead of being read twice.
 template <int N1, int MAT, int GM>
-__global__ void __launch_bounds__(Shape<N1>::BS) NKC_OCC
+__global__ void __launch_bounds__(Shape<N1>::BS) __attribute__((amdgpu_waves_per_eu(Shape<N1>::BS <= 256 ? 3 : 1)))
 adjoint_kernel(AdjArgs a) {
     using S = Shape<N1>;
     constexpr int N2 = S::N2, N3 = S::N3, KH = S::KH, LANES = S::LANES;
@@ -246,8 +249,9 @@ adjoint_kernel(AdjArgs a) {
     // here, so that the role's lambda, kappa, r and g streams are all
     // loaded up front ----
     if (t < N2) s_D[t] = a.D[t];
+    double lm[3][KH];                    // lambda of the role's levels, kept for the update
     if (act) {
-        double lm[3][KH], kp[3][KH], rr[3][KH], gg[KH], pm[KH];
+        double kp[3][KH], rr[3][KH], gg[KH], pm[KH];
         double *g = grp ? a.geps : a.gmu;
 #pragma unroll
         for (int kk = 0; kk < KH; ++kk) {
@@ -463,7 +467,7 @@ adjoint_kernel(AdjArgs a) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const int64_t p = (fo + c) * fs + node;
-            st_stream(a.lam_out + p, a.lam_in[p] + a.dt * w[c]);
+            st_stream(a.lam_out + p, lm[c][kk] + a.dt * w[c]);
         }
     }
 }

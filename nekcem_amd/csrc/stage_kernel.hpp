@@ -34,9 +34,6 @@
 #ifndef NKC_NT_LOAD_K
 #define NKC_NT_LOAD_K 1     // non-temporal loads of the RK register
 #endif
-#ifndef NKC_WAVES_PER_EU
-#define NKC_WAVES_PER_EU 0  // >0: ask the register allocator for this occupancy
-#endif
 
 namespace nkc {
 
@@ -229,11 +226,6 @@ struct Shape {
     }
 };
 
-#if NKC_WAVES_PER_EU > 0
-#define NKC_OCC __attribute__((amdgpu_waves_per_eu(NKC_WAVES_PER_EU)))
-#else
-#define NKC_OCC
-#endif
 
 // OCC = 1 asks the register allocator for 4 waves per SIMD (128 VGPRs,
 // spilling if it must); OCC = 0 leaves it free, except that the FEAT = 1
