@@ -76,7 +76,15 @@ int32_t dev_alloc(double **p, size_t n) {
 template <class T>
 int32_t upload(T **dst, const std::vector<T> &src) {
     HIPCHK(hipMalloc((void **)dst, sizeof(T) * (src.empty() ? 1 : src.size())));
-    if (!src.empty()) HIPCHK(hipMemcpy(*dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice));
+    if (!src.empty()) {
+        const hipError_t e = hipMemcpy(*dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            // no half-filled table is left behind
+            (void)hipFree(*dst);
+            *dst = nullptr;
+            return fail(NKC_ERR_HIP, "hipMemcpy: %s", hipGetErrorString(e));
+        }
+    }
     return NKC_OK;
 }
 
@@ -529,10 +537,16 @@ int32_t capture(nkc_handle *h, double dt) {
         rc = queue_step(h, from, dt);
         hipGraph_t g = nullptr;
         const hipError_t ee = hipStreamEndCapture(h->stream, &g);
-        if (rc) return rc;
-        if (ee != hipSuccess) return fail(NKC_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(ee));
-        HIPCHK(hipGraphInstantiate(&h->gexec[from], g, nullptr, nullptr, 0));
-        HIPCHK(hipGraphDestroy(g));
+        if (rc || ee != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc ? rc : fail(NKC_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(ee));
+        }
+        const hipError_t ei = hipGraphInstantiate(&h->gexec[from], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+            h->gexec[from] = nullptr;
+            return fail(NKC_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(ei));
+        }
     }
     h->graph_dt = dt;
     return NKC_OK;
@@ -803,10 +817,16 @@ int32_t batch_capture(nkc_batch *b) {
         rc = batch_queue_step(b, from);
         hipGraph_t g = nullptr;
         const hipError_t ee = hipStreamEndCapture(b->main, &g);
-        if (rc) return rc;
-        if (ee != hipSuccess) return fail(NKC_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(ee));
-        HIPCHK(hipGraphInstantiate(&b->gexec[from], g, nullptr, nullptr, 0));
-        HIPCHK(hipGraphDestroy(g));
+        if (rc || ee != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc ? rc : fail(NKC_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(ee));
+        }
+        const hipError_t ei = hipGraphInstantiate(&b->gexec[from], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+            b->gexec[from] = nullptr;
+            return fail(NKC_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(ei));
+        }
     }
     b->graph_ok = true;
     return NKC_OK;
@@ -1036,8 +1056,10 @@ int32_t nkc_set_source(nkc_handle *h, const double *amp, const double dir[3], do
                 }
     }
     if (vals.empty()) return NKC_OK;
-    if ((rc = upload(&h->src_off, off))) return rc;
+    // the offset table switches the term on: upload it last, so that a
+    // failed upload leaves the source off rather than half set
     if ((rc = upload(&h->src_amp, vals))) return rc;
+    if ((rc = upload(&h->src_off, off))) return rc;
     for (int c = 0; c < 3; ++c) h->src_dir[c] = dir[c];
     h->src_t0 = t0;
     h->src_tw = tw;
@@ -1097,11 +1119,11 @@ int32_t nkc_set_drude(nkc_handle *h, int64_t ndrude, const int32_t *dindex, cons
             if (kJ) kv[b + c * n3] = kJ[c * ndrude + l];
         }
     }
-    if ((rc = upload(&h->drude_off, off))) return rc;
     if ((rc = upload(&h->dpar, par))) return rc;
     if ((rc = upload(&h->dJ, Jv))) return rc;
     if ((rc = upload(&h->dkJ, kv))) return rc;
     if ((rc = upload(&h->dresJ, kv))) return rc;
+    if ((rc = upload(&h->drude_off, off))) return rc;   // last: switches the term on
     h->ndrude = ndrude;
     h->ndelem = nde;
     return NKC_OK;
@@ -1240,11 +1262,11 @@ int32_t nkc_set_lorentz(nkc_handle *h, int64_t nlor, const int32_t *lindex, cons
             if (kL) kv[b + c * n3] = kL[c * nlor + l];
         }
     }
-    if ((rc = upload(&h->lor_off, off))) return rc;
     if ((rc = upload(&h->lpar, par))) return rc;
     if ((rc = upload(&h->lL, Lv))) return rc;
     if ((rc = upload(&h->lkL, kv))) return rc;
     if ((rc = upload(&h->lresL, kv))) return rc;
+    if ((rc = upload(&h->lor_off, off))) return rc;     // last: switches the term on
     h->nlor = nlor;
     h->nlelem = nle;
     return NKC_OK;
@@ -1287,11 +1309,11 @@ int32_t install_pml(nkc_handle *h, const std::vector<int64_t> &elems, const std:
     for (size_t l = 0; l < elems.size(); ++l) off[elems[l]] = (int32_t)l;
     std::vector<double> zero(P.size(), 0.0);
     int32_t rc;
-    if ((rc = upload(&h->pml_off, off))) return rc;
     if ((rc = upload(&h->psig, sig))) return rc;
     if ((rc = upload(&h->pP, P))) return rc;
     if ((rc = upload(&h->pkP, zero))) return rc;
     if ((rc = upload(&h->presP, zero))) return rc;
+    if ((rc = upload(&h->pml_off, off))) return rc;     // last: switches the term on
     h->npml = (int64_t)elems.size();
     h->pml_elems = elems;
     return NKC_OK;
@@ -1578,15 +1600,22 @@ int32_t nkc_set_materials(nkc_handle *h, const double *eps, const double *mu) {
         if (!(eps[p] > 0.0) || !(mu[p] > 0.0)) return fail(NKC_ERR_ARG, "material not positive at node %lld", (long long)p);
     HIPCHK(hipStreamSynchronize(h->stream));
     free_adjoint(h);                      // the stored trajectory has the old materials
-    set_node_materials(h->hm, eps, mu, h->C0);
-    (void)hipFree(h->m.fco); h->m.fco = nullptr;
-    (void)hipFree(h->m.ieps); h->m.ieps = nullptr;
-    (void)hipFree(h->m.imu); h->m.imu = nullptr;
-    (void)hipFree(h->yz); h->yz = nullptr;
+    // the new tables replace the old ones (host and device) only once all
+    // three are on the device, so a failed upload leaves the handle as it was
+    HostMesh hm = h->hm;
+    set_node_materials(hm, eps, mu, h->C0);
+    double *fco = nullptr, *ieps = nullptr, *imu = nullptr;
     int32_t rc;
-    if ((rc = upload(&h->m.fco, h->hm.fco))) return rc;
-    if ((rc = upload(&h->m.ieps, h->hm.ieps))) return rc;
-    if ((rc = upload(&h->m.imu, h->hm.imu))) return rc;
+    if ((rc = upload(&fco, hm.fco)) || (rc = upload(&ieps, hm.ieps)) || (rc = upload(&imu, hm.imu))) {
+        (void)hipFree(fco); (void)hipFree(ieps); (void)hipFree(imu);
+        return rc;
+    }
+    h->hm = std::move(hm);
+    (void)hipFree(h->yz); h->yz = nullptr;
+    (void)hipFree(h->m.fco); (void)hipFree(h->m.ieps); (void)hipFree(h->m.imu);
+    h->m.fco = fco;
+    h->m.ieps = ieps;
+    h->m.imu = imu;
     h->m.mat = MAT_NODE;
     return drop_graphs(h);
 }
