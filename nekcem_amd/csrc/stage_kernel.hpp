@@ -241,7 +241,14 @@ stage_kernel(StageArgs a) {
     using S = Shape<N1, PACK>;
     constexpr int N2 = S::N2, N3 = S::N3, KH = S::KH, BS = S::BS, LANES = S::LANES;
     __shared__ double s_u[6 * N3];      // element, swizzled
-    __shared__ double s_f[24 * N2];     // [face 0..3][comp 0..5][a + n1 b]
+    // ZF: the z-face fluxes are formed in phase 1 as well (roles 0 and 3,
+    // once per face) and read back from LDS at the end levels, so phase 2
+    // holds no z-neighbour trace or z-face coefficients (N = 9: 147 -> 125
+    // VGPRs, stage 433 -> 387 us plain, 759 -> 673 us with Drude).  Not at
+    // N = 7, where the extra 12 n1^2 doubles of LDS would cost the fourth
+    // workgroup per CU, nor at N = 6, where it measured 7 % slower plain.
+    constexpr bool ZF = N1 >= 9 || N1 <= 6;
+    __shared__ double s_f[(ZF ? 36 : 24) * N2];   // [face][comp 0..5][a + n1 b]
     __shared__ double s_D[N2];          // D(i,l) at [i + n1 l]
 
     const int e = remap_element(blockIdx.x, a.nelt, a.per_xcd);
@@ -308,14 +315,16 @@ stage_kernel(StageArgs a) {
 #pragma unroll
     for (int c = 0; c < 6; ++c)
         nxy[c] = (act && nbr_xy >= 0) ? a.uin[c * fs + (int64_t)nbr_xy * N3 + nbn_xy] : 0.0;
-    // neighbour trace of the z face inside this role's level half
+    // neighbour trace of the z face inside this role's level half (with ZF
+    // only roles 0 and 3 take the z- / z+ face, in phase 1)
     double nz[6];
     const int zface = half ? 5 : 4;
     const int nbr_z = nb[zface];
     const int64_t nbn_z = half ? (int64_t)q : (int64_t)(N1 - 1) * N2 + q;
+    const bool zrole = !ZF || role == 0 || role == 3;
 #pragma unroll
     for (int c = 0; c < 6; ++c)
-        nz[c] = (act && nbr_z >= 0) ? a.uin[c * fs + (int64_t)nbr_z * N3 + nbn_z] : 0.0;
+        nz[c] = (act && zrole && nbr_z >= 0) ? a.uin[c * fs + (int64_t)nbr_z * N3 + nbn_z] : 0.0;
 
     if (t < N2) s_D[t] = a.D[t];
 #pragma unroll
@@ -354,6 +363,22 @@ stage_kernel(StageArgs a) {
             s_f[(role * 6 + c) * N2 + q] = FH[c];
             s_f[(role * 6 + 3 + c) * N2 + q] = FE[c];
         }
+        if constexpr (ZF) if (zrole) {
+            const int kz = half ? N1 - 1 : 0;
+            const int ownz = S::idx(i, j, kz);
+            double cz[4];
+#pragma unroll
+            for (int c = 0; c < 6; ++c) um[c] = s_u[c * N3 + ownz];
+            load_coef<N1, MAT>(a, e, zface, q, cz);
+            if (FEAT == 2 && (imask >> zface) & 1) add_incident(a, eo + ownz, tinc, um);
+            if (FEAT == 2 && (imask >> (8 + zface)) & 1) add_incident(a, (int64_t)nbr_z * N3 + nbn_z, tinc, nz);
+            face_flux<2>(half ? 1.0 : -1.0, nbr_z, um, nz, cz, geo[3 + zface], FH, FE);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                s_f[(zface * 6 + c) * N2 + q] = FH[c];
+                s_f[(zface * 6 + 3 + c) * N2 + q] = FE[c];
+            }
+        }
     }
     __syncthreads();
     if (!act) return;
@@ -378,7 +403,7 @@ stage_kernel(StageArgs a) {
     if (MAT == MAT_ELEM) minv = grp ? a.ieps[e] : a.imu[e];
     const double dt = a.dt_e ? a.dt_e[e] : a.dt;
     double cfz[4];
-    load_coef<N1, MAT>(a, e, zface, q, cfz);
+    if constexpr (!ZF) load_coef<N1, MAT>(a, e, zface, q, cfz);
     // usersrc time factor: one value per stage (and element, with per-element dt)
     double sg = 0.0;
     if (so >= 0) {
@@ -423,7 +448,12 @@ stage_kernel(StageArgs a) {
         r[1] = sgn * (gz * S0_t - gx * S2_r);
         r[2] = sgn * (gx * S1_r - gy * S0_s);
         const int own = S::idx(i, j, k);
-        if (k == (half ? N1 - 1 : 0)) {
+        if constexpr (ZF) {
+            if (k == (half ? N1 - 1 : 0)) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) r[c] += s_f[(zface * 6 + 3 * grp + c) * N2 + q];
+            }
+        } else if (k == (half ? N1 - 1 : 0)) {
             double um[6], FH[3], FE[3];
 #pragma unroll
             for (int c = 0; c < 6; ++c) um[c] = s_u[c * N3 + own];
