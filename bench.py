@@ -42,6 +42,14 @@ HBM_PEAK_GBS = 8000.0
 # next step's first stage ignores it): 24 B.  Mean over a step: 28.8 B.
 B_STAGE = (24.0, 32.0, 32.0, 32.0, 24.0)
 B_ALG = sum(B_STAGE) / len(B_STAGE)
+FP64_PEAK_TFLOPS = 78.6
+
+
+def f_alg(N):
+    """Algorithmic FP64 flops per GLL-pt.field.stage: the contractions
+    (6 (N+1)), curl combine, inverse mass and RK update (19.5), flux and lift
+    per face node amortised over the element (80 / (N+1)); 77.5 at N = 7."""
+    return 6.0 * (N + 1) + 19.5 + 80.0 / (N + 1)
 
 
 def _hip():
@@ -498,7 +506,11 @@ def main():
                      "kernel": "nkc::stage_kernel (fused stage)",
                      "avg_launch_us": avg_launch_s * 1e6,
                      "alg_bytes_per_launch": alg_bytes, "alg_bytes_per_unit": B_ALG,
-                     "frac_at_32B": value * 32.0 / HBM_PEAK_GBS},
+                     "frac_at_32B": value * 32.0 / HBM_PEAK_GBS,
+                     "fp64": {"alg_flops_per_unit": f_alg(args.N),
+                              "achieved_tflops": f_alg(args.N) * units / wall / 1e12,
+                              "peak_tflops": FP64_PEAK_TFLOPS,
+                              "frac": f_alg(args.N) * units / wall / 1e12 / FP64_PEAK_TFLOPS}},
         "cpu_baseline": cpu,
         "parity": parity,
         "batched": bat,
