@@ -214,3 +214,41 @@ def test_adjoint_refuses_pml(oracle):
     s, _, _, _ = _dielectric(oracle, False, with_oracle=False)
     with pytest.raises(Exception, match="PML"):
         s.adjoint_forward(0.005, 4, 2)
+
+
+def test_pml_and_incident_argument_checks(oracle):
+    """The set-up entry points refuse what the kernels cannot honour, with a
+    message and no state change: a PML face that is not a wall, a PML
+    thicker than the mesh, colliding layers, parameters outside
+    pml_errchk's ranges, incident faces out of range; and a PML handle
+    keeps stepping after a refused call."""
+    from nekcem_amd._lib import NkcError
+    print("[case] PML / incident argument checks", flush=True)
+    s = MaxwellSolver.from_box((2, 4, 2), 4, (0, -1, 0), (1, 1, 1), ("P", "PML", "P"))
+    pf = s._pml_faces()
+    assert pf.sum() == 2 * 2 * 2     # y- faces of the bottom layer, y+ of the top
+    bad = pf.copy()
+    bad[0, 1] = 1                    # x+ face of element 0: interior
+    with pytest.raises(NkcError, match="not a wall"):
+        s.setup_pml(1, 3.0, 1e-8, pml_faces=bad)
+    with pytest.raises(NkcError, match="thicker|collide"):
+        s.setup_pml(3, 3.0, 1e-8)
+    with pytest.raises(NkcError, match="thickness"):
+        s.setup_pml(11, 3.0, 1e-8)
+    with pytest.raises(NkcError, match="order"):
+        s.setup_pml(1, 0.5, 1e-8)
+    with pytest.raises(NkcError, match="reflection"):
+        s.setup_pml(1, 3.0, 0.0)
+    ext = s.setup_pml(2, 3.0, 1e-8)
+    assert np.allclose(ext["inner"][2:4], [0.0, 0.0]) and np.allclose(ext["outer"][2:4], [-1.0, 1.0])
+    with pytest.raises(NkcError, match="out of range"):
+        s.set_incident([(16, 1)], (0, 1, 0), 1.0, (1, 0, 0), (0, 0, 1))
+    with pytest.raises(NkcError, match="out of range"):
+        s.set_incident([(0, 7)], (0, 1, 0), 1.0, (1, 0, 0), (0, 0, 1))
+    n = s.npts
+    rng = np.random.default_rng(3)
+    s.set_state(rng.standard_normal((3, n)), rng.standard_normal((3, n)))
+    s.step(1e-3, 3)
+    H, E = s.get_state()
+    assert np.all(np.isfinite(H)) and np.all(np.isfinite(E))
+    s.close()
