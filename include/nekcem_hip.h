@@ -301,20 +301,25 @@ int32_t nkc_info(nkc_handle *h, int64_t out[8]);
 /* Enable (1, default) or disable (0) hipGraph replay in nkc_step. */
 int32_t nkc_set_graph(nkc_handle *h, int32_t enable);
 
-/* Stage-kernel layout (no counterpart in the reference).  Bit 0: the four
- * lane roles of an element get whole waves each (0) or their (N+1)^2
- * lanes packed back to back in fewer waves (1).  Bit 1: the register
- * allocator is left free (0) or held to 128 VGPRs for 4 waves per SIMD
- * (1).  The first nkc_step times the layouts on the handle's mesh and keeps
- * the fastest (an autotuner; a packed layout with the same workgroup as
- * the unpacked one is not timed).  nkc_set_layout forces 0..3, or -1 to
- * tune again at the next step; nkc_tune tunes now: out[5] = {us per stage
- * for layouts 0..3 (NaN: not timed), chosen layout} (out may be NULL).
- * All layouts compute the same arithmetic: results are bit-identical. */
+/* Stage-kernel layout (no counterpart in the reference).  Layouts 0..3
+ * take one element per workgroup.  Bit 0: the four lane roles of an
+ * element get whole waves each (0) or their (N+1)^2 lanes packed back to
+ * back in fewer waves (1).  Bit 1: the register allocator is left free (0)
+ * or held to 128 VGPRs for 4 waves per SIMD (1).  Layouts 4 and 5 take two
+ * elements per workgroup, packed, with the free / 128-VGPR register budget;
+ * they are built for N <= 5 only.  The first nkc_step times the layouts on
+ * the handle's mesh and keeps the fastest (an autotuner; a packed
+ * one-element layout with the same workgroup as the unpacked one, and a
+ * layout not built for the order, are not timed).  nkc_set_layout forces
+ * 0..NKC_NLAYOUT-1 (one built for the order), or -1 to tune again at the
+ * next step; nkc_tune tunes now: out[NKC_NLAYOUT + 1] = {us per stage for
+ * each layout (NaN: not timed), chosen layout} (out may be NULL).  All
+ * layouts compute the same arithmetic: results are bit-identical. */
+#define NKC_NLAYOUT 6
 int32_t nkc_set_layout(nkc_handle *h, int32_t layout);
 int32_t nkc_tune(nkc_handle *h, double *out);
-/* The current layout and the last tuner times without tuning: out[5] as
- * for nkc_tune (layout -1 before the first step). */
+/* The current layout and the last tuner times without tuning:
+ * out[NKC_NLAYOUT + 1] as for nkc_tune (layout -1 before the first step). */
 int32_t nkc_get_layout(nkc_handle *h, double *out);
 
 /* ---- batches of independent cavities --------------------------------

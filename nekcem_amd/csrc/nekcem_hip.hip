@@ -133,54 +133,74 @@ void dev_mesh_free(DevMesh &dm) {
     dm = DevMesh();
 }
 
-// Stage-kernel layouts: bit 0 = PACK (lane roles packed), bit 1 = OCC
-// (4 waves/SIMD register budget).  The residual test entry uses layout 0.
-constexpr int NLAYOUT = 4;
+// Stage-kernel layouts: 0..3 one element per workgroup, bit 0 = PACK (lane
+// roles packed), bit 1 = OCC (4 waves/SIMD register budget); 4, 5 two
+// elements per workgroup, packed, without / with the OCC budget (N <= 5
+// only).  The residual test entry uses layout 0.
+constexpr int NLAYOUT = NKC_NLAYOUT;
+constexpr int EPW_MAX_N1 = 6;           // orders with the two-element layouts
 
-template <int N1, int MAT, int MODE, int PACK, int OCC>
+template <int N1, int MAT, int MODE, int PACK, int OCC, int EPW>
 void launch_t(const StageArgs &a, int grid, hipStream_t s) {
+    const dim3 bs(Shape<N1, PACK>::BS * EPW);
     if (a.pml_off || a.inc_mask)
-        hipLaunchKernelGGL((stage_kernel<N1, MAT, MODE, PACK, OCC, 2>), dim3(grid), dim3(Shape<N1, PACK>::BS), 0, s, a);
+        hipLaunchKernelGGL((stage_kernel<N1, MAT, MODE, PACK, OCC, 2, EPW>), dim3(grid), bs, 0, s, a);
     else if (a.src_off || a.drude_off || a.lor_off)
-        hipLaunchKernelGGL((stage_kernel<N1, MAT, MODE, PACK, OCC, 1>), dim3(grid), dim3(Shape<N1, PACK>::BS), 0, s, a);
+        hipLaunchKernelGGL((stage_kernel<N1, MAT, MODE, PACK, OCC, 1, EPW>), dim3(grid), bs, 0, s, a);
     else
-        hipLaunchKernelGGL((stage_kernel<N1, MAT, MODE, PACK, OCC, 0>), dim3(grid), dim3(Shape<N1, PACK>::BS), 0, s, a);
+        hipLaunchKernelGGL((stage_kernel<N1, MAT, MODE, PACK, OCC, 0, EPW>), dim3(grid), bs, 0, s, a);
 }
 
-template <int N1, int MAT, int PACK, int OCC>
+template <int N1, int MAT, int PACK, int OCC, int EPW>
 void launch_m(const StageArgs &a, int mode, int grid, hipStream_t s) {
-    if (mode == MODE_RK) launch_t<N1, MAT, MODE_RK, PACK, OCC>(a, grid, s);
-    else launch_t<N1, MAT, MODE_RESIDUAL, 0, 0>(a, grid, s);
+    if (mode == MODE_RK) launch_t<N1, MAT, MODE_RK, PACK, OCC, EPW>(a, grid, s);
+    else launch_t<N1, MAT, MODE_RESIDUAL, 0, 0, 1>(a, grid, s);
 }
 
-template <int N1, int PACK, int OCC>
+template <int N1, int PACK, int OCC, int EPW>
 void launch_p(const StageArgs &a, int mat, int mode, int grid, hipStream_t s) {
-    if (mat == MAT_ELEM) launch_m<N1, MAT_ELEM, PACK, OCC>(a, mode, grid, s);
-    else launch_m<N1, MAT_NODE, PACK, OCC>(a, mode, grid, s);
+    if (mat == MAT_ELEM) launch_m<N1, MAT_ELEM, PACK, OCC, EPW>(a, mode, grid, s);
+    else launch_m<N1, MAT_NODE, PACK, OCC, EPW>(a, mode, grid, s);
 }
 
 template <int N1>
 void launch_n(const StageArgs &a, int mat, int mode, int layout, int grid, hipStream_t s) {
     switch (layout) {
-    case 1: launch_p<N1, 1, 0>(a, mat, mode, grid, s); break;
-    case 2: launch_p<N1, 0, 1>(a, mat, mode, grid, s); break;
-    case 3: launch_p<N1, 1, 1>(a, mat, mode, grid, s); break;
-    default: launch_p<N1, 0, 0>(a, mat, mode, grid, s); break;
+    case 1: launch_p<N1, 1, 0, 1>(a, mat, mode, grid, s); break;
+    case 2: launch_p<N1, 0, 1, 1>(a, mat, mode, grid, s); break;
+    case 3: launch_p<N1, 1, 1, 1>(a, mat, mode, grid, s); break;
+    case 4:
+        if constexpr (N1 <= EPW_MAX_N1) launch_p<N1, 1, 0, 2>(a, mat, mode, grid, s);
+        break;
+    case 5:
+        if constexpr (N1 <= EPW_MAX_N1) launch_p<N1, 1, 1, 2>(a, mat, mode, grid, s);
+        break;
+    default: launch_p<N1, 0, 0, 1>(a, mat, mode, grid, s); break;
     }
 }
+
+// Elements per workgroup of a layout.
+int layout_epw(int layout) { return layout >= 4 ? 2 : 1; }
+
+// Whether a layout is built for this order.
+bool layout_built(int n1, int layout) { return layout < 4 || n1 <= EPW_MAX_N1; }
 
 // Lanes a workgroup of the given layout has.
 int stage_block(int n1, int layout) {
     const int n2 = n1 * n1;
-    return (layout & 1) ? (4 * n2 + 63) / 64 * 64 : 4 * ((n2 + 63) / 64 * 64);
+    const int one = (layout & 1) ? (4 * n2 + 63) / 64 * 64 : 4 * ((n2 + 63) / 64 * 64);
+    return one * layout_epw(layout);
 }
 
 int32_t launch_stage(const DevMesh &dm, const StageArgs &a, int mode, hipStream_t s, int layout = -2) {
-    const int per_xcd = (int)((dm.nelt + 7) / 8);
+    const int pk0 = layout == -2 ? (dm.layout < 0 ? 0 : dm.layout) : layout;
+    // the residual entry and orders without the layout run layout 0
+    const int pk = (mode == MODE_RK && layout_built(dm.n1, pk0)) ? pk0 : 0;
+    const int64_t nwg = (dm.nelt + layout_epw(pk) - 1) / layout_epw(pk);
+    const int per_xcd = (int)((nwg + 7) / 8);
     const int grid = 8 * per_xcd;
     StageArgs b = a;
     b.per_xcd = per_xcd;
-    const int pk = layout == -2 ? (dm.layout < 0 ? 0 : dm.layout) : layout;
     switch (dm.n1) {
     case 4: launch_n<4>(b, dm.mat, mode, pk, grid, s); break;
     case 5: launch_n<5>(b, dm.mat, mode, pk, grid, s); break;
@@ -228,7 +248,11 @@ int32_t stage_kflags(int s) {
 int32_t tune_stage(DevMesh &dm, const StageArgs *feat, size_t jsz, size_t lsz, size_t psz, const double *uin,
                    hipStream_t s, double us[NLAYOUT]) {
     for (int l = 0; l < NLAYOUT; ++l) us[l] = NAN;
-    const bool same = stage_block(dm.n1, 0) == stage_block(dm.n1, 1);
+    // not timed: a layout not built for this order, and a packed one-element
+    // layout whose workgroup equals its unpacked twin
+    auto skip = [&](int p) {
+        return !layout_built(dm.n1, p) || (p < 4 && (p & 1) && stage_block(dm.n1, p) == stage_block(dm.n1, p - 1));
+    };
     const size_t blk = 6 * (size_t)dm.fs;
     const size_t jx = (feat && feat->drude_off) ? jsz : 0;
     const size_t lx = (feat && feat->lor_off) ? lsz : 0;
@@ -287,7 +311,7 @@ int32_t tune_stage(DevMesh &dm, const StageArgs *feat, size_t jsz, size_t lsz, s
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = fail(NKC_ERR_HIP, "hipEventCreate");
     for (int round = 0; round < 3 && rc == NKC_OK; ++round)
         for (int p = 0; p < NLAYOUT && rc == NKC_OK; ++p) {
-            if (same && (p & 1)) continue;
+            if (skip(p)) continue;
             if ((rc = launch_stage(dm, a, MODE_RK, s, p))) break;
             if (hipEventRecord(e0, s) != hipSuccess) { rc = fail(NKC_ERR_HIP, "hipEventRecord"); break; }
             for (int r = 0; r < 4 && rc == NKC_OK; ++r) rc = launch_stage(dm, a, MODE_RK, s, p);
@@ -395,7 +419,7 @@ struct nkc_handle {
     double *yz = nullptr;                 // (sqrt(eps/mu), sqrt(mu/eps)) per ieps entry, adjoint only
     int64_t adj_store_cap = 0;            // cap on the adjoint's stage-store window (0: from free memory)
     int32_t adj_grads = 3;                // bit 0: eps/mu gradients, bit 1: Drude gradients
-    double tune_us[4] = {NAN, NAN, NAN, NAN};   // tuner's stage times per layout
+    double tune_us[NLAYOUT] = {NAN, NAN, NAN, NAN, NAN, NAN};   // tuner's stage times per layout
     // on-device diagnostics: node coordinates (3 x fs; from the box or
     // nkc_set_coords), element Jacobians and GLL weights (bm1 = jac w3),
     // partial norms, the 13 final values
@@ -1567,6 +1591,8 @@ int32_t nkc_set_graph(nkc_handle *h, int32_t enable) {
 int32_t nkc_set_layout(nkc_handle *h, int32_t layout) {
     if (!h) return fail(NKC_ERR_ARG, "null handle");
     if (layout < -1 || layout >= NLAYOUT) return fail(NKC_ERR_ARG, "layout %d not in -1..%d", layout, NLAYOUT - 1);
+    if (layout >= 0 && !layout_built(h->m.n1, layout))
+        return fail(NKC_ERR_ARG, "layout %d is not built for N=%d", layout, h->m.n1 - 1);
     HIPCHK(hipStreamSynchronize(h->stream));
     h->m.layout = layout;
     return drop_graphs(h);

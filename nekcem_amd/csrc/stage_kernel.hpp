@@ -235,12 +235,16 @@ struct Shape {
 // FEAT = 1 compiles the in-stage source, Drude and Lorentz terms, FEAT = 2
 // the PML and the incident field as well; FEAT = 0 leaves them all out, so
 // the plain step carries none of their registers or instructions.
-template <int N1, int MAT, int MODE, int PACK, int OCC, int FEAT>
-__global__ void __launch_bounds__((Shape<N1, PACK>::BS)) __attribute__((amdgpu_waves_per_eu(OCC ? 4 : (FEAT == 1 && Shape<N1, PACK>::BS <= 256 ? 3 : 1))))
+// EPW = elements per workgroup: EPW independent element blocks of BS
+// lanes side by side, each with its own LDS, sharing s_D and the barriers
+// (workgroup w takes elements EPW w' .. EPW w' + EPW - 1, w' its slot in the
+// XCD-aware order over workgroups).
+template <int N1, int MAT, int MODE, int PACK, int OCC, int FEAT, int EPW = 1>
+__global__ void __launch_bounds__((Shape<N1, PACK>::BS * EPW)) __attribute__((amdgpu_waves_per_eu(OCC ? 4 : (FEAT == 1 && Shape<N1, PACK>::BS * EPW <= 256 ? 3 : 1))))
 stage_kernel(StageArgs a) {
     using S = Shape<N1, PACK>;
     constexpr int N2 = S::N2, N3 = S::N3, KH = S::KH, BS = S::BS, LANES = S::LANES;
-    __shared__ double s_u[6 * N3];      // element, swizzled
+    __shared__ double s_u_all[EPW][6 * N3];   // element, swizzled
     // ZF: the z-face fluxes are formed in phase 1 as well (roles 0 and 3,
     // once per face) and read back from LDS at the end levels, so phase 2
     // holds no z-neighbour trace or z-face coefficients (N = 9: 147 -> 125
@@ -248,14 +252,19 @@ stage_kernel(StageArgs a) {
     // N = 7, where the extra 12 n1^2 doubles of LDS would cost the fourth
     // workgroup per CU, nor at N = 6, where it measured 7 % slower plain.
     constexpr bool ZF = N1 >= 9 || N1 <= 6;
-    __shared__ double s_f[(ZF ? 36 : 24) * N2];   // [face][comp 0..5][a + n1 b]
+    __shared__ double s_f_all[EPW][(ZF ? 36 : 24) * N2];   // [face][comp 0..5][a + n1 b]
     __shared__ double s_D[N2];          // D(i,l) at [i + n1 l]
 
-    const int e = remap_element(blockIdx.x, a.nelt, a.per_xcd);
-    if (e >= a.nelt) return;   // whole workgroup leaves together
-    const int t = threadIdx.x;
+    const int sub = (EPW > 1) ? (int)threadIdx.x / BS : 0;
+    const int e = remap_element(blockIdx.x, (a.nelt + EPW - 1) / EPW, a.per_xcd) * EPW + sub;
+    if (e - sub >= a.nelt) return;   // whole workgroup leaves together
+    // with EPW > 1 the last workgroup's spare blocks stay for the barriers
+    const bool valid = EPW == 1 || e < a.nelt;
+    double *const s_u = s_u_all[sub];
+    double *const s_f = s_f_all[sub];
+    const int t = (EPW > 1) ? (int)threadIdx.x % BS : (int)threadIdx.x;
     const int role = t / LANES, q = t % LANES;
-    const bool act = q < N2 && role < S::ROLES;
+    const bool act = valid && q < N2 && role < S::ROLES;
     const int i = q % N1, j = q / N1;
     const int grp = role & 1;
     const int half = (role >> 1) & 1;
@@ -265,7 +274,7 @@ stage_kernel(StageArgs a) {
 
     int nb[6];
 #pragma unroll
-    for (int f = 0; f < 6; ++f) nb[f] = a.nbr[e * 6 + f];
+    for (int f = 0; f < 6; ++f) nb[f] = valid ? a.nbr[e * 6 + f] : FACE_NONE;
 
     // ---- phase 0: every global load of the element, issued up front ----
     constexpr int NLOAD = (6 * N3 + BS - 1) / BS;
@@ -273,7 +282,7 @@ stage_kernel(StageArgs a) {
 #pragma unroll
     for (int n = 0; n < NLOAD; ++n) {
         const int v = t + BS * n;
-        ld[n] = (v < 6 * N3) ? a.uin[(v / N3) * fs + eo + (v % N3)] : 0.0;
+        ld[n] = (valid && v < 6 * N3) ? a.uin[(v / N3) * fs + eo + (v % N3)] : 0.0;
     }
     // RK register of this role's fields and levels, up front (KPRE) or, in
     // the 4-waves/SIMD layouts, level by level in phase 2 (3 KH fewer live
@@ -297,7 +306,7 @@ stage_kernel(StageArgs a) {
     }
     // source amplitude of this role's levels (E roles), with the HBM
     // round trip of the other streams
-    const int so = (FEAT && grp == 1 && a.src_off) ? a.src_off[e] : -1;
+    const int so = (FEAT && grp == 1 && a.src_off && valid) ? a.src_off[e] : -1;
     double samp[KH];
 #pragma unroll
     for (int kk = 0; kk < KH; ++kk) {
@@ -326,7 +335,7 @@ stage_kernel(StageArgs a) {
     for (int c = 0; c < 6; ++c)
         nz[c] = (act && zrole && nbr_z >= 0) ? a.uin[c * fs + (int64_t)nbr_z * N3 + nbn_z] : 0.0;
 
-    if (t < N2) s_D[t] = a.D[t];
+    if ((int)threadIdx.x < N2) s_D[threadIdx.x] = a.D[threadIdx.x];
 #pragma unroll
     for (int n = 0; n < NLOAD; ++n) {
         const int v = t + BS * n;
@@ -335,10 +344,10 @@ stage_kernel(StageArgs a) {
             s_u[f * N3 + S::idx(r % N1, (r / N1) % N1, r / N2)] = ld[n];
         }
     }
-    const double *geo = a.geo + (int64_t)e * 9;
-    const int imask = (FEAT == 2 && a.inc_mask) ? a.inc_mask[e] : 0;
+    const double *geo = a.geo + (int64_t)(valid ? e : 0) * 9;
+    const int imask = (FEAT == 2 && a.inc_mask && valid) ? a.inc_mask[e] : 0;
     // stage time for userinc (phase 1); usersrc takes it in phase 2
-    const double tinc = (FEAT == 2 && imask) ? *a.tdev + a.rkc * (a.dt_e ? a.dt_e[e] : a.dt) : 0.0;
+    const double tinc = (FEAT == 2 && imask) ? *a.tdev + a.rkc * (a.dt_e ? a.dt_e[e] : a.dt) : 0.0;   // imask != 0: e valid
     __syncthreads();
 
     // ---- phase 1: role r computes the flux of x/y face r at node (i,j) ----

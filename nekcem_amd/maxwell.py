@@ -32,6 +32,8 @@ from .mesh import BoxMesh
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
 
+NLAYOUT = 6   # stage-kernel layouts (NKC_NLAYOUT)
+
 
 def _d(a):
     return None if a is None else a.ctypes.data_as(_dp)
@@ -467,20 +469,21 @@ class MaxwellSolver:
         return out[:6], out[6:]
 
     def set_layout(self, layout):
-        """Force the stage-kernel layout (0..3), or -1 to re-tune."""
+        """Force the stage-kernel layout (0..NLAYOUT-1), or -1 to re-tune."""
         _lib.check(self._L.nkc_set_layout(self._h, int(layout)))
 
     def layout(self):
-        """dict(us=(tuner times per layout 0..3), layout) without tuning."""
-        out = np.empty(5)
+        """dict(us=(tuner times per layout, NaN: not timed), layout) without
+        tuning."""
+        out = np.empty(NLAYOUT + 1)
         _lib.check(self._L.nkc_get_layout(self._h, _d(out)))
-        return dict(us=tuple(float(v) for v in out[:4]), layout=int(out[4]))
+        return dict(us=tuple(float(v) for v in out[:NLAYOUT]), layout=int(out[NLAYOUT]))
 
     def tune(self):
-        """Run the layout autotuner now: dict(us=(per layout 0..3), layout)."""
-        out = np.empty(5)
+        """Run the layout autotuner now: dict(us=(per layout), layout)."""
+        out = np.empty(NLAYOUT + 1)
         _lib.check(self._L.nkc_tune(self._h, _d(out)))
-        return dict(us=tuple(float(v) for v in out[:4]), layout=int(out[4]))
+        return dict(us=tuple(float(v) for v in out[:NLAYOUT]), layout=int(out[NLAYOUT]))
 
     def adjoint_set_store(self, max_steps):
         """Cap the stage-store window of the adjoint (0: from free memory)."""

@@ -327,29 +327,38 @@ def test_lorentz_residual_and_steps_parity(oracle):
 
 @pytest.mark.parametrize("N", [4, 5, 6, 8, 9])
 def test_lane_layouts_bit_identical_and_tuner(N):
-    """All four stage-kernel layouts give the same bits (graph replay, 3
-    steps, per-element eps, periodic x, PEC y/z), and the tuner times them
-    and picks one."""
+    """All stage-kernel layouts built for the order give the same bits
+    (graph replay, 3 steps, per-element eps, periodic x, PEC y/z): the four
+    one-element layouts, and at N <= 5 the two two-element ones on an odd
+    element count (the last workgroup's second block is empty); the tuner
+    times them and picks one; a layout not built for the order is refused."""
     print(f"[case] lane layouts N={N}", flush=True)
-    from nekcem_amd.maxwell import MaxwellSolver
+    from nekcem_amd._lib import NkcError
+    from nekcem_amd.maxwell import MaxwellSolver, NLAYOUT
     from nekcem_amd.mesh import BoxMesh
     rng = np.random.default_rng(N)
-    mesh = BoxMesh((3, 2, 2), N, (0, 0, 0), (1.0, 0.8, 1.2), ("P", "PEC", "PEC"))
+    nel = (3, 3, 1) if N <= 5 else (3, 2, 2)
+    mesh = BoxMesh(nel, N, (0, 0, 0), (1.0, 0.8, 1.2), ("P", "PEC", "PEC"))
     n = mesh.npts
     eps = np.repeat(rng.uniform(1, 4, mesh.nelt), mesh.n1 ** 3)
     H, E = rng.standard_normal((3, n)), rng.standard_normal((3, n))
     dt = mesh.dt(0.25)
+    layouts = range(NLAYOUT) if N <= 5 else range(4)
     out = []
-    for layout in range(4):
+    for layout in layouts:
         s = MaxwellSolver(mesh, eps=eps)
         s.set_layout(layout)
         s.set_state(H, E)
         s.step(dt, 3)
         out.append(s.get_state())
-        if layout == 3:
+        if layout == layouts[-1]:
             t = s.tune()
-            assert np.isfinite(t["us"][0]) and np.isfinite(t["us"][2]) and t["layout"] in range(4)
+            assert np.isfinite(t["us"][0]) and np.isfinite(t["us"][2]) and t["layout"] in layouts
+            assert all(np.isfinite(t["us"][p]) == (N <= 5) for p in (4, 5))
             assert s.get_state()[1].tobytes() == out[-1][1].tobytes()   # tuning leaves the state
+            if N > 5:
+                with pytest.raises(NkcError, match="not built"):
+                    s.set_layout(4)
         s.close()
     for layout, o in enumerate(out[1:], start=1):
         for a, b in zip(out[0], o):
