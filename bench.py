@@ -406,9 +406,10 @@ def main():
               flush=True)
         return
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only: this build has no multi-GPU path "
-              f"(asked for --gpus {args.gpus}); the element sharding across GPUs is "
-              f"described in DESIGN.md", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the step is "
+              f"built for one GPU's element set; sharding elements in blocks across GPUs "
+              f"with a per-stage exchange of partition-face traces is not built",
+              file=sys.stderr)
         sys.exit(2)
 
     from nekcem_amd.maxwell import MaxwellSolver
