@@ -1,6 +1,8 @@
 """The Fortran host side: nekcem_amd/fortran/box_driver runs NekCEM's
 tests/3dboxper case (4x4x8, N=8, dt=2e-4, 50 steps, userchk at L2 <= 5e-10,
-Linf <= 5e-9) through the iso_c_binding module and the C-ABI."""
+Linf <= 5e-9) through the iso_c_binding module and the C-ABI, then sets a
+zero-sigma PML from Fortran arrays (the fields must not change; B and D come
+back on the PML elements only)."""
 import os
 import subprocess
 
@@ -25,3 +27,4 @@ def test_fortran_box_driver_3dboxper():
     print(r.stdout[-2000:], r.stderr[-2000:])
     assert r.returncode == 0
     assert "3dboxper passed on the HIP path" in r.stdout
+    assert "PML entry points ok" in r.stdout
