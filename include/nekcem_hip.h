@@ -189,6 +189,31 @@ int32_t nkc_get_pml(nkc_handle *h, int32_t which, double *out, int64_t ld);
 int32_t nkc_set_incident(nkc_handle *h, int64_t nface, const int32_t *faces, const double k[3],
                          double omega, const double hamp[3], const double eamp[3]);
 
+/* Graphene sheet current, the device form of userfsrc +
+ * cem_3d_graphene_current (tests/3dgraphene/3dgraphene.usr:235-262,
+ * src/cem_maxwell.F:2827-2932, called from cem_maxwell_flux3d before the
+ * face gather, :941-942): at each of the ngraph face points graphindex
+ * (1-based, NekCEM's face-point numbering (e-1) 6 nxzf + (f-1) nxzf + i)
+ * the side's sheet current J is formed from its face traces (the incident
+ * wave included) and its state, subtracted from -n x H before the gather,
+ * and the state (fjn slots 2..6: a Drude term and two critical-point pairs)
+ * advances with the stage.  graphparams(ld, 12) holds the twelve sheet
+ * parameters per point (a_d, b_d, b_cp1, a_211, a_221, b_11, b_21, b_cp2,
+ * a_212, a_222, b_12, b_22), fjn(ld, 3, 6) the initial state in slots
+ * 2..6 (slot 1, the current, is formed each stage and not read); the RK
+ * register starts at zero.  Yfac = 0.5/Y_0 and yconduc come from the
+ * current materials and follow nkc_set_materials.  Both sides of a sheet
+ * are listed, as uservp does.  ngraph = 0 removes it.  nkc_residual
+ * includes the current of the present state without advancing it; the
+ * adjoint refuses a sheet. */
+int32_t nkc_set_graphene(nkc_handle *h, int64_t ngraph, const int32_t *graphindex,
+                         const double *graphparams, const double *fjn, int64_t ld);
+
+/* Copy back the sheet state (which = 0) or its RK register (1) into slots
+ * 2..6 of fjn(ld, 3, 6) at the listed points; other entries are left as
+ * they are. */
+int32_t nkc_get_graphene(nkc_handle *h, int32_t which, double *fjn, int64_t ld);
+
 /* Drude ADE of cem_maxwell_drude (src/cem_maxwell.F:3095-3147) over the
  * ndrude volume nodes dindex (1-based): resE -= J*bmn, resJ = -a J + b E,
  * J advanced by the same low-storage RK.  params is (a, b) per listed node

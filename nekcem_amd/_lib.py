@@ -86,6 +86,8 @@ SIGNATURES = [
     ("nkc_set_pml_fields", C.c_int32, [C.c_void_p, _dp, _dp, C.c_int64]),
     ("nkc_get_pml", C.c_int32, [C.c_void_p, C.c_int32, _dp, C.c_int64]),
     ("nkc_set_incident", C.c_int32, [C.c_void_p, C.c_int64, _ip, _dp, C.c_double, _dp, _dp]),
+    ("nkc_set_graphene", C.c_int32, [C.c_void_p, C.c_int64, _ip, _dp, _dp, C.c_int64]),
+    ("nkc_get_graphene", C.c_int32, [C.c_void_p, C.c_int32, _dp, C.c_int64]),
     ("nkc_set_layout", C.c_int32, [C.c_void_p, C.c_int32]),
     ("nkc_tune", C.c_int32, [C.c_void_p, _dp]),
     ("nkc_get_layout", C.c_int32, [C.c_void_p, _dp]),

@@ -17,6 +17,7 @@
 #include <map>
 #include <string>
 #include <vector>
+#include <array>
 
 #include "../../include/nekcem_hip.h"
 #include "adjoint_kernel.hpp"
@@ -143,7 +144,7 @@ constexpr int EPW_MAX_N1 = 6;           // orders with the two-element layouts
 template <int N1, int MAT, int MODE, int PACK, int OCC, int EPW>
 void launch_t(const StageArgs &a, int grid, hipStream_t s) {
     const dim3 bs(Shape<N1, PACK>::BS * EPW);
-    if (a.pml_off || a.inc_mask)
+    if (a.pml_off || a.inc_mask || a.gr_off)
         hipLaunchKernelGGL((stage_kernel<N1, MAT, MODE, PACK, OCC, 2, EPW>), dim3(grid), bs, 0, s, a);
     else if (a.src_off || a.drude_off || a.lor_off)
         hipLaunchKernelGGL((stage_kernel<N1, MAT, MODE, PACK, OCC, 1, EPW>), dim3(grid), bs, 0, s, a);
@@ -243,10 +244,10 @@ int32_t stage_kflags(int s) {
 // and incident-field set-up (feat, may be null) it times the kernel
 // instance the step runs, the auxiliary fields and their registers
 // redirected to scratch: the state is untouched.  A packed layout whose
-// workgroup equals the unpacked one is not timed.  jsz, lsz, psz: doubles of
-// the Drude / Lorentz / PML state blocks.
-int32_t tune_stage(DevMesh &dm, const StageArgs *feat, size_t jsz, size_t lsz, size_t psz, const double *uin,
-                   hipStream_t s, double us[NLAYOUT]) {
+// workgroup equals the unpacked one is not timed.  jsz, lsz, psz, gsz:
+// doubles of the Drude / Lorentz / PML / graphene-sheet state blocks.
+int32_t tune_stage(DevMesh &dm, const StageArgs *feat, size_t jsz, size_t lsz, size_t psz, size_t gsz,
+                   const double *uin, hipStream_t s, double us[NLAYOUT]) {
     for (int l = 0; l < NLAYOUT; ++l) us[l] = NAN;
     // not timed: a layout not built for this order, and a packed one-element
     // layout whose workgroup equals its unpacked twin
@@ -257,8 +258,9 @@ int32_t tune_stage(DevMesh &dm, const StageArgs *feat, size_t jsz, size_t lsz, s
     const size_t jx = (feat && feat->drude_off) ? jsz : 0;
     const size_t lx = (feat && feat->lor_off) ? lsz : 0;
     const size_t px = (feat && feat->pml_off) ? psz : 0;
+    const size_t gx = (feat && feat->gr_off) ? gsz : 0;
     double *scr = nullptr;
-    int32_t rc = dev_alloc(&scr, 2 * blk + 2 * jx + 2 * lx + 2 * px);
+    int32_t rc = dev_alloc(&scr, 2 * blk + 2 * jx + 2 * lx + 2 * px + 2 * gx);
     if (rc) return rc;
     StageArgs a = mesh_args(dm);
     if (feat) {
@@ -284,6 +286,13 @@ int32_t tune_stage(DevMesh &dm, const StageArgs *feat, size_t jsz, size_t lsz, s
             a.psig = feat->psig;
             a.pP = scr + 2 * blk + 2 * jx + 2 * lx;
             a.pkP = scr + 2 * blk + 2 * jx + 2 * lx + px;
+        }
+        if (gx) {
+            a.gr_off = feat->gr_off;
+            a.gpar = feat->gpar;
+            a.gA = feat->gA;
+            a.gAn = scr + 2 * blk + 2 * jx + 2 * lx + 2 * px;
+            a.gK = scr + 2 * blk + 2 * jx + 2 * lx + 2 * px + gx;
         }
         if (jx) {
             a.drude_off = feat->drude_off;
@@ -403,6 +412,14 @@ struct nkc_handle {
     int64_t ninc = 0;
     int32_t *inc_mask = nullptr;
     double inc_k[3] = {0, 0, 0}, inc_om = 0, inc_h[3] = {0, 0, 0}, inc_e[3] = {0, 0, 0};
+    // graphene sheet: blocks of n2 face nodes, one per sheet face side
+    // (gr_face[b] = e*6 + kf); listed point l -> (block, node) at gr_list
+    // with its 0-based face-point number; state in gA[cur]
+    int64_t ngraph = 0;
+    std::vector<int64_t> gr_face;
+    std::vector<std::array<int64_t, 3>> gr_list;
+    int32_t *gr_off = nullptr;
+    double *gpar = nullptr, *gA[2] = {nullptr, nullptr}, *gK = nullptr;
     // Drude
     int64_t ndrude = 0, ndelem = 0;
     int32_t *drude_off = nullptr;
@@ -485,6 +502,12 @@ size_t pml_size(const nkc_handle *h) {
     return (size_t)h->npml * 6 * n3;
 }
 
+// doubles in one graphene-sheet state block (blocks x 15 x n2)
+size_t sheet_size(const nkc_handle *h) {
+    const size_t n2 = (size_t)h->m.n1 * h->m.n1;
+    return h->gr_face.size() * 15 * n2;
+}
+
 StageArgs base_args(nkc_handle *h) {
     StageArgs a = mesh_args(h->m);
     a.tdev = h->tdev;
@@ -519,6 +542,11 @@ StageArgs base_args(nkc_handle *h) {
         a.inc_e[c] = h->inc_e[c];
     }
     a.inc_om = h->inc_om;
+    a.gr_off = h->gr_off;
+    a.gpar = h->gpar;
+    a.gA = h->gA[h->cur];
+    a.gAn = h->gA[h->cur ^ 1];
+    a.gK = h->gK;
     return a;
 }
 
@@ -529,6 +557,8 @@ int32_t queue_step(nkc_handle *h, int from, double dt) {
         const int src = (from + s) & 1;
         a.uin = h->m.u[src];
         a.uout = h->m.u[src ^ 1];
+        a.gA = h->gA[src];
+        a.gAn = h->gA[src ^ 1];
         a.kbuf = h->m.k;
         a.rka = RK4A[s];
         a.kflags = stage_kflags(s);
@@ -611,6 +641,8 @@ void free_handle(nkc_handle *h) {
     (void)hipFree(h->lpar); (void)hipFree(h->lresL);
     (void)hipFree(h->pml_off); (void)hipFree(h->psig); (void)hipFree(h->pP); (void)hipFree(h->pkP);
     (void)hipFree(h->presP); (void)hipFree(h->inc_mask);
+    (void)hipFree(h->gr_off); (void)hipFree(h->gpar); (void)hipFree(h->gA[0]); (void)hipFree(h->gA[1]);
+    (void)hipFree(h->gK);
     (void)hipFree(h->xyz); (void)hipFree(h->djac); (void)hipFree(h->dw);
     (void)hipFree(h->dpart); (void)hipFree(h->dout);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -983,7 +1015,8 @@ int32_t nkc_step(nkc_handle *h, double *time, double dt, int64_t nsteps) {
     }
     if (h->m.layout < 0) {
         const StageArgs f = base_args(h);
-        const int32_t rc = tune_stage(h->m, &f, drude_size(h), lorentz_size(h), pml_size(h), h->m.u[h->cur], h->stream, h->tune_us);
+        const int32_t rc = tune_stage(h->m, &f, drude_size(h), lorentz_size(h), pml_size(h), sheet_size(h), h->m.u[h->cur], h->stream,
+                       h->tune_us);
         if (rc) return rc;
     }
     if (h->use_graph) {
@@ -1500,6 +1533,138 @@ int32_t nkc_get_pml(nkc_handle *h, int32_t which, double *out, int64_t ld) {
     return NKC_OK;
 }
 
+namespace {
+
+// sqrt(eps/mu) at in-element node v of element e (yconduc of
+// cem_maxwell_materials, src/cem_maxwell.F:276-286).
+double node_conduc(const HostMesh &hm, int64_t e, int64_t v) {
+    const int64_t n3 = (int64_t)hm.n1 * hm.n1 * hm.n1;
+    const int64_t i = hm.mat ? e * n3 + v : e;
+    return std::sqrt(hm.imu[i] / hm.ieps[i]);
+}
+
+// Yfac = 0.5/Y_0 and yconduc of every node of every sheet block into gpar
+// slots 12, 13 (Y_0 as cem_maxwell_materials forms it: the face sum of
+// yconduc, doubled on PEC faces, halved).
+void sheet_coefs(const nkc_handle *h, std::vector<double> &par) {
+    const HostMesh &hm = h->hm;
+    const int n1 = hm.n1, n2 = n1 * n1;
+    for (size_t b = 0; b < h->gr_face.size(); ++b) {
+        const int64_t e = h->gr_face[b] / 6;
+        const int kf = (int)(h->gr_face[b] % 6);
+        const int32_t nb = hm.nbr[e * 6 + kf];
+        for (int q = 0; q < n2; ++q) {
+            const double ym = node_conduc(hm, e, face_node(n1, kf, q));
+            double sum = ym;
+            if (nb >= 0) sum = ym + node_conduc(hm, nb, face_node(n1, kf ^ 1, q));
+            else if (nb == HOST_FACE_PEC) sum = 2.0 * ym;
+            const double Y0 = 0.5 * sum;
+            par[(b * 14 + 12) * n2 + q] = 0.5 / Y0;
+            par[(b * 14 + 13) * n2 + q] = ym;
+        }
+    }
+}
+
+void free_sheet(nkc_handle *h) {
+    (void)hipFree(h->gr_off); h->gr_off = nullptr;
+    (void)hipFree(h->gpar); h->gpar = nullptr;
+    (void)hipFree(h->gA[0]); h->gA[0] = nullptr;
+    (void)hipFree(h->gA[1]); h->gA[1] = nullptr;
+    (void)hipFree(h->gK); h->gK = nullptr;
+    h->gr_face.clear();
+    h->gr_list.clear();
+    h->ngraph = 0;
+}
+
+}  // namespace
+
+int32_t nkc_set_graphene(nkc_handle *h, int64_t ngraph, const int32_t *graphindex, const double *graphparams,
+                         const double *fjn, int64_t ld) {
+    if (!h) return fail(NKC_ERR_ARG, "null handle");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    free_adjoint(h);
+    h->m.layout = -1;                     // the step's kernel instance may change: tune again
+    free_sheet(h);
+    int32_t rc = drop_graphs(h);
+    if (rc || ngraph == 0) return rc;
+    const int64_t nelt = h->m.nelt;
+    const int n1 = h->m.n1, n2 = n1 * n1;
+    const int64_t nxzfl = nelt * 6 * n2;
+    if (ngraph < 0 || !graphindex || !graphparams || !fjn) return fail(NKC_ERR_ARG, "bad graphene point list");
+    if (ld < nxzfl) return fail(NKC_ERR_ARG, "ld < nxzfl (%lld)", (long long)nxzfl);
+    // blocks: every face side with a listed point, and the side across it
+    std::vector<int32_t> off(nelt * 6, -1);
+    std::vector<int64_t> face;
+    auto block = [&](int64_t ef) {
+        if (off[ef] < 0) {
+            off[ef] = (int32_t)face.size();
+            face.push_back(ef);
+        }
+        return off[ef];
+    };
+    const int *kf_ed = kf_of_ed();
+    std::vector<char> seen(nxzfl, 0);
+    std::vector<std::array<int64_t, 3>> list;
+    list.reserve(ngraph);
+    for (int64_t l = 0; l < ngraph; ++l) {
+        const int64_t j = (int64_t)graphindex[l] - 1;
+        if (j < 0 || j >= nxzfl) return fail(NKC_ERR_ARG, "graphindex(%lld) = %d out of range", (long long)l + 1, graphindex[l]);
+        if (seen[j]) return fail(NKC_ERR_ARG, "graphindex(%lld) = %d listed twice", (long long)l + 1, graphindex[l]);
+        seen[j] = 1;
+        const int64_t e = j / (6 * n2);
+        const int kf = kf_ed[(j / n2) % 6 + 1];
+        const int q = (int)(j % n2);
+        for (int c = 0; c < 12; ++c)
+            if (!std::isfinite(graphparams[c * ld + j])) return fail(NKC_ERR_ARG, "graphparams(%lld, %d) not finite", (long long)j + 1, c + 1);
+        const int32_t b = block(e * 6 + kf);
+        const int32_t nb = h->hm.nbr[e * 6 + kf];
+        if (nb >= 0) block((int64_t)nb * 6 + (kf ^ 1));
+        list.push_back({j, b, q});
+    }
+    const size_t nb2 = face.size();
+    std::vector<double> par(nb2 * 14 * n2, 0.0), st(nb2 * 15 * n2, 0.0);
+    for (const auto &x : list) {
+        const int64_t j = x[0], b = x[1], q = x[2];
+        for (int c = 0; c < 12; ++c) par[(b * 14 + c) * n2 + q] = graphparams[c * ld + j];
+        // fjn(ld, 3, 6): slots 2..6 are the state
+        for (int sl = 0; sl < 5; ++sl)
+            for (int c = 0; c < 3; ++c) st[(b * 15 + 3 * sl + c) * n2 + q] = fjn[j + ld * (c + 3 * (sl + 1))];
+    }
+    h->gr_face = face;
+    sheet_coefs(h, par);
+    const std::vector<double> zero(st.size(), 0.0);
+    if ((rc = upload(&h->gpar, par)) || (rc = upload(&h->gA[h->cur], st)) || (rc = upload(&h->gA[h->cur ^ 1], st)) ||
+        (rc = upload(&h->gK, zero))) {
+        free_sheet(h);
+        return rc;
+    }
+    if ((rc = upload(&h->gr_off, off))) {   // last: a non-null gr_off means the sheet is whole
+        free_sheet(h);
+        return rc;
+    }
+    h->gr_list = std::move(list);
+    h->ngraph = ngraph;
+    return NKC_OK;
+}
+
+int32_t nkc_get_graphene(nkc_handle *h, int32_t which, double *fjn, int64_t ld) {
+    if (!h || !fjn) return fail(NKC_ERR_ARG, "null handle or output");
+    if (!h->ngraph) return fail(NKC_ERR_ARG, "no graphene sheet set");
+    if (which < 0 || which > 1) return fail(NKC_ERR_ARG, "which must be 0 (state) or 1 (RK register)");
+    const int n2 = h->m.n1 * h->m.n1;
+    if (ld < h->m.nelt * 6 * n2) return fail(NKC_ERR_ARG, "ld < nxzfl");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    std::vector<double> buf(sheet_size(h));
+    HIPCHK(hipMemcpy(buf.data(), which == 0 ? h->gA[h->cur] : h->gK, sizeof(double) * buf.size(),
+                     hipMemcpyDeviceToHost));
+    for (const auto &x : h->gr_list) {
+        const int64_t j = x[0], b = x[1], q = x[2];
+        for (int sl = 0; sl < 5; ++sl)
+            for (int c = 0; c < 3; ++c) fjn[j + ld * (c + 3 * (sl + 1))] = buf[(b * 15 + 3 * sl + c) * n2 + q];
+    }
+    return NKC_OK;
+}
+
 int32_t nkc_set_incident(nkc_handle *h, int64_t nface, const int32_t *faces, const double k[3], double omega,
                          const double hamp[3], const double eamp[3]) {
     if (!h) return fail(NKC_ERR_ARG, "null handle");
@@ -1609,7 +1774,8 @@ int32_t nkc_tune(nkc_handle *h, double *out) {
     if (!h) return fail(NKC_ERR_ARG, "null handle");
     HIPCHK(hipStreamSynchronize(h->stream));
     const StageArgs f = base_args(h);
-    int32_t rc = tune_stage(h->m, &f, drude_size(h), lorentz_size(h), pml_size(h), h->m.u[h->cur], h->stream, h->tune_us);
+    int32_t rc = tune_stage(h->m, &f, drude_size(h), lorentz_size(h), pml_size(h), sheet_size(h), h->m.u[h->cur], h->stream,
+                       h->tune_us);
     if (rc) return rc;
     if ((rc = drop_graphs(h))) return rc;
     if (out) {
@@ -1643,6 +1809,14 @@ int32_t nkc_set_materials(nkc_handle *h, const double *eps, const double *mu) {
     h->m.ieps = ieps;
     h->m.imu = imu;
     h->m.mat = MAT_NODE;
+    if (h->ngraph) {
+        // the sheet's Yfac and yconduc follow the materials
+        const size_t n2 = (size_t)h->m.n1 * h->m.n1;
+        std::vector<double> par(h->gr_face.size() * 14 * n2);
+        HIPCHK(hipMemcpy(par.data(), h->gpar, sizeof(double) * par.size(), hipMemcpyDeviceToHost));
+        sheet_coefs(h, par);
+        HIPCHK(hipMemcpy(h->gpar, par.data(), sizeof(double) * par.size(), hipMemcpyHostToDevice));
+    }
     return drop_graphs(h);
 }
 
@@ -1863,6 +2037,7 @@ int32_t nkc_adjoint_forward(nkc_handle *h, double *time, double dt, int64_t nste
     if (h->nlor) return fail(NKC_ERR_UNSUPPORTED, "the adjoint of a Lorentz region is not built");
     if (h->npml) return fail(NKC_ERR_UNSUPPORTED, "the adjoint of a PML is not built");
     if (h->ninc) return fail(NKC_ERR_UNSUPPORTED, "the adjoint with an incident field is not built");
+    if (h->ngraph) return fail(NKC_ERR_UNSUPPORTED, "the adjoint with a graphene sheet is not built");
     free_adjoint(h);
     auto &A = h->adj;
     A.nsteps = nsteps;
@@ -2170,7 +2345,7 @@ int32_t nkc_batch_step(nkc_batch *b, const double *dt, double *times, int64_t ns
     for (auto &dm : b->bins)
         if (dm.layout < 0) {
             double us[NLAYOUT];
-            const int32_t rc = tune_stage(dm, nullptr, 0, 0, 0, dm.u[b->cur], b->main, us);
+            const int32_t rc = tune_stage(dm, nullptr, 0, 0, 0, 0, dm.u[b->cur], b->main, us);
             if (rc) return rc;
         }
     if (!b->graph_ok) {

@@ -127,6 +127,19 @@ struct StageArgs {
     const int32_t *__restrict__ inc_mask;
     const double *__restrict__ xyz;
     double inc_k[3], inc_om, inc_h[3], inc_e[3];
+    // graphene sheet current (userfsrc + cem_3d_graphene_current):
+    // gr_off[e*6 + f] = the sheet block of face f (kernel order) or -1.  Per
+    // block and face node q (the flux's in-face index, the same on both
+    // sides of a face): 14 doubles at gpar[(b*14 + c)*n2 + q] (the twelve
+    // sheet parameters, Yfac = 0.5/Y_0, yconduc) and the state (fjn slots
+    // 2..6 x 3 components) at [(b*15 + c)*n2 + q], read from gA and written
+    // to gAn (both sides of a face read each other's state, so the stages
+    // alternate two copies), the RK register gK in place.
+    const int32_t *__restrict__ gr_off;
+    const double *__restrict__ gpar;
+    const double *__restrict__ gA;
+    double *__restrict__ gAn;
+    double *__restrict__ gK;
 };
 
 // Incident plane wave at volume node p (global index), time t, added to a
@@ -149,13 +162,86 @@ __device__ __forceinline__ void ncross(double sg, const double v[3], double o[3]
     if (AX == 2) { o[0] = -sg * v[1]; o[1] = sg * v[0];  o[2] = 0.0; }
 }
 
+// Sheet current J at node q of sheet block b for the side whose outward
+// normal is sg e_AX, from that side's traces tr (H then E, the incident wave
+// included) and the stage-input state (cem_3d_graphene_current,
+// src/cem_maxwell.F:2860-2886):
+//   tmp = Yfac (-n x H + yconduc n x (E x n)),
+//   J = (s2 + s3 + s5 - (b_cp1 + b_cp2) tmp) / (1 - (b_cp1 + b_cp2) Yfac).
+// With upd the side's state takes this stage's RK update (:2887-2929):
+// f = tmp - Yfac J drives the Drude slot 2 and the critical-point pairs
+// (3, 4) and (5, 6).
+template <int AX, int N2>
+__device__ __forceinline__ void sheet_current(const StageArgs &a, int b, int q, double sg, const double tr[6],
+                                              bool upd, double dt, double J[3]) {
+    const double *p = a.gpar + (int64_t)b * 14 * N2 + q;
+    const double *A = a.gA + (int64_t)b * 15 * N2 + q;
+    double nH[3];
+    ncross<AX>(-sg, tr, nH);
+    double nEn[3] = {tr[3], tr[4], tr[5]};
+    nEn[AX] = 0.0;   // n x (E x n) = E - n (n . E) for n = +-e_AX
+    const double Yfac = p[12 * N2], yc = p[13 * N2];
+    const double cpf = p[2 * N2] + p[7 * N2];
+    const double jnf = 1.0 - cpf * Yfac;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const double A0 = A[c * N2], A1 = A[(3 + c) * N2], A3 = A[(9 + c) * N2];
+        const double tmp = Yfac * (nH[c] + yc * nEn[c]);
+        J[c] = (A0 + A1 + A3 - cpf * tmp) / jnf;
+        if (upd) {
+            const double A2 = A[(6 + c) * N2], A4 = A[(12 + c) * N2];
+            const double f = tmp - Yfac * J[c];
+            double r[5];
+            r[0] = -p[0] * A0 + p[N2] * f;
+            r[1] = A2 + p[5 * N2] * f;
+            r[2] = -p[3 * N2] * A1 - p[4 * N2] * A2 + p[6 * N2] * f;
+            r[3] = A4 + p[10 * N2] * f;
+            r[4] = -p[8 * N2] * A3 - p[9 * N2] * A4 + p[11 * N2] * f;
+            const double Ao[5] = {A0, A1, A2, A3, A4};
+            const int64_t o = (int64_t)b * 15 * N2 + q;
+#pragma unroll
+            for (int sl = 0; sl < 5; ++sl) {
+                const int64_t x = o + (3 * sl + c) * N2;
+                const double kn = a.rka * a.gK[x] + dt * r[sl];
+                a.gK[x] = kn;
+                a.gAn[x] = Ao[sl] + a.rkb * kn;
+            }
+        }
+    }
+}
+
+// The sheet term of face f's flux at node q: -(J own + J neighbour), the
+// sum the face gather forms of what userfsrc subtracts from -n x H on each
+// side.  The own side's state advances when upd.
+template <int AX, int N2>
+__device__ __forceinline__ void sheet_flux(const StageArgs &a, int e, int f, int nbr, int q, double sg,
+                                           const double um[6], const double up[6], bool upd, double dt,
+                                           double g[3]) {
+    g[0] = g[1] = g[2] = 0.0;
+    double J[3];
+    const int b = a.gr_off[e * 6 + f];
+    if (b >= 0) {
+        sheet_current<AX, N2>(a, b, q, sg, um, upd, dt, J);
+        g[0] -= J[0]; g[1] -= J[1]; g[2] -= J[2];
+    }
+    if (nbr >= 0) {
+        const int bn = a.gr_off[nbr * 6 + (f ^ 1)];
+        if (bn >= 0) {
+            sheet_current<AX, N2>(a, bn, q, -sg, up, false, dt, J);
+            g[0] -= J[0]; g[1] -= J[1]; g[2] -= J[2];
+        }
+    }
+}
+
 // Flux at one face node (cem_maxwell_flux3d loops 1-2 + gs + PEC), scaled by
 // the lift factor L.  um/up: own/neighbour fields in HBM order (H then E).
-// Returns FH (lift into resH) and FE (lift into resE).
+// Returns FH (lift into resH) and FE (lift into resE).  g (or null): the
+// sheet term added to the gathered -n x H before the PEC rule.
 template <int AX>
 __device__ __forceinline__ void face_flux(double sg, int type, const double um[6],
                                           const double up[6], const double cf[4],
-                                          double L, double FH[3], double FE[3]) {
+                                          double L, double FH[3], double FE[3],
+                                          const double *g = nullptr) {
     double sE[3], sH[3];
     if (type >= 0) {
         // n x [E] = (-n x E-) + (-n' x E+), n' = -n
@@ -163,6 +249,7 @@ __device__ __forceinline__ void face_flux(double sg, int type, const double um[6
         double dH[3] = {up[0] - um[0], up[1] - um[1], up[2] - um[2]};
         ncross<AX>(sg, dE, sE);
         ncross<AX>(sg, dH, sH);
+        if (g) { sH[0] += g[0]; sH[1] += g[1]; sH[2] += g[2]; }
     } else {
         // no partner: gs leaves -n x E-, -n x H-; PEC doubles E, zeroes H
         double Em[3] = {-um[3], -um[4], -um[5]};
@@ -173,6 +260,7 @@ __device__ __forceinline__ void face_flux(double sg, int type, const double um[6
             sH[0] = 0.0; sH[1] = 0.0; sH[2] = 0.0;
         } else {
             ncross<AX>(sg, Hm, sH);
+            if (g) { sH[0] += g[0]; sH[1] += g[1]; sH[2] += g[2]; }
         }
     }
     double fu[3], fw[3];
@@ -348,6 +436,7 @@ stage_kernel(StageArgs a) {
     const int imask = (FEAT == 2 && a.inc_mask && valid) ? a.inc_mask[e] : 0;
     // stage time for userinc (phase 1); usersrc takes it in phase 2
     const double tinc = (FEAT == 2 && imask) ? *a.tdev + a.rkc * (a.dt_e ? a.dt_e[e] : a.dt) : 0.0;   // imask != 0: e valid
+    const double dt_stage = (FEAT == 2 && a.dt_e && valid) ? a.dt_e[e] : a.dt;   // the sheet's RK update
     __syncthreads();
 
     // ---- phase 1: role r computes the flux of x/y face r at node (i,j) ----
@@ -365,8 +454,14 @@ stage_kernel(StageArgs a) {
         const double sg = (role & 1) ? 1.0 : -1.0;
         if (FEAT == 2 && (imask >> role) & 1) add_incident(a, eo + own, tinc, um);
         if (FEAT == 2 && (imask >> (8 + role)) & 1) add_incident(a, (int64_t)nbr_xy * N3 + nbn_xy, tinc, nxy);
-        if (role < 2) face_flux<0>(sg, nbr_xy, um, nxy, cf, geo[3 + role], FH, FE);
-        else          face_flux<1>(sg, nbr_xy, um, nxy, cf, geo[3 + role], FH, FE);
+        double gsh[3];
+        const bool sheet = FEAT == 2 && a.gr_off;
+        if (sheet) {
+            if (role < 2) sheet_flux<0, N2>(a, e, role, nbr_xy, q, sg, um, nxy, MODE == MODE_RK, dt_stage, gsh);
+            else          sheet_flux<1, N2>(a, e, role, nbr_xy, q, sg, um, nxy, MODE == MODE_RK, dt_stage, gsh);
+        }
+        if (role < 2) face_flux<0>(sg, nbr_xy, um, nxy, cf, geo[3 + role], FH, FE, sheet ? gsh : nullptr);
+        else          face_flux<1>(sg, nbr_xy, um, nxy, cf, geo[3 + role], FH, FE, sheet ? gsh : nullptr);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             s_f[(role * 6 + c) * N2 + q] = FH[c];
@@ -381,7 +476,8 @@ stage_kernel(StageArgs a) {
             load_coef<N1, MAT>(a, e, zface, q, cz);
             if (FEAT == 2 && (imask >> zface) & 1) add_incident(a, eo + ownz, tinc, um);
             if (FEAT == 2 && (imask >> (8 + zface)) & 1) add_incident(a, (int64_t)nbr_z * N3 + nbn_z, tinc, nz);
-            face_flux<2>(half ? 1.0 : -1.0, nbr_z, um, nz, cz, geo[3 + zface], FH, FE);
+            if (sheet) sheet_flux<2, N2>(a, e, zface, nbr_z, q, half ? 1.0 : -1.0, um, nz, MODE == MODE_RK, dt_stage, gsh);
+            face_flux<2>(half ? 1.0 : -1.0, nbr_z, um, nz, cz, geo[3 + zface], FH, FE, sheet ? gsh : nullptr);
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 s_f[(zface * 6 + c) * N2 + q] = FH[c];
@@ -468,7 +564,11 @@ stage_kernel(StageArgs a) {
             for (int c = 0; c < 6; ++c) um[c] = s_u[c * N3 + own];
             if (FEAT == 2 && (imask >> zface) & 1) add_incident(a, eo + own, tinc, um);
             if (FEAT == 2 && (imask >> (8 + zface)) & 1) add_incident(a, (int64_t)nbr_z * N3 + nbn_z, tinc, nz);
-            face_flux<2>(half ? 1.0 : -1.0, nbr_z, um, nz, cfz, geo[3 + zface], FH, FE);
+            // both groups form this face's flux; group 0 advances the sheet
+            double gsh[3];
+            const bool sheet = FEAT == 2 && a.gr_off;
+            if (sheet) sheet_flux<2, N2>(a, e, zface, nbr_z, q, half ? 1.0 : -1.0, um, nz, MODE == MODE_RK && grp == 0, dt, gsh);
+            face_flux<2>(half ? 1.0 : -1.0, nbr_z, um, nz, cfz, geo[3 + zface], FH, FE, sheet ? gsh : nullptr);
 #pragma unroll
             for (int c = 0; c < 3; ++c) r[c] += grp ? FE[c] : FH[c];
         }

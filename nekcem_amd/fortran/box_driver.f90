@@ -163,6 +163,35 @@ program box_driver
     end if
     write(6, '(a,es12.4)') 'box_driver: PML entry points ok, max |B,D| =', maxval(abs(bd))
   end block
+  ! The graphene-sheet entry points from Fortran, with the arrays of the
+  ! .usr's /usergraph/ block: a Drude-only sheet (b_d = 1.5) on the x+ face
+  ! of element 1 starting from zero; after 5 steps its Drude slot (fjn(:,:,2))
+  ! has charged up, the critical-point slots 3..6 are still zero, and
+  ! ngraph = 0 removes the sheet.
+  block
+    integer(c_int32_t), allocatable :: graphindex(:)
+    real(c_double), allocatable :: graphparams(:,:), fjn(:,:,:)
+    integer :: i
+    allocate(graphindex(n2), graphparams(nxzfl,12), fjn(nxzfl,3,6))
+    graphparams = 0; fjn = 0
+    do i = 1, n2
+       graphindex(i) = n2*(2-1) + i          ! element 1, face 2 (x+)
+       graphparams(graphindex(i), 2) = 1.5d0
+    end do
+    call nkc_check(nkc_set_graphene(h, int(n2, c_int64_t), graphindex, graphparams, fjn, &
+         int(nxzfl, c_int64_t)), 'nkc_set_graphene')
+    call nkc_check(nkc_step(h, time, dt, 5_c_int64_t), 'nkc_step (graphene)')
+    call nkc_check(nkc_get_graphene(h, 0_c_int32_t, fjn, int(nxzfl, c_int64_t)), 'nkc_get_graphene')
+    if (.not. (maxval(abs(fjn(graphindex, :, 2))) > 0 .and. maxval(abs(fjn(graphindex, :, 2))) < 1d10) &
+         .or. maxval(abs(fjn(:, :, 3:6))) /= 0) then
+       write(6, '(a)') 'box_driver: graphene state not as expected'
+       error stop 1
+    end if
+    call nkc_check(nkc_set_graphene(h, 0_c_int64_t, graphindex, graphparams, fjn, int(nxzfl, c_int64_t)), &
+         'nkc_set_graphene (remove)')
+    write(6, '(a,es12.4)') 'box_driver: graphene entry points ok, max |fjn(:,:,2)| =', &
+         maxval(abs(fjn(graphindex, :, 2)))
+  end block
   call nkc_check(nkc_destroy(h), 'nkc_destroy')
   write(6, '(a)') 'box_driver: 3dboxper passed on the HIP path'
 

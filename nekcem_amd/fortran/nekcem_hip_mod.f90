@@ -36,6 +36,7 @@ module nekcem_hip
        nkc_residual, nkc_set_source, nkc_set_source_time, nkc_set_drude, nkc_get_drude, &
        nkc_set_lorentz, nkc_get_lorentz, nkc_set_materials, &
        nkc_set_pml, nkc_set_pml_fields, nkc_get_pml, nkc_set_incident, &
+       nkc_set_graphene, nkc_get_graphene, &
        nkc_adjoint_forward, nkc_adjoint_backward, nkc_adjoint_get_drude, &
        nkc_check_finite, nkc_sync, nkc_stream, nkc_info, &
        nkc_set_graph, nkc_last_error, nkc_check, nkc_error_message
@@ -177,6 +178,27 @@ module nekcem_hip
        real(c_double), intent(in) :: k(3), hamp(3), eamp(3)
        real(c_double), value :: omega
      end function nkc_set_incident
+
+     ! graphene sheet current of userfsrc on the face points graphindex
+     ! (1-based face numbering), graphparams(ld,12), fjn(ld,3,6) as the .usr
+     ! common block /usergraph/ holds them; ngraph = 0 removes it
+     integer(c_int32_t) function nkc_set_graphene(h, ngraph, graphindex, graphparams, fjn, ld) &
+          bind(C, name='nkc_set_graphene')
+       import :: c_int32_t, c_int64_t, c_ptr, c_double
+       type(c_ptr), value :: h
+       integer(c_int64_t), value :: ngraph, ld
+       integer(c_int32_t), intent(in) :: graphindex(*)
+       real(c_double), intent(in) :: graphparams(*), fjn(*)
+     end function nkc_set_graphene
+
+     ! sheet state (which = 0) or RK register (1) into fjn(ld,3,2:6)
+     integer(c_int32_t) function nkc_get_graphene(h, which, fjn, ld) bind(C, name='nkc_get_graphene')
+       import :: c_int32_t, c_int64_t, c_ptr, c_double
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: which
+       real(c_double), intent(inout) :: fjn(*)
+       integer(c_int64_t), value :: ld
+     end function nkc_get_graphene
 
      ! per-node permittivity and permeability, npts each
      integer(c_int32_t) function nkc_set_materials(h, eps, mu) bind(C, name='nkc_set_materials')

@@ -360,6 +360,38 @@ class MaxwellSolver:
         _lib.check(self._L.nkc_set_incident(self._h, len(f), _i(f) if len(f) else None, _d(kk), float(omega),
                                             _d(hh), _d(ee)))
 
+    def set_graphene(self, gindex, params, G=None):
+        """Graphene sheet current (userfsrc + cem_3d_graphene_current) on
+        the face points gindex (0-based, NekCEM's face-point numbering
+        e 6 n2 + (f-1) n2 + i, f in the reference's face order), with the
+        twelve sheet parameters per point (params (n, 12) or (12,)) and the
+        initial state G (15, n): fjn slots 2..6 x 3 components, row
+        3 (slot - 2) + component (None: zero).  gindex = [] removes it."""
+        gindex = np.asarray(gindex, dtype=np.int64)
+        n = len(gindex)
+        if n == 0:
+            _lib.check(self._L.nkc_set_graphene(self._h, 0, None, None, None, 0))
+            self._gindex = None
+            return
+        ld = self.mesh.nelt * 6 * (self.mesh.N + 1) ** 2
+        par = np.zeros((12, ld))
+        par[:, gindex] = np.broadcast_to(np.asarray(params, dtype=np.float64), (n, 12)).T
+        fjn = np.zeros((6, 3, ld))
+        if G is not None:
+            fjn[1:, :, gindex] = np.asarray(G, dtype=np.float64).reshape(5, 3, n)
+        gi = np.ascontiguousarray(gindex + 1, dtype=np.int32)
+        _lib.check(self._L.nkc_set_graphene(self._h, n, _i(gi), _d(par), _d(fjn), ld))
+        self._gindex = gindex
+
+    def get_graphene(self, which="state"):
+        """The sheet state (or its RK register, which="register") as
+        (15, n) in the order of set_graphene's gindex."""
+        w = {"state": 0, "register": 1}[which]
+        ld = self.mesh.nelt * 6 * (self.mesh.N + 1) ** 2
+        fjn = np.zeros((6, 3, ld))
+        _lib.check(self._L.nkc_get_graphene(self._h, w, _d(fjn), ld))
+        return fjn[1:, :, self._gindex].reshape(15, -1)
+
     def set_drude(self, dindex, a, b, J=None, kJ=None):
         """Drude ADE on volume nodes dindex (0-based): a = gamma, b =
         omega_p^2 per listed node; J, kJ (3, ndrude) or None (zero)."""

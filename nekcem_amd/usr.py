@@ -217,3 +217,114 @@ def dipole_amplitude(x, y, z, bm1, width=0.1):
     sin(-omega t) bm1 with i0 norm = 1: the bm1-weighted amplitude for
     set_source(..., direction=(0, 0, 1), omega=-2.0)."""
     return -np.exp(-0.5 * (x / width) ** 2 - 0.5 * (y / width) ** 2 - 0.5 * (z / width) ** 2) * bm1
+
+
+# ---------------------------------------------------------------------------
+# 3dgraphene (tests/3dgraphene/3dgraphene.usr): two superimposed normally
+# incident plane waves striking a graphene sheet on y = 0, whose surface
+# current is a Drude term plus two critical-point terms.
+# ---------------------------------------------------------------------------
+# userchk tolerances (3dgraphene.usr:483-497), order Hx Hy Hz Ex Ey Ez
+GRAPHENE_L2TOL = (5e-4, 1e-14, 5e-4, 5e-4, 1e-14, 5e-4)
+GRAPHENE_LINFTOL = (5e-3, 5e-12, 5e-3, 5e-3, 5e-12, 5e-3)
+# sheet parameters of uservp (3dgraphene.usr:390-401): a_d, b_d, b_cp1,
+# a_211, a_221, b_11, b_21, b_cp2, a_212, a_222, b_12, b_22
+GRAPHENE_SHEET = (0.0, 1.499, -2.599e-03, 4.632e+05, 1.090e+03, -1.391, -3.125e+02, -1.049e-03, 4.271e+05,
+                  2.742e+02, -7.769e-02, 4.268e+02)
+
+
+def graphene_params(sheet=GRAPHENE_SHEET):
+    """The /userparam/ block of uservp (3dgraphene.usr:385-418): omega,
+    eps1, eps2, mu1, mu2, the complex sheet conductivity at omega and the TE /
+    TM reflection and transmission coefficients (sheet: the twelve
+    parameters)."""
+    omega, eps1, eps2, mu1, mu2 = 5.0, 1.0, 1.0, 1.0, 1.0
+    a_d, b_d, b_cp1, a_211, a_221, b_11, b_21, b_cp2, a_212, a_222, b_12, b_22 = sheet
+    w = omega
+    s_d = b_d / (a_d - 1j * w)
+    s_1 = (1j / w) * ((a_211 * b_11 + 1j * w * b_21) / (w ** 2 - a_211 + 1j * w * a_221) + b_11) - b_cp1
+    s_2 = (1j / w) * ((a_212 * b_12 + 1j * w * b_22) / (w ** 2 - a_212 + 1j * w * a_222) + b_12) - b_cp2
+    sg = s_d + s_1 + s_2
+    z1, z2 = np.sqrt(mu1 / eps1), np.sqrt(mu2 / eps2)
+    den = z1 + z2 + sg * z1 * z2
+    return dict(omega=omega, eps1=eps1, eps2=eps2, mu1=mu1, mu2=mu2, sigma=sg,
+                reflte=(z1 - z2 + sg * z1 * z2) / den, trante=2 * z1 / den,
+                refltm=(z2 - z1 - z1 * z2 * sg) / den, trantm=2 * z2 / den)
+
+
+def sheet_faces(lo_y, hi_y, elements=None):
+    """Faces (element, Ed's face) whose every point has |y| <= 1e-8, the
+    tests of uservp (3dgraphene.usr:420-470): on an axis-aligned box only a
+    y face can, face 1 (y = lo) or 3 (y = hi).  elements: the elements to
+    look at (default all)."""
+    faces = []
+    for e in (range(len(lo_y)) if elements is None else elements):
+        if abs(lo_y[e]) <= 1e-8:
+            faces.append((int(e), 1))
+        if abs(hi_y[e]) <= 1e-8:
+            faces.append((int(e), 3))
+    return faces
+
+
+def graphene_incident(upper, lo_y, hi_y):
+    """userinc (3dgraphene.usr:7-47) on the upper elements' sheet faces:
+    (faces, k, omega, hamp, eamp) for MaxwellSolver.set_incident, with
+    uinc = cos(-ky y - omega t): hz += u, ex += eta u, ez += u, hx -= u / eta
+    (the upper region's eta and ky; eps = mu = 1 on both sides)."""
+    p = graphene_params()
+    faces = sheet_faces(lo_y, hi_y, np.nonzero(upper)[0])
+    eta = np.sqrt(p["mu1"] / p["eps1"])
+    ky = p["omega"] * np.sqrt(p["mu1"] * p["eps1"])
+    return faces, (0.0, -ky, 0.0), p["omega"], (-1.0 / eta, 0.0, 1.0), (eta, 0.0, 1.0)
+
+
+def graphene_initial_state(sheet=GRAPHENE_SHEET):
+    """userini's sheet state (3dgraphene.usr:97-147): fjn slots 2..6 of
+    every sheet point, (5, 3), from half the tangential complex E field at
+    the interface at t = 0."""
+    p = graphene_params(sheet)
+    a_d, b_d, b_cp1, a_211, a_221, b_11, b_21, b_cp2, a_212, a_222, b_12, b_22 = sheet
+    w = p["omega"]
+    z1 = np.sqrt(p["mu1"] / p["eps1"])
+    enpar = np.array([0.5 * z1 * (1.0 - p["reflte"]), 0.0, 0.5 * (1.0 + p["refltm"])])
+    out = np.zeros((5, 3))
+    out[0] = (b_d / (a_d - 1j * w) * enpar).real
+    f4 = (a_211 * b_11 + 1j * w * b_21) / (w ** 2 - a_211 + 1j * w * a_221)
+    out[2] = (f4 * enpar).real
+    out[1] = ((1j / w) * (f4 + b_11) * enpar).real
+    f6 = (a_212 * b_12 + 1j * w * b_22) / (w ** 2 - a_212 + 1j * w * a_222)
+    out[4] = (f6 * enpar).real
+    out[3] = ((1j / w) * (f6 + b_12) * enpar).real
+    return out
+
+
+def graphene_solution(t, y, eps, mu, upper_nodes, pml_nodes, inner, outer, order, referr, sheet=GRAPHENE_SHEET):
+    """usersol (3dgraphene.usr:150-221): (H, E), (3, npts) each, the complex
+    reflected / transmitted waves' real parts, damped inside the PML; y,
+    eps, mu per node, upper_nodes / pml_nodes per node, inner / outer the
+    PML planes (x-, x+, y-, y+, z-, z+)."""
+    p = graphene_params(sheet)
+    om = p["omega"]
+    eta = np.sqrt(mu / eps)
+    ky = om * np.sqrt(eps * mu)
+    H = np.zeros((3, len(y)))
+    E = np.zeros((3, len(y)))
+    d = outer[3] - inner[3]
+    smax = -(order + 1) * np.log(referr) / (2 * eta * d)
+    fac = np.where(pml_nodes, smax * d / (order + 1) * np.abs((y - inner[3]) / d) ** (order + 1), 0.0)
+    uu = np.exp(1j * (ky * y - om * t) - eta * fac)
+    up = upper_nodes
+    H[2] = np.where(up, (p["reflte"] * uu).real, H[2])
+    E[0] = np.where(up, -(p["reflte"] * eta * uu).real, E[0])
+    E[2] = np.where(up, (p["refltm"] * uu).real, E[2])
+    H[0] = np.where(up, (p["refltm"] * uu / eta).real, H[0])
+    d = inner[2] - outer[2]
+    smax = -(order + 1) * np.log(referr) / (2 * eta * d)
+    fac = np.where(pml_nodes, smax * d / (order + 1) * np.abs((inner[2] - y) / d) ** (order + 1), 0.0)
+    uu = np.exp(1j * (-ky * y - om * t) - eta * fac)
+    lo = ~upper_nodes
+    H[2] = np.where(lo, (p["trante"] * uu).real, H[2])
+    E[0] = np.where(lo, (p["trante"] * eta * uu).real, E[0])
+    E[2] = np.where(lo, (p["trantm"] * uu).real, E[2])
+    H[0] = np.where(lo, -(p["trantm"] * uu / eta).real, H[0])
+    return H, E

@@ -486,6 +486,14 @@ typedef struct orc_mesh {
     const int8_t *inc_on;                 /* nxzfl or NULL */
     const double *xm1, *ym1, *zm1;        /* npts */
     double inc_k[3], inc_omega, inc_h[3], inc_e[3];
+    /* graphene surface current (cem_3d_graphene_current,
+     * cem_maxwell.F:2827-2932) at the face points gindex[0..ngraph) (0-based
+     * nxzfl numbers); per listed point the twelve sheet parameters of
+     * tests/3dgraphene/3dgraphene.usr:390-401 (a_d, b_d, b_cp1, a_211, a_221,
+     * b_11, b_21, b_cp2, a_212, a_222, b_12, b_22) at gpar[12 l ..] */
+    int64_t ngraph;
+    const int32_t *gindex;
+    const double *gpar;
 } orc_mesh;
 
 /* resE = W curl H; resH = -W curl E (cem_maxwell.F:574-582 + 1428-1497) */
@@ -525,7 +533,8 @@ static void wght_curl(const orc_mesh *m, const double *u1, const double *u2, con
 
 static void maxwell_op_body(const orc_mesh *m, double rktime, const double *hn, const double *en,
                             double *reshn, double *resen, const double *J, double *resJ,
-                            const double *L, double *resL, const double *P, double *resP);
+                            const double *L, double *resL, const double *P, double *resP,
+                            const double *G, double *resG);
 
 /* cem_maxwell's 3-D branch alone (cem_maxwell.F:574-582 + chsign):
  * resE = W curl H, resH = -W curl E (maxwell_wght_curl :1428-1497). */
@@ -544,7 +553,7 @@ void orc_curl(const orc_mesh *m, const double *hn, const double *en, double *res
 void orc_maxwell_op(const orc_mesh *m, double rktime, const double *hn, const double *en,
                     double *reshn, double *resen, const double *J, double *resJ)
 {
-    maxwell_op_body(m, rktime, hn, en, reshn, resen, J, resJ, NULL, NULL, NULL, NULL);
+    maxwell_op_body(m, rktime, hn, en, reshn, resen, J, resJ, NULL, NULL, NULL, NULL, NULL, NULL);
 }
 
 /* The same with the Lorentz state L = (J, P) (6 blocks of nlorentz: J then
@@ -553,7 +562,7 @@ void orc_maxwell_op_ade(const orc_mesh *m, double rktime, const double *hn, cons
                         double *reshn, double *resen, const double *J, double *resJ,
                         const double *L, double *resL)
 {
-    maxwell_op_body(m, rktime, hn, en, reshn, resen, J, resJ, L, resL, NULL, NULL);
+    maxwell_op_body(m, rktime, hn, en, reshn, resen, J, resJ, L, resL, NULL, NULL, NULL, NULL);
 }
 
 /* The same with the PML state P = (pmlbn, pmldn): 6 blocks of npts, and
@@ -562,12 +571,44 @@ void orc_maxwell_op_full(const orc_mesh *m, double rktime, const double *hn, con
                          double *reshn, double *resen, const double *J, double *resJ,
                          const double *L, double *resL, const double *P, double *resP)
 {
-    maxwell_op_body(m, rktime, hn, en, reshn, resen, J, resJ, L, resL, P, resP);
+    maxwell_op_body(m, rktime, hn, en, reshn, resen, J, resJ, L, resL, P, resP, NULL, NULL);
+}
+
+/* The same with the graphene state G (fjn slots 2..6 of the .usr: 15 blocks
+ * of ngraph, block 3 (slot - 2) + component) and its residual resG, or NULL. */
+void orc_maxwell_op_all(const orc_mesh *m, double rktime, const double *hn, const double *en,
+                        double *reshn, double *resen, const double *J, double *resJ,
+                        const double *L, double *resL, const double *P, double *resP,
+                        const double *G, double *resG)
+{
+    maxwell_op_body(m, rktime, hn, en, reshn, resen, J, resJ, L, resL, P, resP, G, resG);
+}
+
+/* Face trace of H and E at face point i (restrict_to_face), with the
+ * userinc plane wave added where it is on (cem_maxwell.F:497-498: after the
+ * restriction, before the flux). */
+static void face_trace(const orc_mesh *m, int64_t i, double rktime, const double *hn, const double *en,
+                       double fh[3], double fe[3])
+{
+    int64_t npts = m->npts, v = m->cemface[i];
+    for (int c = 0; c < 3; ++c) {
+        fh[c] = hn[c * npts + v];
+        fe[c] = en[c * npts + v];
+    }
+    if (m->inc_on && m->inc_on[i]) {
+        double u = cos(m->inc_k[0] * m->xm1[v] + m->inc_k[1] * m->ym1[v] + m->inc_k[2] * m->zm1[v] -
+                       m->inc_omega * rktime);
+        for (int c = 0; c < 3; ++c) {
+            fh[c] = fh[c] + m->inc_h[c] * u;
+            fe[c] = fe[c] + m->inc_e[c] * u;
+        }
+    }
 }
 
 static void maxwell_op_body(const orc_mesh *m, double rktime, const double *hn, const double *en,
                             double *reshn, double *resen, const double *J, double *resJ,
-                            const double *L, double *resL, const double *P, double *resP)
+                            const double *L, double *resL, const double *P, double *resP,
+                            const double *G, double *resG)
 {
     int64_t npts = m->npts, k = m->nxzfl;
     const double *H1 = hn, *H2 = hn + npts, *H3 = hn + 2 * npts;
@@ -585,21 +626,10 @@ static void maxwell_op_body(const orc_mesh *m, double rktime, const double *hn, 
     double *s = m->srflx;
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < k; ++i) {
-        int64_t v = m->cemface[i];
-        double fh1 = H1[v], fh2 = H2[v], fh3 = H3[v];
-        double fe1 = E1[v], fe2 = E2[v], fe3 = E3[v];
-        if (m->inc_on && m->inc_on[i]) {
-            /* userinc (cem_maxwell.F:497-498): added to fhn/fen after the
-             * restriction, before the flux */
-            double u = cos(m->inc_k[0] * m->xm1[v] + m->inc_k[1] * m->ym1[v] + m->inc_k[2] * m->zm1[v] -
-                           m->inc_omega * rktime);
-            fh1 = fh1 + m->inc_h[0] * u;
-            fh2 = fh2 + m->inc_h[1] * u;
-            fh3 = fh3 + m->inc_h[2] * u;
-            fe1 = fe1 + m->inc_e[0] * u;
-            fe2 = fe2 + m->inc_e[1] * u;
-            fe3 = fe3 + m->inc_e[2] * u;
-        }
+        double fh[3], fe[3];
+        face_trace(m, i, rktime, hn, en, fh, fe);
+        double fh1 = fh[0], fh2 = fh[1], fh3 = fh[2];
+        double fe1 = fe[0], fe2 = fe[1], fe3 = fe[2];
         double nx = m->unx[i], ny = m->uny[i], nz = m->unz[i];
         s[0 * k + i] = -ny * fe3 + nz * fe2;
         s[1 * k + i] = -nz * fe1 + nx * fe3;
@@ -607,6 +637,42 @@ static void maxwell_op_body(const orc_mesh *m, double rktime, const double *hn, 
         s[3 * k + i] = -ny * fh3 + nz * fh2;
         s[4 * k + i] = -nz * fh1 + nx * fh3;
         s[5 * k + i] = -nx * fh2 + ny * fh1;
+    }
+    /* userfsrc of tests/3dgraphene/3dgraphene.usr:235-262: the sheet
+     * current (cem_3d_graphene_current, cem_maxwell.F:2827-2932) from the
+     * stage's face traces and the sheet state, subtracted from -n x H on each
+     * listed point before the face sum.  The state's RK update is the
+     * caller's (same coefficients, same stage). */
+    if (m->ngraph > 0 && G && resG) {
+        int64_t ng = m->ngraph;
+#pragma omp parallel for schedule(static)
+        for (int64_t l = 0; l < ng; ++l) {
+            int64_t j = m->gindex[l], v = m->cemface[j];
+            const double *p = m->gpar + 12 * l;
+            double fh[3], fe[3];
+            face_trace(m, j, rktime, hn, en, fh, fe);
+            double nx = m->unx[j], ny = m->uny[j], nz = m->unz[j];
+            double nH[3] = {-ny * fh[2] + nz * fh[1], nx * fh[2] - nz * fh[0], -nx * fh[1] + ny * fh[0]};
+            double ndotE = nx * fe[0] + ny * fe[1] + nz * fe[2];
+            double nEn[3] = {fe[0] - nx * ndotE, fe[1] - ny * ndotE, fe[2] - nz * ndotE};
+            double Yfac = 0.5 / m->Y0[j];
+            double yc = sqrt(m->eps[v] / m->mu[v]);
+            double cpfac = p[2] + p[7];
+            double jnfac = 1.0 - cpfac * Yfac;
+            for (int c = 0; c < 3; ++c) {
+                double A0 = G[(0 + c) * ng + l], A1 = G[(3 + c) * ng + l], A2 = G[(6 + c) * ng + l];
+                double A3 = G[(9 + c) * ng + l], A4 = G[(12 + c) * ng + l];
+                double tmp = Yfac * (nH[c] + yc * nEn[c]);
+                double J1 = (A0 + A1 + A3 - cpfac * tmp) / jnfac;
+                double f = tmp - Yfac * J1;
+                resG[(0 + c) * ng + l] = -p[0] * A0 + p[1] * f;
+                resG[(3 + c) * ng + l] = A2 + p[5] * f;
+                resG[(6 + c) * ng + l] = -p[3] * A1 - p[4] * A2 + p[6] * f;
+                resG[(9 + c) * ng + l] = A4 + p[10] * f;
+                resG[(12 + c) * ng + l] = -p[8] * A3 - p[9] * A4 + p[11] * f;
+                s[(3 + c) * k + j] = s[(3 + c) * k + j] - J1;
+            }
+        }
     }
     /* gs_op_fields(gsh_face, srflx, nxzfl, 6, add) */
     orc_gs_add(m->partner, s, k, 6, k);
@@ -797,21 +863,39 @@ static void rk4_upd(double *h, double *kh, const double *resh, double cb, double
 /* nsteps of cem_maxwell_op_rk with the Lorentz state L/kL/resL (6*nlorentz)
  * and the PML state P/kP/resP (6*npts; rk_maxwell_ab's PML updates,
  * cem_maxwell.F:1942-1954) as well (NULL: none); otherwise as orc_rk_run. */
+void orc_rk_run_all(const orc_mesh *m, double *hn, double *en, double *khn, double *ken,
+                    double *reshn, double *resen, double *J, double *kJ, double *resJ,
+                    double *L, double *kL, double *resL, double *P, double *kP, double *resP,
+                    double *G, double *kG, double *resG, double dt, int64_t nsteps, double *time);
+
 void orc_rk_run_full(const orc_mesh *m, double *hn, double *en, double *khn, double *ken,
                      double *reshn, double *resen, double *J, double *kJ, double *resJ,
                      double *L, double *kL, double *resL, double *P, double *kP, double *resP,
                      double dt, int64_t nsteps, double *time)
 {
+    orc_rk_run_all(m, hn, en, khn, ken, reshn, resen, J, kJ, resJ, L, kL, resL, P, kP, resP, NULL, NULL, NULL,
+                   dt, nsteps, time);
+}
+
+/* The same with the graphene state G/kG/resG (15*ngraph; the four rk4_upd
+ * calls of cem_3d_graphene_current, cem_maxwell.F:2915-2929, at the stage's
+ * coefficients) as well. */
+void orc_rk_run_all(const orc_mesh *m, double *hn, double *en, double *khn, double *ken,
+                    double *reshn, double *resen, double *J, double *kJ, double *resJ,
+                    double *L, double *kL, double *resL, double *P, double *kP, double *resP,
+                    double *G, double *kG, double *resG, double dt, int64_t nsteps, double *time)
+{
     int64_t n3 = 3 * m->npts;
     for (int64_t st = 0; st < nsteps; ++st) {
         for (int r = 0; r < 5; ++r) {
             double rktime = *time + dt * RK4C[r];
-            maxwell_op_body(m, rktime, hn, en, reshn, resen, J, resJ, L, resL, P, resP);
+            maxwell_op_body(m, rktime, hn, en, reshn, resen, J, resJ, L, resL, P, resP, G, resG);
             rk4_upd(hn, khn, reshn, RK4B[r], RK4A[r], dt, n3);
             rk4_upd(en, ken, resen, RK4B[r], RK4A[r], dt, n3);
             if (m->ndrude > 0 && J) rk4_upd(J, kJ, resJ, RK4B[r], RK4A[r], dt, 3 * m->ndrude);
             if (m->nlorentz > 0 && L) rk4_upd(L, kL, resL, RK4B[r], RK4A[r], dt, 6 * m->nlorentz);
             if (m->npml > 0 && P) rk4_upd(P, kP, resP, RK4B[r], RK4A[r], dt, 6 * m->npts);
+            if (m->ngraph > 0 && G) rk4_upd(G, kG, resG, RK4B[r], RK4A[r], dt, 15 * m->ngraph);
         }
         *time = *time + dt;
     }

@@ -74,6 +74,8 @@ def lib():
         L.orc_maxwell_op_full.argtypes = [C.c_void_p, C.c_double] + [_dp] * 10
         L.orc_curl.argtypes = [C.c_void_p] + [_dp] * 4
         L.orc_rk_run_full.argtypes = [C.c_void_p] + [_dp] * 15 + [C.c_double, C.c_int64, _dp]
+        L.orc_maxwell_op_all.argtypes = [C.c_void_p, C.c_double] + [_dp] * 12
+        L.orc_rk_run_all.argtypes = [C.c_void_p] + [_dp] * 18 + [C.c_double, C.c_int64, _dp]
         L.orc_rk_coeffs.argtypes = [_dp, _dp, _dp]
         L.orc_cem_error.argtypes = [_dp, _dp, _dp, C.c_int64, _dp, _dp]
         L.orc_maxwell_vjp.argtypes = [C.c_void_p, C.c_double, _dp, _dp, _bp, _dp, _dp, _dp, _dp,
@@ -118,6 +120,7 @@ class _Mesh(C.Structure):
         ("inc_on", _bp), ("xm1", _dp), ("ym1", _dp), ("zm1", _dp),
         ("inc_k", C.c_double * 3), ("inc_omega", C.c_double), ("inc_h", C.c_double * 3),
         ("inc_e", C.c_double * 3),
+        ("ngraph", C.c_int64), ("gindex", _ip), ("gpar", _dp),
     ]
 
 
@@ -241,6 +244,7 @@ class BoxCase:
         self.pmlsigma = None
         self.pmltag = np.zeros(nelt, dtype=np.int64)
         self.inc_on = None
+        self.gindex = None
         self.inc = None
 
     # -- setup pieces ---------------------------------------------------
@@ -404,6 +408,15 @@ class BoxCase:
         self.inc = (tuple(float(x) for x in k), float(omega), tuple(float(x) for x in h_amp),
                     tuple(float(x) for x in e_amp))
 
+    def set_graphene(self, gindex, params):
+        """Graphene sheet current (cem_3d_graphene_current) on the face
+        points gindex (0-based nxzfl numbers) with the twelve sheet
+        parameters per point (params: (n, 12) or (12,) for all).  Its state
+        (fjn slots 2..6) is G: (15, n), block 3 (slot - 2) + component."""
+        self.gindex = np.ascontiguousarray(gindex, dtype=np.int32)
+        n = len(self.gindex)
+        self.gpar = np.ascontiguousarray(np.broadcast_to(np.asarray(params, dtype=np.float64), (n, 12)))
+
     def set_drude(self, dindex, a, b):
         """Drude ADE over volume nodes dindex with per-node a (gamma) and
         b (omega_p^2)."""
@@ -468,10 +481,14 @@ class BoxCase:
             m.inc_omega = om
             m.inc_h = (C.c_double * 3)(*hh)
             m.inc_e = (C.c_double * 3)(*ee)
+        if self.gindex is not None and len(self.gindex):
+            m.ngraph, m.gindex, m.gpar = len(self.gindex), _p(self.gindex), _p(self.gpar)
+        else:
+            m.ngraph = 0
         self._keep = m
         return m
 
-    def residual(self, H, E, t=0.0, J=None, L=None, P=None):
+    def residual(self, H, E, t=0.0, J=None, L=None, P=None, G=None):
         """(resH, resE[, resJ][, resL][, resP]) = cem_maxwell_op(H, E) incl.
         inverse mass; J the Drude current (3*ndrude), L the Lorentz state (J
         then P, 6*nlorentz), P the PML state (pmlbn, pmldn: (6, npts))."""
@@ -490,8 +507,12 @@ class BoxCase:
         if P is not None:
             P = np.ascontiguousarray(P, dtype=np.float64).reshape(6, self.npts)
             rP = np.zeros_like(P)
-        lib().orc_maxwell_op_full(C.byref(m), t, _p(H), _p(E), _p(rH), _p(rE), _p(J), _p(rJ), _p(L), _p(rL),
-                                  _p(P), _p(rP))
+        rG = None
+        if G is not None:
+            G = np.ascontiguousarray(G, dtype=np.float64).reshape(15, -1)
+            rG = np.zeros_like(G)
+        lib().orc_maxwell_op_all(C.byref(m), t, _p(H), _p(E), _p(rH), _p(rE), _p(J), _p(rJ), _p(L), _p(rL),
+                                 _p(P), _p(rP), _p(G), _p(rG))
         out = (rH, rE)
         if J is not None:
             out += (rJ,)
@@ -499,6 +520,8 @@ class BoxCase:
             out += (rL,)
         if P is not None:
             out += (rP,)
+        if G is not None:
+            out += (rG,)
         return out
 
     def curl(self, H, E):
@@ -511,7 +534,8 @@ class BoxCase:
         lib().orc_curl(C.byref(m), _p(H), _p(E), _p(rH), _p(rE))
         return rH, rE
 
-    def run(self, H, E, dt, nsteps, t=0.0, kH=None, kE=None, J=None, kJ=None, L=None, kL=None, P=None, kP=None):
+    def run(self, H, E, dt, nsteps, t=0.0, kH=None, kE=None, J=None, kJ=None, L=None, kL=None, P=None, kP=None,
+            G=None, kG=None):
         """nsteps of cem_maxwell_op_rk from (H, E) at time t.  Returns a dict
         with H, E, kH, kE, time (and J, kJ with Drude, L, kL with Lorentz,
         P, kP with a PML)."""
@@ -537,11 +561,17 @@ class BoxCase:
             P = np.zeros((6, self.npts)) if P is None else np.array(P, dtype=np.float64).reshape(6, self.npts)
             kP = np.zeros((6, self.npts)) if kP is None else np.array(kP, dtype=np.float64).reshape(6, self.npts)
             rP = np.zeros((6, self.npts))
+        rG = None
+        if self.gindex is not None and len(self.gindex):
+            ng = len(self.gindex)
+            G = np.zeros((15, ng)) if G is None else np.array(G, dtype=np.float64).reshape(15, ng)
+            kG = np.zeros((15, ng)) if kG is None else np.array(kG, dtype=np.float64).reshape(15, ng)
+            rG = np.zeros((15, ng))
         tt = np.array([t], dtype=np.float64)
         m = self._mesh()
-        lib().orc_rk_run_full(C.byref(m), _p(H), _p(E), _p(kH), _p(kE), _p(rH), _p(rE),
-                              _p(J), _p(kJ), _p(rJ), _p(L), _p(kL), _p(rL), _p(P), _p(kP), _p(rP),
-                              dt, nsteps, _p(tt))
+        lib().orc_rk_run_all(C.byref(m), _p(H), _p(E), _p(kH), _p(kE), _p(rH), _p(rE),
+                             _p(J), _p(kJ), _p(rJ), _p(L), _p(kL), _p(rL), _p(P), _p(kP), _p(rP),
+                             _p(G), _p(kG), _p(rG), dt, nsteps, _p(tt))
         out = dict(H=H, E=E, kH=kH, kE=kE, time=float(tt[0]))
         if J is not None:
             out.update(J=J, kJ=kJ)
@@ -549,6 +579,8 @@ class BoxCase:
             out.update(L=L, kL=kL)
         if P is not None:
             out.update(P=P, kP=kP)
+        if G is not None:
+            out.update(G=G, kG=kG)
         return out
 
     def vjp(self, H, E, kH, kE, t=0.0, J=None, kJ=None):

@@ -28,3 +28,4 @@ def test_fortran_box_driver_3dboxper():
     assert r.returncode == 0
     assert "3dboxper passed on the HIP path" in r.stdout
     assert "PML entry points ok" in r.stdout
+    assert "graphene entry points ok" in r.stdout

@@ -635,3 +635,103 @@ def test_drude_rk_adjoint_gradient_matches_central_differences(oracle):
             fd = (loss(gam, wp, eps + h * d) - loss(gam, wp, eps - h * d)) / (2 * h)
             an = np.dot(ge, d)
         assert abs(fd - an) <= 1e-6 * abs(fd), which
+
+
+# A sheet whose critical-point and instantaneous terms carry most of the
+# conductivity at omega = 5 (|sigma_cp| ~ 1 against the Drude term's 0.2),
+# so that a dropped or mis-signed critical-point term misses the tolerance
+# (with the reference sheet they are 1e-3 of sigma).
+STRONG_SHEET = (0.5, 1.0, -0.2, 30.0, 2.0, 1.0, 5.0, -0.1, 50.0, 4.0, -0.5, 3.0)
+
+
+def _graphene_oracle(oracle, sheet, run_sheet=None, n1=9):
+    """tests/3dgraphene: 4 x 12 x 4 elements on [-2.5, 2.5] x [-5, 5] x
+    [-2.5, 2.5] (the .box rescaled by usrdat2), periodic in x and z, PML on
+    y (thick 2, order 3, referr 1e-10), the incident wave on the upper
+    elements' y = 0 faces and the sheet on both sides of y = 0.  run_sheet:
+    parameters the run uses (default sheet; the solution always uses
+    sheet)."""
+    from nekcem_amd import usr
+    c = oracle.BoxCase(oracle.BoxSpec(nel=(4, 12, 4), N=n1 - 1, lo=(-2.5, -5.0, -2.5), hi=(2.5, 5.0, 2.5),
+                                      bc=("P", "PML", "P")))
+    ey = c.exyz[1]
+    lo_y, hi_y = -5.0 + 10.0 * ey / 12, -5.0 + 10.0 * (ey + 1) / 12
+    upper = usr.probe_upper(lo_y, hi_y, n1)
+    c.set_pml(2, 3.0, 1e-10)
+    n2 = n1 * n1
+    faces, k, om, ha, ea = usr.graphene_incident(upper, lo_y, hi_y)
+    assert len(faces) == 16
+    on = np.zeros(c.nxzfl, dtype=np.int8)
+    for e, f in faces:
+        on[(e * 6 + f - 1) * n2:(e * 6 + f) * n2] = 1
+    c.set_incident(on, k, om, ha, ea)
+    sheet_faces = usr.sheet_faces(lo_y, hi_y)
+    assert len(sheet_faces) == 32    # both sides of y = 0
+    gidx = np.concatenate([np.arange((e * 6 + f - 1) * n2, (e * 6 + f) * n2) for e, f in sheet_faces])
+    c.set_graphene(gidx, sheet if run_sheet is None else run_sheet)
+    G0 = np.repeat(usr.graphene_initial_state(sheet).reshape(15, 1), len(gidx), axis=1)
+    one = np.ones(c.npts)
+    upn, pmn = np.repeat(upper, n1 ** 3), np.repeat(c.pmltag != 0, n1 ** 3)
+
+    def sol(t):
+        return usr.graphene_solution(t, c.ym1, one, one, upn, pmn, c.pmlinner, c.pmlouter, 3.0, 1e-10, sheet)
+    return c, sol, G0
+
+
+def _graphene_errors(c, sol, G0, nsteps, every=50):
+    """Run from usersol / userini and yield the cem_error (l2, linf) of the
+    six fields every `every` steps."""
+    H, E = sol(0.0)
+    P, kP, G, kG, kH, kE, t = np.concatenate([H, E]), None, G0, None, None, None, 0.0
+    for _ in range(nsteps // every):
+        out = c.run(H, E, 0.005, every, t=t, kH=kH, kE=kE, P=P, kP=kP, G=G, kG=kG)
+        H, E, kH, kE, P, kP, t = out["H"], out["E"], out["kH"], out["kE"], out["P"], out["kP"], out["time"]
+        G, kG = out["G"], out["kG"]
+        Hs, Es = sol(t)
+        err = [c.cem_error(H[i], Hs[i]) for i in range(3)] + [c.cem_error(E[i], Es[i]) for i in range(3)]
+        yield t, np.array([a for a, _ in err]), np.array([b for _, b in err])
+
+
+@pytest.mark.parametrize("strong", [False, True], ids=["reference_sheet", "strong_cp_sheet"])
+def test_3dgraphene_reference_tolerance(oracle, strong):
+    """The reference's 3dgraphene (tests/3dgraphene, N = 8, dt = 0.005, 1000
+    steps) meets its userchk tolerances every 50 steps: the exact reflected /
+    transmitted waves of a sheet of conductivity sigma(omega) pin the sheet
+    current (cem_3d_graphene_current) entering the flux through userfsrc,
+    with the incident wave and the PML of the 3ddielectric set-up.  The
+    strong critical-point sheet (200 steps) pins the terms the reference
+    sheet barely excites."""
+    from nekcem_amd import usr
+    c, sol, G0 = _graphene_oracle(oracle, STRONG_SHEET if strong else usr.GRAPHENE_SHEET)
+    l2tol, litol = np.array(usr.GRAPHENE_L2TOL), np.array(usr.GRAPHENE_LINFTOL)
+    if strong:
+        # the same fields' tolerance; Hy, Ey stay at roundoff
+        l2tol[[1, 4]], litol[[1, 4]] = 1e-13, 5e-12
+    for t, l2, li in _graphene_errors(c, sol, G0, 200 if strong else 1000):
+        assert np.all(l2 <= l2tol) and np.all(li <= litol), (t, l2, li)
+    assert l2[0] > 1e-6
+
+
+@pytest.mark.parametrize("broken", ["no_sheet", "no_cp1", "no_cp2", "cp_sign", "drude_sign"])
+def test_3dgraphene_fails_with_planted_mistakes(oracle, broken):
+    """The same check misses its tolerance within 100 steps when the run's
+    sheet drops or mis-signs a term the solution has (the planted mistakes
+    are in the sheet parameters handed to the oracle)."""
+    from nekcem_amd import usr
+    s = list(STRONG_SHEET)
+    if broken == "no_sheet":
+        s = [0.0] * 12
+    elif broken == "no_cp1":
+        s[2] = s[3] = s[4] = s[5] = s[6] = 0.0
+    elif broken == "no_cp2":
+        s[7] = s[8] = s[9] = s[10] = s[11] = 0.0
+    elif broken == "cp_sign":
+        s[2], s[7] = -s[2], -s[7]
+    else:
+        s[1] = -s[1]
+    c, sol, G0 = _graphene_oracle(oracle, STRONG_SHEET, run_sheet=s)
+    l2tol, litol = np.array(usr.GRAPHENE_L2TOL), np.array(usr.GRAPHENE_LINFTOL)
+    bad = False
+    for t, l2, li in _graphene_errors(c, sol, G0, 100):
+        bad |= bool(np.any(l2 > l2tol) or np.any(li > litol))
+    assert bad

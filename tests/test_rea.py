@@ -87,3 +87,16 @@ def test_box_dielectric_and_boxpml():
     r = rea.read_case(os.path.join(CASES, "3dboxpml.rea"))
     assert r["nel"] == 216 and np.sum(r["cbc"] == "PML") == 6 * 36
     assert r["param"][77:80] == [1.0, 3.0, 1e-8] and r["param"][12] == 0.1
+
+
+def test_box_graphene():
+    """3dgraphene.rea reads its mesh from 3dgraphene.box: 4x12x4 on
+    [-1, 1]^3, P in x and z, PML walls in y; y = 0 is an element boundary
+    (between layers 6 and 7), where the sheet lies."""
+    r = rea.read_case(os.path.join(CASES, "3dgraphene.rea"))
+    assert r["in_box"] and r["nel"] == 192 and r["nelxyz"] == (4, 12, 4)
+    assert r["param"][77:80] == [2.0, 3.0, 1e-10] and r["param"][11] == 1000 and r["param"][12] == -0.005
+    _check_boxes(r)
+    assert np.sum(r["cbc"] == "PML") == 2 * 16
+    ylo = r["yc"].reshape(r["nel"], -1).min(axis=1)
+    assert np.sum(np.abs(ylo) <= 1e-12) == 16
