@@ -2,7 +2,8 @@
 tests/3dboxper case (4x4x8, N=8, dt=2e-4, 50 steps, userchk at L2 <= 5e-10,
 Linf <= 5e-9) through the iso_c_binding module and the C-ABI, then sets a
 zero-sigma PML from Fortran arrays (the fields must not change; B and D come
-back on the PML elements only)."""
+back on the PML elements only) and a Drude-only graphene sheet from the
+/usergraph/ arrays (its Drude slot charges up, the other slots stay zero)."""
 import os
 import subprocess
 
