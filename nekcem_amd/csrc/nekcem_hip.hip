@@ -240,8 +240,8 @@ int32_t stage_kflags(int s) {
 // Autotuner over the stage kernel's layouts (lane packing x register
 // budget): time each on this mesh, alternating, best of three rounds of
 // four launches, and keep the fastest.  It writes only a scratch output and
-// a scratch RK register, and with the handle's source, Drude, Lorentz, PML
-// and incident-field set-up (feat, may be null) it times the kernel
+// a scratch RK register, and with the handle's source, Drude, Lorentz, PML,
+// incident-field and graphene-sheet set-up (feat, may be null) it times the kernel
 // instance the step runs, the auxiliary fields and their registers
 // redirected to scratch: the state is untouched.  A packed layout whose
 // workgroup equals the unpacked one is not timed.  jsz, lsz, psz, gsz:

@@ -16,7 +16,8 @@
 // (r-derivative), columns (s-derivative) and z-columns (t-derivative).  In
 // the face phase the four roles take one x/y face each with the lanes mapped
 // onto face nodes; the fluxes go through LDS to the owning lanes.  The two z
-// faces are evaluated by the roles whose level half holds them.  Geometry is per element (axis-aligned affine hexahedra):
+// faces are evaluated by the roles whose level half holds them.  Geometry is
+// per element (axis-aligned affine hexahedra):
 // g = (rx, sy, tz)/J and a lift factor per face (area / (J w3) at the face
 // node, constant on an affine face).
 //
@@ -321,8 +322,9 @@ struct Shape {
 // three elements per CU instead of two; a few spills, 10 % faster at N = 7
 // with a source).  The autotuner picks.
 // FEAT = 1 compiles the in-stage source, Drude and Lorentz terms, FEAT = 2
-// the PML and the incident field as well; FEAT = 0 leaves them all out, so
-// the plain step carries none of their registers or instructions.
+// the PML, the incident field and the graphene sheet as well; FEAT = 0
+// leaves them all out, so the plain step carries none of their registers or
+// instructions.
 // EPW = elements per workgroup: EPW independent element blocks of BS
 // lanes side by side, each with its own LDS, sharing s_D and the barriers
 // (workgroup w takes elements EPW w' .. EPW w' + EPW - 1, w' its slot in the
