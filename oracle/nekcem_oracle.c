@@ -39,6 +39,34 @@
  *                      (rk_maxwell_ab), cem_common.F:2-16 (rk_c), :18-76
  *                      (rk4_upd), :78-114 (rk_storage, ifrk45 constants)
  *   orc_cem_error      cem_common.F:1335-1355 (cem_error)
+ *   orc_curl           cem_maxwell.F:574-582 + chsign, maxwell_wght_curl
+ *                      :1428-1497: the volume term alone
+ *   orc_maxwell_op_full, orc_rk_run_full
+ *                      the above with pml_step (cem_maxwell_pml.F:508-592)
+ *                      and the PML rk4_upd of rk_maxwell_ab (cem_maxwell.F:
+ *                      1942-1954), a userinc-style incident wave added to
+ *                      the face traces (cem_maxwell.F:497-498) and the
+ *                      sinusoidal usersrc time factor
+ *   orc_maxwell_op_all, orc_rk_run_all
+ *                      the above with userfsrc's graphene sheet current
+ *                      (cem_3d_graphene_current, cem_maxwell.F:2827-2932,
+ *                      called from cem_maxwell_flux3d :941-942)
+ *   orc_maxwell_vjp, orc_drude_vjp
+ *                      the transposes of the residual's derivatives (the
+ *                      discrete adjoint's checker; no reference counterpart)
+ *
+ * Each is pinned in tests/test_oracle.py by checks that do not restate it:
+ * GLL nodes, weights and D against closed forms and exact polynomial
+ * derivatives; the RK constants by their stability polynomial; geometry,
+ * face pointers, partners, materials and dxmin against what a box fixes;
+ * the operator by the reference's own 3dboxper / 3dboxpec cases at their
+ * userchk tolerances, exact curls of polynomials, spectral convergence and
+ * the skew-adjointness of the central flux; the Drude and Lorentz ADEs and
+ * the sources by closed-form uniform-field solutions; the PML, incident
+ * wave and graphene sheet by the reference's 3ddielectric and 3dgraphene
+ * exact solutions at their userchk tolerances, with planted mistakes
+ * failing; the VJPs by dot-product (transpose) identities and central
+ * differences.  No function is left unpinned.
  *
  * Built by oracle/Makefile into oracle/_build/libnekcem_oracle.so (gcc,
  * -fopenmp).  OpenMP parallelises over elements / face nodes only; summation

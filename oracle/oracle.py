@@ -22,7 +22,8 @@ way the reference does for its ``.box``/``.re2`` cases:
 
 Parity status of each C function is recorded in tests/test_oracle.py, which
 pins it against closed forms, invariants and the reference's own analytic
-test cases (tests/3dboxper, tests/3dboxpec) at the reference's tolerances.
+test cases (tests/3dboxper, tests/3dboxpec, tests/3ddielectric,
+tests/3dgraphene) at the reference's tolerances.
 """
 from __future__ import annotations
 
