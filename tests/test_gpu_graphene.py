@@ -282,3 +282,23 @@ def test_graphene_set_get_and_checks(oracle):
     s.step(0.002, 5)
     gH2, gE2 = s.get_state()
     assert np.array_equal(gH, gH2) and np.array_equal(gE, gE2)
+
+
+def test_graphene_graph_equals_direct(oracle):
+    """The captured step (two graphs, one per starting copy) and direct
+    launches give the same bits for fields and sheet state over an odd
+    number of steps split across calls."""
+    from nekcem_amd._lib import check
+    outs = []
+    for graph in (True, False):
+        s, _, _, _, _, _ = _planes(oracle, N=4)
+        if not graph:
+            check(s._L.nkc_set_graph(s._h, 0))
+        s.step(0.002, 3)
+        s.step(0.002, 4)
+        outs.append((s.get_state(), s.get_graphene("state"), s.get_graphene("register")))
+        s.close()
+    (H0, E0), G0, K0 = outs[0]
+    (H1, E1), G1, K1 = outs[1]
+    assert np.array_equal(H0, H1) and np.array_equal(E0, E1)
+    assert np.array_equal(G0, G1) and np.array_equal(K0, K1)
