@@ -1616,6 +1616,8 @@ int32_t nkc_set_graphene(nkc_handle *h, int64_t ngraph, const int32_t *graphinde
         const int q = (int)(j % n2);
         for (int c = 0; c < 12; ++c)
             if (!std::isfinite(graphparams[c * ld + j])) return fail(NKC_ERR_ARG, "graphparams(%lld, %d) not finite", (long long)j + 1, c + 1);
+        for (int c = 0; c < 15; ++c)
+            if (!std::isfinite(fjn[j + ld * (3 + c)])) return fail(NKC_ERR_ARG, "fjn(%lld, %d, %d) not finite", (long long)j + 1, c % 3 + 1, c / 3 + 2);
         const int32_t b = block(e * 6 + kf);
         const int32_t nb = h->hm.nbr[e * 6 + kf];
         if (nb >= 0) block((int64_t)nb * 6 + (kf ^ 1));

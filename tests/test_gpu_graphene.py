@@ -242,8 +242,8 @@ def test_graphene_materials_follow(oracle):
 
 def test_graphene_set_get_and_checks(oracle):
     """Round trip of the state, removal, and the argument checks: an index
-    out of range, one listed twice, a short leading dimension; the adjoint
-    refuses a sheet."""
+    out of range, one listed twice, a short leading dimension, a state that
+    is not finite; the adjoint refuses a sheet."""
     from nekcem_amd._lib import NkcError
     s, ref, H, E, G0, (gidx, par) = _planes(oracle, N=4)
     assert np.array_equal(s.get_graphene("state"), G0)
@@ -268,6 +268,10 @@ def test_graphene_set_get_and_checks(oracle):
         raw([3, 3], nxzfl)
     with pytest.raises(NkcError, match="ld"):
         raw([1], nxzfl - 1)
+    f18[3 * nxzfl] = np.nan          # fjn(1, 1, 2): point 1's Drude slot
+    with pytest.raises(NkcError, match="not finite"):
+        raw([1], nxzfl)
+    f18[3 * nxzfl] = 0.0
     # a refused call leaves no sheet: the handle steps as the bare oracle
     s.set_state(H, E)
     s.step(0.002, 5)
