@@ -1,9 +1,11 @@
 """The graphene sheet current on the HIP path: the reference's 3dgraphene
 case, built from its .rea + .box, against the oracle (parity) and against
 its exact solution at its userchk tolerances; sheets on an interior z plane
-(both sides listed), an interior x plane (one side listed) and a PEC wall at
-N = 7, with random per-point sheet parameters and two materials, against the
-oracle; every layout bit-identical; set/get round trip and argument checks.
+(both sides listed), an interior x plane and a periodic wrap face (one side
+listed) and a PEC wall at N = 7 and 4, with random per-point sheet
+parameters and two materials, against the oracle; every layout
+bit-identical; graph replay equal to direct launches; set/get round trip and
+argument checks.
 
 Tolerances: parity 1e-11 relative L2 per field (bm1-weighted) and for the
 sheet state (relative L2 over the listed points); residuals 1e-11 of the
@@ -144,7 +146,8 @@ def _planes(oracle, N=7, nel=(4, 3, 4), seed=3):
     by element parity), random smooth fields; sheets with random per-point
     parameters and state on: the interior z plane between layers 1 and 2
     (both sides), the interior x plane between columns 1 and 2 (the x+ side
-    only) and the y- PEC wall of the bottom layer."""
+    only), the y- PEC wall of the bottom layer and, in the first z layer,
+    the periodic x wrap (the x- side only)."""
     n1 = N + 1
     n2, n3 = n1 * n1, n1 ** 3
     print(f"[case] sheet planes N={N} nel={nel}", flush=True)
@@ -164,6 +167,8 @@ def _planes(oracle, N=7, nel=(4, 3, 4), seed=3):
             faces.append((e, 2))
         if ey[e] == 0:
             faces.append((e, 1))
+        if ex[e] == 0 and ez[e] == 0:
+            faces.append((e, 4))     # x- face on the periodic wrap, one side listed
     gidx = _sheet_index(faces, n2)
     rng = np.random.default_rng(seed)
     n = len(gidx)
