@@ -485,7 +485,7 @@ def main():
             if args.dispersive_dirs >= 0 else None)
 
     line = {
-        "metric": "G(GLL-pt*field*stage)/s FP64, 32^3el N=7",
+        "metric": f"G(GLL-pt*field*stage)/s FP64, {args.nel}^3el N={args.N}",
         "value": value,
         "unit": "G(GLL-pt*field*stage)/s",
         "n_gpus": 1,
