@@ -2,22 +2,28 @@
 
 ``python -m nekcem_amd.build`` or ``nekcem_amd.build.build()``.  The library
 lands in ``nekcem_amd/lib/``; the Fortran iso_c_binding module is compiled
-alongside when amdflang is present.
+alongside when amdflang is present.  The host code and each polynomial
+order's kernel instances are separate translation units (``csrc/order_n*.hip``),
+compiled in parallel into ``nekcem_amd/_build/`` and linked into one library.
 """
 from __future__ import annotations
 
+import concurrent.futures as cf
 import os
 import shutil
 import subprocess
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SRC = os.path.join(HERE, "csrc", "nekcem_hip.hip")
+CSRC = os.path.join(HERE, "csrc")
+SRC = os.path.join(CSRC, "nekcem_hip.hip")
+ORDERS = (4, 5, 6, 7, 8, 9, 10)
+OBJDIR = os.path.join(HERE, "_build")
 OUT = os.path.join(HERE, "lib", "libnekcem_hip.so")
 ARCH = os.environ.get("NKC_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [
-    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
     # keep hipcc from forming packed-FP32 instructions
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     # fuse a*b+c only where one source expression says so: every kernel
@@ -34,15 +40,35 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose=False):
-    os.makedirs(os.path.dirname(OUT), exist_ok=True)
-    tmp = OUT + ".tmp"
-    cmd = [hipcc()] + HIPCC_FLAGS + ["-I", os.path.join(ROOT, "include"), "-o", tmp, SRC]
+def sources():
+    return [SRC] + [os.path.join(CSRC, f"order_n{n}.hip") for n in ORDERS]
+
+
+def _compile(src, extra):
+    obj = os.path.join(OBJDIR, os.path.basename(src) + ".o")
+    cmd = [hipcc()] + HIPCC_FLAGS + list(extra) + ["-I", os.path.join(ROOT, "include"), "-c", "-o", obj, src]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
-        raise RuntimeError("hipcc failed:\n" + r.stderr[-4000:])
-    if verbose and r.stderr:
-        print(r.stderr)
+        raise RuntimeError(f"hipcc failed on {os.path.basename(src)}:\n" + r.stderr[-4000:])
+    return obj, r.stderr
+
+
+def build(verbose=False, extra=(), jobs=None):
+    os.makedirs(os.path.dirname(OUT), exist_ok=True)
+    os.makedirs(OBJDIR, exist_ok=True)
+    srcs = sources()
+    jobs = jobs or min(len(srcs), max(1, min(os.cpu_count() or 1, 8)))
+    with cf.ThreadPoolExecutor(jobs) as ex:
+        res = list(ex.map(lambda s: _compile(s, extra), srcs))
+    if verbose:
+        for _, err in res:
+            if err:
+                print(err)
+    tmp = OUT + ".tmp"
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + [o for o, _ in res]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("link failed:\n" + r.stderr[-4000:])
     os.replace(tmp, OUT)
     build_fortran()
     return OUT

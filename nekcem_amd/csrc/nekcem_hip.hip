@@ -23,6 +23,7 @@
 #include "adjoint_kernel.hpp"
 #include "diag_kernels.hpp"
 #include "mesh_tables.hpp"
+#include "order_launch.hpp"
 #include "stage_kernel.hpp"
 
 using namespace nkc;
@@ -139,46 +140,6 @@ void dev_mesh_free(DevMesh &dm) {
 // elements per workgroup, packed, without / with the OCC budget (N <= 5
 // only).  The residual test entry uses layout 0.
 constexpr int NLAYOUT = NKC_NLAYOUT;
-constexpr int EPW_MAX_N1 = 6;           // orders with the two-element layouts
-
-template <int N1, int MAT, int MODE, int PACK, int OCC, int EPW>
-void launch_t(const StageArgs &a, int grid, hipStream_t s) {
-    const dim3 bs(Shape<N1, PACK>::BS * EPW);
-    if (a.pml_off || a.inc_mask || a.gr_off)
-        hipLaunchKernelGGL((stage_kernel<N1, MAT, MODE, PACK, OCC, 2, EPW>), dim3(grid), bs, 0, s, a);
-    else if (a.src_off || a.drude_off || a.lor_off)
-        hipLaunchKernelGGL((stage_kernel<N1, MAT, MODE, PACK, OCC, 1, EPW>), dim3(grid), bs, 0, s, a);
-    else
-        hipLaunchKernelGGL((stage_kernel<N1, MAT, MODE, PACK, OCC, 0, EPW>), dim3(grid), bs, 0, s, a);
-}
-
-template <int N1, int MAT, int PACK, int OCC, int EPW>
-void launch_m(const StageArgs &a, int mode, int grid, hipStream_t s) {
-    if (mode == MODE_RK) launch_t<N1, MAT, MODE_RK, PACK, OCC, EPW>(a, grid, s);
-    else launch_t<N1, MAT, MODE_RESIDUAL, 0, 0, 1>(a, grid, s);
-}
-
-template <int N1, int PACK, int OCC, int EPW>
-void launch_p(const StageArgs &a, int mat, int mode, int grid, hipStream_t s) {
-    if (mat == MAT_ELEM) launch_m<N1, MAT_ELEM, PACK, OCC, EPW>(a, mode, grid, s);
-    else launch_m<N1, MAT_NODE, PACK, OCC, EPW>(a, mode, grid, s);
-}
-
-template <int N1>
-void launch_n(const StageArgs &a, int mat, int mode, int layout, int grid, hipStream_t s) {
-    switch (layout) {
-    case 1: launch_p<N1, 1, 0, 1>(a, mat, mode, grid, s); break;
-    case 2: launch_p<N1, 0, 1, 1>(a, mat, mode, grid, s); break;
-    case 3: launch_p<N1, 1, 1, 1>(a, mat, mode, grid, s); break;
-    case 4:
-        if constexpr (N1 <= EPW_MAX_N1) launch_p<N1, 1, 0, 2>(a, mat, mode, grid, s);
-        break;
-    case 5:
-        if constexpr (N1 <= EPW_MAX_N1) launch_p<N1, 1, 1, 2>(a, mat, mode, grid, s);
-        break;
-    default: launch_p<N1, 0, 0, 1>(a, mat, mode, grid, s); break;
-    }
-}
 
 // Elements per workgroup of a layout.
 int layout_epw(int layout) { return layout >= 4 ? 2 : 1; }
@@ -203,13 +164,13 @@ int32_t launch_stage(const DevMesh &dm, const StageArgs &a, int mode, hipStream_
     StageArgs b = a;
     b.per_xcd = per_xcd;
     switch (dm.n1) {
-    case 4: launch_n<4>(b, dm.mat, mode, pk, grid, s); break;
-    case 5: launch_n<5>(b, dm.mat, mode, pk, grid, s); break;
-    case 6: launch_n<6>(b, dm.mat, mode, pk, grid, s); break;
-    case 7: launch_n<7>(b, dm.mat, mode, pk, grid, s); break;
-    case 8: launch_n<8>(b, dm.mat, mode, pk, grid, s); break;
-    case 9: launch_n<9>(b, dm.mat, mode, pk, grid, s); break;
-    case 10: launch_n<10>(b, dm.mat, mode, pk, grid, s); break;
+    case 4: launch_stage_n4(b, dm.mat, mode, pk, grid, s); break;
+    case 5: launch_stage_n5(b, dm.mat, mode, pk, grid, s); break;
+    case 6: launch_stage_n6(b, dm.mat, mode, pk, grid, s); break;
+    case 7: launch_stage_n7(b, dm.mat, mode, pk, grid, s); break;
+    case 8: launch_stage_n8(b, dm.mat, mode, pk, grid, s); break;
+    case 9: launch_stage_n9(b, dm.mat, mode, pk, grid, s); break;
+    case 10: launch_stage_n10(b, dm.mat, mode, pk, grid, s); break;
     default: return fail(NKC_ERR_UNSUPPORTED, "polynomial order N=%d not built", dm.n1 - 1);
     }
     HIPCHK(hipGetLastError());
@@ -738,14 +699,6 @@ int32_t copy_fields_d2h(const double *src, int64_t fs, int64_t src_off, double *
     return NKC_OK;
 }
 
-template <int N1, int MAT>
-void launch_adj_t(const AdjArgs &a, int grid, hipStream_t s) {
-    if (a.rs)
-        hipLaunchKernelGGL((adjoint_kernel<N1, MAT, 1>), dim3(grid), dim3(Shape<N1>::BS), 0, s, a);
-    else
-        hipLaunchKernelGGL((adjoint_kernel<N1, MAT, 0>), dim3(grid), dim3(Shape<N1>::BS), 0, s, a);
-}
-
 int32_t launch_adjoint(const DevMesh &dm, const AdjArgs &a0, hipStream_t s) {
     const int per_xcd = (int)((dm.nelt + 7) / 8);
     const int grid = 8 * per_xcd;
@@ -758,17 +711,16 @@ int32_t launch_adjoint(const DevMesh &dm, const AdjArgs &a0, hipStream_t s) {
     a.imu = dm.imu;
     a.fs = dm.fs;
     a.nelt = (int32_t)dm.nelt;
-#define NKC_ADJ_CASE(n)                                                            \
-    case n:                                                                        \
-        if (dm.mat == MAT_ELEM) launch_adj_t<n, MAT_ELEM>(a, grid, s);             \
-        else launch_adj_t<n, MAT_NODE>(a, grid, s);                                \
-        break;
     switch (dm.n1) {
-        NKC_ADJ_CASE(4) NKC_ADJ_CASE(5) NKC_ADJ_CASE(6) NKC_ADJ_CASE(7)
-        NKC_ADJ_CASE(8) NKC_ADJ_CASE(9) NKC_ADJ_CASE(10)
+    case 4: launch_adjoint_n4(a, dm.mat, grid, s); break;
+    case 5: launch_adjoint_n5(a, dm.mat, grid, s); break;
+    case 6: launch_adjoint_n6(a, dm.mat, grid, s); break;
+    case 7: launch_adjoint_n7(a, dm.mat, grid, s); break;
+    case 8: launch_adjoint_n8(a, dm.mat, grid, s); break;
+    case 9: launch_adjoint_n9(a, dm.mat, grid, s); break;
+    case 10: launch_adjoint_n10(a, dm.mat, grid, s); break;
     default: return fail(NKC_ERR_UNSUPPORTED, "polynomial order N=%d not built", dm.n1 - 1);
     }
-#undef NKC_ADJ_CASE
     HIPCHK(hipGetLastError());
     return NKC_OK;
 }
