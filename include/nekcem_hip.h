@@ -106,9 +106,12 @@ int32_t nkc_create_hex(int32_t N, int32_t ifupwind, int64_t nelt, const double *
 int32_t nkc_destroy(nkc_handle *h);
 
 /* !$ACC UPDATE DEVICE / HOST of hn, en, khn, ken (src/EMWAVE:4-23).  Each is
- * 3 components at stride ld (= lpts1 in the reference) with ld >= npts.
+ * 3 components at stride ld (= lpts1 in the reference) with ld >= npts;
+ * entries npts..ld-1 of each component are neither read nor written.
  * khn/ken may be NULL: set_state then zeroes k (a1 = 0 makes k from a
- * previous step irrelevant), get_state skips them. */
+ * previous step irrelevant), get_state skips them.  After nkc_step, khn/ken
+ * hold the register of the step's fourth stage unless nkc_set_keep_k(h, 1)
+ * is in effect (see there). */
 int32_t nkc_set_state(nkc_handle *h, const double *hn, const double *en,
                       const double *khn, const double *ken, int64_t ld);
 int32_t nkc_get_state(nkc_handle *h, double *hn, double *en, double *khn,
@@ -120,6 +123,16 @@ int32_t nkc_get_state(nkc_handle *h, double *hn, double *en, double *khn,
  * src/cem_drive.F:639).  Asynchronous on the handle's stream; the next
  * get_state or nkc_sync waits for it. */
 int32_t nkc_step(nkc_handle *h, double *time, double dt, int64_t nsteps);
+
+/* The RK register khn/ken as rk4_upd leaves it (src/cem_common.F:18-76).
+ * rk4_upd writes k in all five stages, but the next step's first stage
+ * multiplies it by a_1 = 0, so by default (keep = 0) nkc_step's last stage
+ * skips that write (10 % of a step's bytes): khn/ken read after a step then
+ * hold the fourth stage's register, while the fields are unaffected.  With
+ * keep = 1 the last stage writes it, and khn/ken after nkc_step (and
+ * nkc_adjoint_forward) equal the reference's.  The Drude, Lorentz, PML and
+ * graphene registers are always written. */
+int32_t nkc_set_keep_k(nkc_handle *h, int32_t keep);
 
 /* One evaluation of cem_maxwell_op (src/cem_maxwell.F:484-508) on the
  * current state: reshn/resen = inverse-mass-scaled residual, 3 components at

@@ -107,6 +107,7 @@ SIGNATURES = [
     ("nkc_stream", C.c_int32, [C.c_void_p, C.POINTER(C.c_void_p)]),
     ("nkc_info", C.c_int32, [C.c_void_p, C.POINTER(C.c_int64)]),
     ("nkc_set_graph", C.c_int32, [C.c_void_p, C.c_int32]),
+    ("nkc_set_keep_k", C.c_int32, [C.c_void_p, C.c_int32]),
     ("nkc_batch_create", C.c_int32, [C.c_int32, C.POINTER(NkcBoxDesc), C.POINTER(C.c_void_p)]),
     ("nkc_batch_destroy", C.c_int32, [C.c_void_p]),
     ("nkc_batch_set_state", C.c_int32, [C.c_void_p, C.c_int32, _dp, _dp, C.c_int64]),

@@ -352,6 +352,7 @@ struct nkc_handle {
     unsigned long long *red = nullptr;   // check_finite result
     int cur = 0;                          // which u buffer holds the state
     bool use_graph = true;
+    bool keep_k = false;                  // last stage writes the RK register (nkc_set_keep_k)
     double graph_dt = NAN;
     hipGraphExec_t gexec[2] = {nullptr, nullptr};
     // tdev[0] = time at the start of the next step, tdev[1] = stage time for
@@ -511,6 +512,12 @@ StageArgs base_args(nkc_handle *h) {
     return a;
 }
 
+// RK-register flags of stage s for this handle: with keep_k the last stage
+// writes the register too, so khn/ken after a step are rk4_upd's.
+int32_t handle_kflags(const nkc_handle *h, int s) {
+    return stage_kflags(s) | ((h->keep_k && s == 4) ? 2 : 0);
+}
+
 // Queue one RK step starting from buffer `from` (ends in 1-from).
 int32_t queue_step(nkc_handle *h, int from, double dt) {
     for (int s = 0; s < 5; ++s) {
@@ -522,7 +529,7 @@ int32_t queue_step(nkc_handle *h, int from, double dt) {
         a.gAn = h->gA[src ^ 1];
         a.kbuf = h->m.k;
         a.rka = RK4A[s];
-        a.kflags = stage_kflags(s);
+        a.kflags = handle_kflags(h, s);
         a.rkb = RK4B[s];
         a.rkc = RK4C[s];
         a.dt = dt;
@@ -742,7 +749,7 @@ int32_t queue_step_store(nkc_handle *h, double dt, int64_t slot) {
         a.uout = su + (slot * 5 + s + 1) * blk;
         a.kbuf = h->m.k;
         a.rka = RK4A[s];
-        a.kflags = stage_kflags(s);
+        a.kflags = handle_kflags(h, s);
         a.rkb = RK4B[s];
         a.rkc = RK4C[s];
         a.dt = dt;
@@ -1705,6 +1712,14 @@ int32_t nkc_set_graph(nkc_handle *h, int32_t enable) {
     h->use_graph = enable != 0;
     if (!h->use_graph) return drop_graphs(h);
     return NKC_OK;
+}
+
+int32_t nkc_set_keep_k(nkc_handle *h, int32_t keep) {
+    if (!h) return fail(NKC_ERR_ARG, "null handle");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (h->keep_k == (keep != 0)) return NKC_OK;
+    h->keep_k = keep != 0;
+    return drop_graphs(h);   // the captured step carries the stage flags
 }
 
 int32_t nkc_set_layout(nkc_handle *h, int32_t layout) {

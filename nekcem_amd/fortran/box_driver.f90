@@ -5,11 +5,15 @@
 ! tests/3dboxper/3dboxper.usr:169-216 (istep <= 10 or mod(istep,50) == 0,
 ! L2 <= 5e-10 and Linf <= 5e-9 per field, else exit status 1).
 !
-! The arrays are built in the reference's COMMON layout (hn(lpts1,3),
-! face arrays (lx1*lz1*6*nelt) in Ed's face order, 1-based cemface and partner
-! map) for straight-sided box elements: cofactor metrics rx = (hy/2)(hz/2),
-! sy = (hx/2)(hz/2), tz = (hx/2)(hy/2), jac = hx hy hz/8, bm1 = jac w_i w_j w_k,
-! area = face Jacobian x w_a w_b (src/nek5_coef.F:555-635, 1150-1237).
+! The arrays are built in the reference's COMMON layout (hn(lpts1,3) with
+! lpts1 = lx1**3 * lelt above npts, as lelt = lelg + 3 in the case's SIZE
+! makes it; face arrays (lx1*lz1*6*nelt) in Ed's face order, 1-based cemface
+! and partner map) for straight-sided box elements; every array crosses the
+! C-ABI at the COMMON stride (lpts1 or lxzfl1), and the padding rows beyond
+! npts / nxzfl must come back untouched.  Straight-sided box elements have
+! cofactor metrics rx = (hy/2)(hz/2), sy = (hx/2)(hz/2), tz = (hx/2)(hy/2),
+! jac = hx hy hz/8, bm1 = jac w_i w_j w_k, area = face Jacobian x w_a w_b
+! (src/nek5_coef.F:555-635, 1150-1237).
 program box_driver
   use iso_c_binding
   use nekcem_hip
@@ -17,6 +21,9 @@ program box_driver
   integer, parameter :: N = 8, n1 = N + 1, nelx = 4, nely = 4, nelz = 8
   integer, parameter :: nelt = nelx*nely*nelz, n2 = n1*n1, n3 = n2*n1
   integer, parameter :: npts = n3*nelt, nxzfl = 6*n2*nelt
+  ! COMMON capacities: lelt = lelg + 3 (tests/3dboxper/SIZE:15)
+  integer, parameter :: lelt = nelt + 3, lpts1 = n3*lelt, lxzfl1 = 6*n2*lelt
+  real(c_double), parameter :: pad = -7.25d0
   real(c_double), parameter :: pi = 3.141592653589793238462643383279502884d0
   real(c_double), target :: z(n1), w(n1), dxm1(n1, n1)
   real(c_double), allocatable, target :: xm1(:), ym1(:), zm1(:)
@@ -43,7 +50,8 @@ program box_driver
   allocate(tx(npts), ty(npts), tz(npts), jac(npts), bm1(npts))
   allocate(unx(nxzfl), uny(nxzfl), unz(nxzfl), area(nxzfl), cemface(nxzfl), partner(nxzfl))
   allocate(eps(npts), mu(npts), z0(nxzfl), z1(nxzfl), y0(nxzfl), y1(nxzfl))
-  allocate(hn(npts,3), en(npts,3), sh(npts,3), se(npts,3))
+  allocate(hn(lpts1,3), en(lpts1,3), sh(npts,3), se(npts,3))
+  hn = pad; en = pad
 
   do e = 1, nelt
      ex = mod(e-1, nelx); ey = mod((e-1)/nelx, nely); ez = (e-1)/(nelx*nely)
@@ -114,13 +122,15 @@ program box_driver
 
   time = 0
   dt = 2d-4
-  call usersol(time, hn, en)                      ! userini
-  call nkc_check(nkc_set_state(h, hn, en, c_null_ptr, c_null_ptr, int(npts, c_int64_t)), 'nkc_set_state')
+  call usersol(time, sh, se)                      ! userini
+  hn(1:npts,:) = sh; en(1:npts,:) = se
+  call nkc_check(nkc_set_state(h, hn, en, c_null_ptr, c_null_ptr, int(lpts1, c_int64_t)), 'nkc_set_state')
   do istep = 1, 50
      call nkc_check(nkc_step(h, time, dt, 1_c_int64_t), 'nkc_step')
      if (istep <= 10 .or. mod(istep, 50) == 0) then          ! userchk
-        call nkc_check(nkc_get_state(h, hn, en, c_null_ptr, c_null_ptr, int(npts, c_int64_t)), &
+        call nkc_check(nkc_get_state(h, hn, en, c_null_ptr, c_null_ptr, int(lpts1, c_int64_t)), &
              'nkc_get_state')
+        call check_pad(hn, 'hn'); call check_pad(en, 'en')
         call usersol(time, sh, se)
         do c = 1, 3
            call cem_error(hn(:,c), sh(:,c), l2, linf)
@@ -138,13 +148,14 @@ program box_driver
   block
     integer(c_int32_t), target :: pmlptr(2)
     real(c_double), allocatable :: sigma(:,:), bd(:,:)
-    allocate(sigma(npts,3), bd(npts,6))
-    sigma = 0; bd = 0
+    allocate(sigma(lpts1,3), bd(lpts1,6))
+    sigma = 0; sigma(npts+1:,:) = pad; bd = pad
     pmlptr = [1, 2]
     call nkc_check(nkc_set_pml(h, 2_c_int64_t, pmlptr, sigma, c_null_ptr, c_null_ptr, &
-         int(npts, c_int64_t)), 'nkc_set_pml')
+         int(lpts1, c_int64_t)), 'nkc_set_pml')
     call nkc_check(nkc_step(h, time, dt, 10_c_int64_t), 'nkc_step (PML)')
-    call nkc_check(nkc_get_state(h, hn, en, c_null_ptr, c_null_ptr, int(npts, c_int64_t)), 'nkc_get_state')
+    call nkc_check(nkc_get_state(h, hn, en, c_null_ptr, c_null_ptr, int(lpts1, c_int64_t)), 'nkc_get_state')
+    call check_pad(hn, 'hn'); call check_pad(en, 'en')
     call usersol(time, sh, se)
     do c = 1, 3
        call cem_error(hn(:,c), sh(:,c), l2, linf)
@@ -152,16 +163,17 @@ program box_driver
        call cem_error(en(:,c), se(:,c), l2, linf)
        if (l2 > 5d-10 .or. linf > 5d-9) call fail(60, 3+c, l2, linf)
     end do
-    call nkc_check(nkc_get_pml(h, 0_c_int32_t, bd, int(npts, c_int64_t)), 'nkc_get_pml')
+    call nkc_check(nkc_get_pml(h, 0_c_int32_t, bd, int(lpts1, c_int64_t)), 'nkc_get_pml')
     if (.not. (maxval(abs(bd(1:2*n3, :))) > 0 .and. maxval(abs(bd(1:2*n3, :))) < 1d10)) then
-       write(6, '(a,es12.4)') 'box_driver: PML state not as expected, max |B,D| =', maxval(abs(bd))
+       write(6, '(a,es12.4)') 'box_driver: PML state not as expected, max |B,D| =', maxval(abs(bd(1:2*n3, :)))
        error stop 1
     end if
-    if (maxval(abs(bd(2*n3+1:, :))) /= 0) then
-       write(6, '(a)') 'box_driver: PML state outside the PML elements'
+    ! nodes outside the PML elements and the padding rows are not written
+    if (any(bd(2*n3+1:, :) /= pad)) then
+       write(6, '(a)') 'box_driver: PML state written outside the PML elements'
        error stop 1
     end if
-    write(6, '(a,es12.4)') 'box_driver: PML entry points ok, max |B,D| =', maxval(abs(bd))
+    write(6, '(a,es12.4)') 'box_driver: PML entry points ok, max |B,D| =', maxval(abs(bd(1:2*n3, :)))
   end block
   ! The graphene-sheet entry points from Fortran, with the arrays of the
   ! .usr's /usergraph/ block: a Drude-only sheet (b_d = 1.5) on the x+ face
@@ -172,22 +184,23 @@ program box_driver
     integer(c_int32_t), allocatable :: graphindex(:)
     real(c_double), allocatable :: graphparams(:,:), fjn(:,:,:)
     integer :: i
-    allocate(graphindex(n2), graphparams(nxzfl,12), fjn(nxzfl,3,6))
+    allocate(graphindex(n2), graphparams(lxzfl1,12), fjn(lxzfl1,3,6))
     graphparams = 0; fjn = 0
+    graphparams(nxzfl+1:,:) = pad; fjn(nxzfl+1:,:,:) = pad
     do i = 1, n2
        graphindex(i) = n2*(2-1) + i          ! element 1, face 2 (x+)
        graphparams(graphindex(i), 2) = 1.5d0
     end do
     call nkc_check(nkc_set_graphene(h, int(n2, c_int64_t), graphindex, graphparams, fjn, &
-         int(nxzfl, c_int64_t)), 'nkc_set_graphene')
+         int(lxzfl1, c_int64_t)), 'nkc_set_graphene')
     call nkc_check(nkc_step(h, time, dt, 5_c_int64_t), 'nkc_step (graphene)')
-    call nkc_check(nkc_get_graphene(h, 0_c_int32_t, fjn, int(nxzfl, c_int64_t)), 'nkc_get_graphene')
+    call nkc_check(nkc_get_graphene(h, 0_c_int32_t, fjn, int(lxzfl1, c_int64_t)), 'nkc_get_graphene')
     if (.not. (maxval(abs(fjn(graphindex, :, 2))) > 0 .and. maxval(abs(fjn(graphindex, :, 2))) < 1d10) &
-         .or. maxval(abs(fjn(:, :, 3:6))) /= 0) then
+         .or. maxval(abs(fjn(1:nxzfl, :, 3:6))) /= 0 .or. any(fjn(nxzfl+1:, :, :) /= pad)) then
        write(6, '(a)') 'box_driver: graphene state not as expected'
        error stop 1
     end if
-    call nkc_check(nkc_set_graphene(h, 0_c_int64_t, graphindex, graphparams, fjn, int(nxzfl, c_int64_t)), &
+    call nkc_check(nkc_set_graphene(h, 0_c_int64_t, graphindex, graphparams, fjn, int(lxzfl1, c_int64_t)), &
          'nkc_set_graphene (remove)')
     write(6, '(a,es12.4)') 'box_driver: graphene entry points ok, max |fjn(:,:,2)| =', &
          maxval(abs(fjn(graphindex, :, 2)))
@@ -196,6 +209,16 @@ program box_driver
   write(6, '(a)') 'box_driver: 3dboxper passed on the HIP path'
 
 contains
+
+  ! the rows npts+1..lpts1 of a COMMON field array hold the pad value still
+  subroutine check_pad(u, what)
+    real(c_double), intent(in) :: u(:,:)
+    character(len=*), intent(in) :: what
+    if (any(u(npts+1:, :) /= pad)) then
+       write(6, '(a,a)') 'box_driver: padding rows written in ', what
+       error stop 1
+    end if
+  end subroutine check_pad
 
   subroutine fail(istep, c, l2, linf)
     integer, intent(in) :: istep, c

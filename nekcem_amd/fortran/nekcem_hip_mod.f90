@@ -39,7 +39,7 @@ module nekcem_hip
        nkc_set_graphene, nkc_get_graphene, &
        nkc_adjoint_forward, nkc_adjoint_backward, nkc_adjoint_get_drude, &
        nkc_check_finite, nkc_sync, nkc_stream, nkc_info, &
-       nkc_set_graph, nkc_last_error, nkc_check, nkc_error_message
+       nkc_set_graph, nkc_set_keep_k, nkc_last_error, nkc_check, nkc_error_message
 
   interface
      integer(c_int32_t) function nkc_create(d, h) bind(C, name='nkc_create')
@@ -262,6 +262,13 @@ module nekcem_hip
        type(c_ptr), value :: h
        integer(c_int32_t), value :: enable
      end function nkc_set_graph
+
+     ! 1: khn/ken after nkc_step are rk4_upd's register (last stage writes it)
+     integer(c_int32_t) function nkc_set_keep_k(h, keep) bind(C, name='nkc_set_keep_k')
+       import :: c_int32_t, c_ptr
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: keep
+     end function nkc_set_keep_k
 
      type(c_ptr) function nkc_last_error() bind(C, name='nkc_last_error')
        import :: c_ptr

@@ -282,6 +282,12 @@ class MaxwellSolver:
         _lib.check(self._L.nkc_curl(self._h, _d(rH), _d(rE), n))
         return rH, rE
 
+    def set_keep_k(self, keep=True):
+        """With keep, the RK register read back by get_state(with_k=True)
+        after a step is rk4_upd's (the last stage writes it); without, it
+        is the fourth stage's (the default: the write is skipped)."""
+        _lib.check(self._L.nkc_set_keep_k(self._h, 1 if keep else 0))
+
     def step(self, dt, nsteps=1):
         t = C.c_double(self.time)
         _lib.check(self._L.nkc_step(self._h, C.byref(t), float(dt), int(nsteps)))

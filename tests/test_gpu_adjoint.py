@@ -13,6 +13,7 @@ CASES = [
     dict(nel=(2, 2, 2), N=7, bc=("PEC", "PEC", "PEC"), upwind=True),
     dict(nel=(3, 2, 2), N=4, bc=("P", "PEC", "P"), upwind=False),
     dict(nel=(2, 2, 2), N=8, bc=("PEC", "P", "P"), upwind=True),
+    dict(nel=(2, 1, 2), N=9, bc=("P", "PEC", "P"), upwind=True),
 ]
 
 
