@@ -319,6 +319,20 @@ int32_t nkc_cem_error(nkc_handle *h, int32_t sol, double time, double *out);
 int32_t nkc_cem_error_fields(nkc_handle *h, const double *hx, const double *ex, int64_t ld,
                              double *out);
 
+/* nkc_cem_error without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same kernels and the same out[12] as nkc_cem_error, bit for
+ * bit; the partial norms go to a fixed 107 KB device array of the library
+ * (one per GPU, held until the process ends) instead of handle memory.
+ * The node coordinates must already be on the device (nkc_set_coords, or
+ * on a box handle one earlier nkc_cem_error / nkc_init_solution), else
+ * NKC_ERR_ARG; a mesh needing more partial norms than the array holds
+ * (never, at its 1024-workgroup cap) is refused with NKC_ERR_ARG.  The
+ * array is shared by every call on a GPU: calls that overlap on one GPU
+ * (from several threads or streams) race. */
+int32_t nkc_cem_error_noalloc(nkc_handle *h, int32_t sol, double time, double *out);
+
 /* chk_blowup (src/cem_common.F:1150-1165): *maxabs = max |u| over the six
  * fields; returns NKC_ERR_BLOWUP if a value is non-finite or above 1e20. */
 int32_t nkc_check_finite(nkc_handle *h, double *maxabs);

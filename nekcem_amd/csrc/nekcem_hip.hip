@@ -18,6 +18,7 @@
 #include <string>
 #include <vector>
 #include <array>
+#include <mutex>
 
 #include "../../include/nekcem_hip.h"
 #include "adjoint_kernel.hpp"
@@ -672,6 +673,8 @@ int32_t make_handle(const HostMesh &hm, double C0, nkc_handle **out) {
             rc = fail(NKC_ERR_HIP, "hipStreamCreate"); break;
         }
         if ((rc = dev_mesh_upload(hm, h->m))) break;
+        // element Jacobians and GLL weights of bm1 for the diagnostics
+        if ((rc = upload(&h->djac, hm.jac)) || (rc = upload(&h->dw, hm.w))) break;
         if (hipMalloc((void **)&h->red, 8) != hipSuccess) { rc = fail(NKC_ERR_HIP, "hipMalloc"); break; }
         if (hipMalloc((void **)&h->tdev, 2 * sizeof(double)) != hipSuccess ||
             hipMemset(h->tdev, 0, 2 * sizeof(double)) != hipSuccess) { rc = fail(NKC_ERR_HIP, "hipMalloc time"); break; }
@@ -1196,18 +1199,25 @@ int32_t nkc_init_solution(nkc_handle *h, int32_t sol, double time) {
 
 namespace {
 
-int32_t run_error(nkc_handle *h, int sol, double time, const double *exact, double *out) {
-    int32_t rc;
-    if ((rc = ensure_diag(h))) return rc;
+// Workgroups of the cem_error partials kernel for this handle's mesh.
+int error_blocks(const nkc_handle *h) {
+    return (int)std::min<int64_t>(DIAG_BLOCKS, (h->m.npts + DIAG_BS - 1) / DIAG_BS);
+}
+
+// cem_error's two kernels with the partial norms in part (error_blocks(h)
+// x DIAG_NV doubles, every entry written by the first kernel before the
+// second reads it) and the DIAG_NV totals in dout; out[12] on the host.
+int32_t error_launch(nkc_handle *h, int sol, double time, const double *exact, double *part, double *dout,
+                     double *out) {
     const DevMesh &m = h->m;
-    const int nb = (int)std::min<int64_t>(DIAG_BLOCKS, (m.npts + DIAG_BS - 1) / DIAG_BS);
+    const int nb = error_blocks(h);
     hipLaunchKernelGGL(error_partials, dim3(nb), dim3(DIAG_BS), 0, h->stream, sol, time, (const double *)m.u[h->cur],
                        exact, m.fs, m.npts, m.n1, (const double *)h->xyz, (const double *)h->djac,
-                       (const double *)h->dw, h->dpart);
-    hipLaunchKernelGGL(error_final, dim3(1), dim3(64), 0, h->stream, (const double *)h->dpart, nb, h->dout);
+                       (const double *)h->dw, part);
+    hipLaunchKernelGGL(error_final, dim3(1), dim3(64), 0, h->stream, (const double *)part, nb, dout);
     HIPCHK(hipGetLastError());
     double v[DIAG_NV];
-    HIPCHK(hipMemcpyAsync(v, h->dout, sizeof v, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(v, dout, sizeof v, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     for (int c = 0; c < 6; ++c) {
         const double r = v[c] / v[12];
@@ -1215,6 +1225,12 @@ int32_t run_error(nkc_handle *h, int sol, double time, const double *exact, doub
         out[6 + c] = v[6 + c];
     }
     return NKC_OK;
+}
+
+int32_t run_error(nkc_handle *h, int sol, double time, const double *exact, double *out) {
+    int32_t rc;
+    if ((rc = ensure_diag(h))) return rc;
+    return error_launch(h, sol, time, exact, h->dpart, h->dout, out);
 }
 
 }  // namespace
@@ -1237,6 +1253,66 @@ int32_t nkc_cem_error_fields(nkc_handle *h, const double *hx, const double *ex, 
     if (!(rc = copy_fields_h2d(tmp, m.fs, 0, hx, ex, m.npts, ld))) rc = run_error(h, SOL_FIELDS, 0.0, tmp, out);
     (void)hipFree(tmp);
     return rc;
+}
+
+}  // extern "C"
+
+// ---- scratch of the no-allocation cem_error ------------------------------
+// The partial norms (at most DIAG_BLOCKS x DIAG_NV doubles) and the DIAG_NV
+// totals of nkc_cem_error_noalloc live in this array, one copy per GPU for
+// the life of the process (107 KB).  Host code never passes its name to a
+// kernel: the device address comes from hipGetSymbolAddress, once per
+// device, and is kept in scratch_dev.
+namespace {
+constexpr int64_t SCRATCH_PART = (int64_t)DIAG_BLOCKS * DIAG_NV;
+constexpr int64_t SCRATCH_DOUBLES = SCRATCH_PART + DIAG_NV;
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex scratch_mu;
+double *scratch_dev[SCRATCH_MAX_DEVICES] = {};
+}  // namespace
+
+static __device__ __align__(256) double scratch[SCRATCH_DOUBLES];
+
+namespace {
+
+// Device address of `scratch` on device dev, looked up with that device
+// current the first time and cached.
+int32_t scratch_ptr(int dev, double **out) {
+    if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) return fail(NKC_ERR_ARG, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_dev[dev]) {
+        int prev = 0;
+        HIPCHK(hipGetDevice(&prev));
+        if (prev != dev) HIPCHK(hipSetDevice(dev));
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (prev != dev) HIPCHK(hipSetDevice(prev));
+        if (e != hipSuccess || !p)
+            return fail(NKC_ERR_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        scratch_dev[dev] = static_cast<double *>(p);
+    }
+    *out = scratch_dev[dev];
+    return NKC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t nkc_cem_error_noalloc(nkc_handle *h, int32_t sol, double time, double *out) {
+    if (!h || !out) return fail(NKC_ERR_ARG, "null handle or output");
+    if (sol != SOL_BOXPER && sol != SOL_BOXPEC) return fail(NKC_ERR_ARG, "unknown solution %d", sol);
+    if (!h->xyz)
+        return fail(NKC_ERR_ARG, "no node coordinates on the device: nkc_set_coords (or, on a box handle, one "
+                                 "nkc_cem_error or nkc_init_solution call) must come first");
+    const int64_t need = (int64_t)error_blocks(h) * DIAG_NV;
+    if (need > SCRATCH_PART)
+        return fail(NKC_ERR_ARG, "%lld partial norms exceed the scratch of %lld", (long long)need,
+                    (long long)SCRATCH_PART);
+    double *sp = nullptr;
+    const int32_t rc = scratch_ptr(h->device, &sp);
+    if (rc) return rc;
+    return error_launch(h, sol, time, nullptr, sp, sp + SCRATCH_PART, out);
 }
 
 int32_t nkc_set_lorentz(nkc_handle *h, int64_t nlor, const int32_t *lindex, const double *params,
