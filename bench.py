@@ -11,15 +11,19 @@ state resident in HBM for the whole timed region.
 
 Also reported:
   roofline      the fused stage kernel (5 launches per step): algorithmic
-                bytes per launch = B_ALG x 6 x npts, B_ALG = 28.8 B per
-                GLL-pt.field.stage (read u, read k, write k, write u = 32 B,
-                less the k read of stage 1 and the k write of stage 5), over
-                its average launch duration from HIP events recorded on the
-                library's stream; frac_at_32B is the same rate priced at the
-                plain 32 B floor (G/s x 32 B / peak);
+                bytes per launch = B_ALG x 6 x npts, B_ALG = 32 B per
+                GLL-pt.field.stage (the 2N-storage floor: read u, read k,
+                write k, write u), over its average launch duration from HIP
+                events recorded on the library's stream.  The kernel skips
+                the k read of stage 1 and the k write of stage 5 (a_1 = 0),
+                so it moves 28.8 B per unit on average: achieved_moved and
+                frac_moved price the same launches at that count.  traffic
+                is the HBM bytes per launch that rocprofv3 --pmc FETCH_SIZE
+                and WRITE_SIZE counted for this workload (the summary named
+                in traffic_source);
   cpu_baseline  the CPU oracle (a loop-for-loop C restatement of the
                 reference path, OpenMP) on the same workload for a bounded
-                number of steps on this host's cores;
+                number of steps on this host's cores, and on one core;
   parity        relative L2 (bm1-weighted) between the GPU and the CPU
                 oracle after those steps.
 """
@@ -36,12 +40,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# Algorithmic HBM bytes per GLL-pt.field of each of the 5 stages of a
-# 2N-storage step: read u, read k, write k, write u (32 B), except that the
-# first stage need not read k (a_1 = 0) and the last need not write it (the
-# next step's first stage ignores it): 24 B.  Mean over a step: 28.8 B.
+# Algorithmic HBM bytes per GLL-pt.field.stage: the 2N-storage floor, read
+# u, read k, write k, write u.
+B_ALG = 32.0
+# What the kernel moves of them per stage: the first stage need not read k
+# (a_1 = 0) and the last need not write it (the next step's first stage
+# ignores it), 24 B there.  Mean over a step: 28.8 B.
 B_STAGE = (24.0, 32.0, 32.0, 32.0, 24.0)
-B_ALG = sum(B_STAGE) / len(B_STAGE)
+B_MOVED = sum(B_STAGE) / len(B_STAGE)
 FP64_PEAK_TFLOPS = 78.6
 
 
@@ -170,7 +176,7 @@ def batched(ncav=256, steps=50, warmup=3, seed=3, log=sys.stderr):
                         f"per-element eps U[1,4], Gaussian Ez (width 0.2), own CFL-0.25 dt, seed {seed}"}
 
 
-def adjoint(nel=16, N=7, nsteps=500, every=50, ndir=32, nprobe=64, seed=4, log=sys.stderr):
+def adjoint(nel=16, N=7, nsteps=500, every=50, ndir=32, nprobe=64, seed=4, cap_steps=10, log=sys.stderr):
     """Checkpointed adjoint of the probe loss on the dielectric box: time
     relative to the forward, and agreement with central differences
     (h = 1e-6) of the HIP forward along ndir seeded random directions."""
@@ -228,6 +234,27 @@ def adjoint(nel=16, N=7, nsteps=500, every=50, ndir=32, nprobe=64, seed=4, log=s
     t_adj = time.perf_counter() - t0
     print(f"[bench] adjoint: forward {t_fwd:.3f} s, forward+checkpoints {t_fwd_ckpt:.3f} s, "
           f"backward {t_adj:.3f} s, loss {loss:.4e}, store {info}", file=log, flush=True)
+    # the same sweep with the stage store capped at `cap_steps` steps: one
+    # window held at a time, every window start recomputed
+    s.adjoint_set_store(cap_steps)
+    s.time = 0.0
+    s.set_state(H0, E0)
+    s.sync()
+    t0 = time.perf_counter()
+    s.adjoint_forward(dt, nsteps, every)
+    s.sync()
+    t_fwd_cap = time.perf_counter() - t0
+    info_cap = s.adjoint_info()
+    t0 = time.perf_counter()
+    ge_cap, _ = s.adjoint_backward(np.zeros((3, n)), lamE)
+    t_adj_cap = time.perf_counter() - t0
+    s.adjoint_set_store(0)
+    s.adjoint_release()
+    capped = {"store_steps": cap_steps, "t_forward_checkpointed_s": t_fwd_cap, "t_adjoint_s": t_adj_cap,
+              "adjoint_over_forward": t_adj_cap / t_fwd, "adjoint_store": info_cap,
+              "gradient_bitwise_equal_to_default": bool(np.array_equal(ge_cap, ge))}
+    print(f"[bench] adjoint, store capped at {cap_steps} steps: backward {t_adj_cap:.3f} s, "
+          f"store {info_cap}", file=log, flush=True)
     h = 1e-6
     errs = []
     for _ in range(ndir):
@@ -243,7 +270,7 @@ def adjoint(nel=16, N=7, nsteps=500, every=50, ndir=32, nprobe=64, seed=4, log=s
     return {"metric": "adjoint/fwd time", "value": t_adj / t_fwd, "unit": "x",
             "t_forward_s": t_fwd, "t_forward_checkpointed_s": t_fwd_ckpt, "t_adjoint_s": t_adj,
             "gradient_over_forward": (t_fwd_ckpt + t_adj) / t_fwd,
-            "adjoint_store": info,
+            "adjoint_store": info, "capped_store": capped,
             "fd_directions": ndir, "fd_h": h, "fd_max_rel_err": err, "loss": loss,
             "workload": f"PEC box {nel}^3 elements N={N} ({n} points), per-node eps = 2 + 0.3 s(x), "
                         f"Gaussian-pulse Ez source, {nprobe} probe nodes, {nsteps} steps, "
@@ -355,15 +382,37 @@ def dispersive(nel=24, N=9, nsteps=2000, every=100, ndir=2, nprobe=64, seed=5, l
                         f"gradient w.r.t. per-element gamma and omega_p"}
 
 
-def load_traffic(npts):
-    """HBM bytes per stage launch from the committed rocprofv3 --pmc summary."""
-    p = os.path.join(ROOT, "profiles", "stage_kernel_box32_N7_traffic.json")
+def load_traffic(nel, N, npts):
+    """HBM bytes per stage launch of this workload from its rocprofv3 --pmc
+    summary under profiles/ (FETCH_SIZE and WRITE_SIZE per dispatch, scaled
+    by the counters' reading for a device copy of known size): (bytes,
+    summary file) or (None, None) when no summary covers the workload."""
+    name = f"stage_kernel_box{nel}_N{N}_traffic"
+    p = os.path.join(ROOT, "profiles", name + ".json")
     if not os.path.exists(p):
-        return None
+        return None, None
     d = json.load(open(p))
     if d.get("npts") != npts:
-        return None
-    return d.get("hbm_bytes_per_launch")
+        return None, None
+    return d.get("hbm_bytes_per_launch"), f"profiles/{name}.txt"
+
+
+def cpu_facts():
+    """nproc, the cores this process may run on, and the CPU model."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"nproc": os.cpu_count(), "usable_cores": usable, "cpu_model": model}
 
 
 def main():
@@ -375,6 +424,8 @@ def main():
     ap.add_argument("--N", type=int, default=7)
     ap.add_argument("--cpu-steps", type=int, default=16,
                     help="steps of the CPU oracle for cpu_baseline (0 = skip)")
+    ap.add_argument("--cpu-steps-1core", type=int, default=1,
+                    help="steps of the one-thread CPU oracle sample (0 = skip)")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=50,
                     help="steps of the batched-cavity speedup measurement (0 = skip)")
@@ -384,7 +435,7 @@ def main():
     ap.add_argument("--adjoint-only", action="store_true")
     ap.add_argument("--adjoint-steps", type=int, default=500)
     ap.add_argument("--adjoint-every", type=int, default=50)
-    ap.add_argument("--dispersive-dirs", type=int, default=2,
+    ap.add_argument("--dispersive-dirs", type=int, default=8,
                     help="FD directions of the dispersive (Drude) adjoint check (-1 = skip the measurement)")
     ap.add_argument("--dispersive-steps", type=int, default=2000)
     ap.add_argument("--dispersive-only", action="store_true")
@@ -447,7 +498,8 @@ def main():
     avg_launch_s = ev_ms * 1e-3 / launches
     alg_bytes = B_ALG * 6 * npts
     achieved = alg_bytes / avg_launch_s / 1e9
-    traffic = load_traffic(npts)
+    moved = B_MOVED * 6 * npts / avg_launch_s / 1e9
+    traffic, traffic_src = load_traffic(args.nel, args.N, npts)
 
     # CPU baseline + parity on the same workload (bounded sample)
     cpu = None
@@ -463,6 +515,16 @@ def main():
                "cores": cores, "kind": "port",
                "sample": f"{args.cpu_steps} RK5 step(s) of the same {args.nel}^3-element N={args.N} "
                          f"workload, C oracle with OpenMP on {cores} threads, {cpu_s:.1f} s"}
+        cpu.update(cpu_facts())
+        if args.cpu_steps_1core > 0:
+            O.set_num_threads(1)
+            c0 = time.perf_counter()
+            ref.run(H, E, dt, args.cpu_steps_1core)
+            one_s = time.perf_counter() - c0
+            O.set_num_threads(cores)
+            cpu["value_1core"] = npts * 30 * args.cpu_steps_1core / one_s / 1e9
+            cpu["sample_1core"] = (f"{args.cpu_steps_1core} RK5 step(s) of the same workload, "
+                                   f"C oracle on 1 thread, {one_s:.1f} s")
         g = MaxwellSolver(mesh, eps=eps)
         g.set_state(H, E)
         g.step(dt, args.cpu_steps)
@@ -504,10 +566,12 @@ def main():
                    "graph": not args.no_graph, "stage_layout": layout},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": traffic_src,
                      "kernel": "nkc::stage_kernel (fused stage)",
                      "avg_launch_us": avg_launch_s * 1e6,
                      "alg_bytes_per_launch": alg_bytes, "alg_bytes_per_unit": B_ALG,
-                     "frac_at_32B": value * 32.0 / HBM_PEAK_GBS,
+                     "moved_bytes_per_unit": B_MOVED, "achieved_moved": moved,
+                     "frac_moved": moved / HBM_PEAK_GBS,
                      "fp64": {"alg_flops_per_unit": f_alg(args.N),
                               "achieved_tflops": f_alg(args.N) * units / wall / 1e12,
                               "peak_tflops": FP64_PEAK_TFLOPS,

@@ -88,7 +88,6 @@ int32_t nkc_create(const nkc_desc *d, nkc_handle **h);
 /* Same handle from a box description (no reference arrays needed). */
 int32_t nkc_create_box(const nkc_box_desc *d, nkc_handle **h);
 
-/* Mirror of acc_copy_all_out (src/cem_drive.F:482). */
 /* A handle from the mesh arrays NekCEM's readers fill (rdmesh/rdbdry and
  * bin_rd1_mesh/bin_rd1_bc, src/nek5_connect2.F:1774-2240): corners
  * xc/yc/zc[8*nelt] (corners 1-4 on t = -1 counter-clockwise from
@@ -103,6 +102,8 @@ int32_t nkc_create_box(const nkc_box_desc *d, nkc_handle **h);
 int32_t nkc_create_hex(int32_t N, int32_t ifupwind, int64_t nelt, const double *xc,
                        const double *yc, const double *zc, const char *cbc, const double *bc,
                        const double *eps_elem, const double *mu_elem, nkc_handle **h);
+
+/* Mirror of acc_copy_all_out (src/cem_drive.F:482): frees the handle. */
 int32_t nkc_destroy(nkc_handle *h);
 
 /* !$ACC UPDATE DEVICE / HOST of hn, en, khn, ken (src/EMWAVE:4-23).  Each is
@@ -282,10 +283,17 @@ int32_t nkc_adjoint_get_drude(nkc_handle *h, double *ga, double *gb);
  * 70 % of the free device memory, or at most max_steps when that is > 0
  * (0 restores the default); it applies from the next nkc_adjoint_forward.
  * When whole segments fit (S = every, no cap), the forward keeps the stages
- * of its last K segments, K as many as 80 % of the free memory holds, and
- * the backward sweep recomputes only the others.
+ * of its last K segments, K as many as half of the free memory holds, and
+ * the backward sweep recomputes only the others.  So by default the adjoint
+ * may hold up to about half of the device's free memory from
+ * nkc_adjoint_forward until nkc_adjoint_release, the next forward, a
+ * set_* call or nkc_destroy; a cap (max_steps > 0) holds it to the
+ * checkpoints plus one window of 5 max_steps stages.
  * nkc_adjoint_info: out[5] = {every, segments, S, K, device bytes held}. */
 int32_t nkc_adjoint_set_store(nkc_handle *h, int64_t max_steps);
+/* Free everything the adjoint holds (checkpoints, stage store, adjoint
+ * state, gradients); a backward sweep then needs a new forward. */
+int32_t nkc_adjoint_release(nkc_handle *h);
 /* material = 1 (default): the backward sweep returns dL/deps and dL/dmu; 0:
  * it skips them (zeros are returned), which drops the stage residual from
  * the store (half its size: larger windows) and from the adjoint stage;

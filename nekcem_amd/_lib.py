@@ -94,6 +94,7 @@ SIGNATURES = [
     ("nkc_batch_layout", C.c_int32, [C.c_void_p, C.POINTER(C.c_int32), C.c_int64]),
     ("nkc_adjoint_set_store", C.c_int32, [C.c_void_p, C.c_int64]),
     ("nkc_adjoint_set_grads", C.c_int32, [C.c_void_p, C.c_int32]),
+    ("nkc_adjoint_release", C.c_int32, [C.c_void_p]),
     ("nkc_adjoint_info", C.c_int32, [C.c_void_p, C.POINTER(C.c_int64)]),
     ("nkc_adjoint_apply", C.c_int32, [C.c_void_p, _dp, _dp, _dp, _dp, C.c_int64]),
     ("nkc_create_hex", C.c_int32, [C.c_int32, C.c_int32, C.c_int64, _dp, _dp, _dp, C.c_char_p, _dp, _dp, _dp,

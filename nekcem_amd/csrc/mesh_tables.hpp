@@ -14,6 +14,7 @@
 // (src/cem_maxwell.F:970-975) from the impedance averages of
 // cem_maxwell_materials (src/cem_maxwell.F:262-325), and 1/eps, 1/mu.
 #pragma once
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <string>
@@ -551,6 +552,97 @@ inline int32_t build_hex(int n1, int ifupwind, int64_t nelt, const double *xc, c
             flux_coef(0.5 * Y0, Y0 - Ym, 0.5 * Z0, Z0 - Zm, C0, &m.fco[(e * 6 + kf) * 4]);
         }
     }
+    return NKC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// PML layers and sigma as cem_maxwell_init sets them up from the faces whose
+// boundary code is 'PML' (pml_face[6 e + f - 1], reference face order):
+// pml_errchk (src/cem_maxwell_pml.F:2-29), the layers `thick` elements
+// inward from each such wall along its axis (march_faces on a conforming
+// axis-aligned mesh), pml_extent_and_tags (:261-431) for the inner / outer
+// plane of each of the six layers, and pml_calc_sigma (:433-505):
+// sigma_axis = sigmamax ((x - inner) / (outer - inner))^order with
+// sigmamax = -(order + 1) log(referr) / (2 eta width), eta = sqrt(mu/eps).
+// xyz: node coordinates, 3 components at stride fs.  Out: the PML elements
+// (ascending), sigma (3 n3 per PML element, component-major), extents
+// (pmlinner[6] then pmlouter[6], order x-, x+, y-, y+, z-, z+).
+// ---------------------------------------------------------------------------
+inline int32_t pml_tables(const HostMesh &hm, const double *xyz, int64_t fs, const int8_t *pml_face, int thick,
+                          double order, double referr, std::vector<int64_t> &elems, std::vector<double> &sig,
+                          double extents[12], std::string &err) {
+    char buf[200];
+    auto bad = [&](const char *msg) { err = msg; return (int32_t)NKC_ERR_ARG; };
+    if (thick < 1 || thick > 10) { snprintf(buf, sizeof buf, "PML thickness %d not in 1..10", thick); return bad(buf); }
+    if (!(order >= 1.0 && order <= 10.0)) { snprintf(buf, sizeof buf, "PML order %g not in 1..10", order); return bad(buf); }
+    if (!(referr > 0.0 && referr <= 1.0)) {
+        snprintf(buf, sizeof buf, "PML reflection error %g not in (0, 1]", referr);
+        return bad(buf);
+    }
+    const int64_t nelt = hm.nelt, n3 = (int64_t)hm.n1 * hm.n1 * hm.n1;
+    const int *kf_ed = kf_of_ed();
+    std::vector<int32_t> tag(nelt, 0);
+    for (int64_t e = 0; e < nelt; ++e)
+        for (int f = 1; f <= 6; ++f) {
+            if (!pml_face[e * 6 + f - 1]) continue;
+            const int kf = kf_ed[f];
+            if (hm.nbr[e * 6 + kf] >= 0) {
+                snprintf(buf, sizeof buf, "PML face %d of element %lld is not a wall", f, (long long)e + 1);
+                return bad(buf);
+            }
+            int64_t cur = e;
+            for (int l = 0; l < thick; ++l) {
+                tag[cur] |= 1 << kf;
+                if ((tag[cur] >> (kf ^ 1)) & 1) {
+                    snprintf(buf, sizeof buf, "PML layers collide in element %lld", (long long)cur + 1);
+                    return bad(buf);
+                }
+                if (l + 1 == thick) break;
+                const int32_t nb = hm.nbr[cur * 6 + (kf ^ 1)];
+                if (nb < 0) {
+                    snprintf(buf, sizeof buf, "the PML is thicker than the mesh at element %lld", (long long)cur + 1);
+                    return bad(buf);
+                }
+                cur = nb;
+            }
+        }
+    const double inf = 1e20;
+    double inner[6] = {-inf, inf, -inf, inf, -inf, inf}, outer[6] = {inf, -inf, inf, -inf, inf, -inf};
+    elems.clear();
+    for (int64_t e = 0; e < nelt; ++e) {
+        if (!tag[e]) continue;
+        elems.push_back(e);
+        for (int f = 0; f < 6; ++f) {
+            if (!((tag[e] >> f) & 1)) continue;
+            const int ax = f / 2;
+            double lo = inf, hi = -inf;
+            for (int64_t v = 0; v < n3; ++v) {
+                const double x = xyz[ax * fs + e * n3 + v];
+                lo = std::min(lo, x);
+                hi = std::max(hi, x);
+            }
+            if (f & 1) { inner[f] = std::min(inner[f], lo); outer[f] = std::max(outer[f], hi); }
+            else       { inner[f] = std::max(inner[f], hi); outer[f] = std::min(outer[f], lo); }
+        }
+    }
+    sig.assign(elems.size() * 3 * n3, 0.0);
+    for (size_t l = 0; l < elems.size(); ++l) {
+        const int64_t e = elems[l];
+        for (int f = 0; f < 6; ++f) {
+            if (!((tag[e] >> f) & 1)) continue;
+            const int ax = f / 2;
+            const double width = std::fabs(outer[f] - inner[f]);
+            for (int64_t v = 0; v < n3; ++v) {
+                const int64_t g = e * n3 + v;
+                const size_t mi = hm.mat ? (size_t)g : (size_t)e;
+                const double eta = std::sqrt((1.0 / hm.imu[mi]) / (1.0 / hm.ieps[mi]));
+                const double smax = -(order + 1.0) * std::log(referr) / (2.0 * eta * width);
+                const double z01 = (xyz[ax * fs + g] - inner[f]) / (outer[f] - inner[f]);
+                sig[(l * 3 + ax) * n3 + v] = smax * std::pow(z01, order);
+            }
+        }
+    }
+    for (int f = 0; f < 6; ++f) { extents[f] = inner[f]; extents[6 + f] = outer[f]; }
     return NKC_OK;
 }
 

@@ -1455,77 +1455,18 @@ int32_t nkc_setup_pml(nkc_handle *h, const int8_t *pml_face, int32_t thick, doub
     if (!h) return fail(NKC_ERR_ARG, "null handle");
     int32_t rc = reset_pml(h);
     if (rc || !pml_face) return rc;
-    // pml_errchk (src/cem_maxwell_pml.F:2-29)
-    if (thick < 1 || thick > 10) return fail(NKC_ERR_ARG, "PML thickness %d not in 1..10", thick);
-    if (!(order >= 1.0 && order <= 10.0)) return fail(NKC_ERR_ARG, "PML order %g not in 1..10", order);
-    if (!(referr > 0.0 && referr <= 1.0)) return fail(NKC_ERR_ARG, "PML reflection error %g not in (0, 1]", referr);
     if ((rc = ensure_coords(h))) return rc;
-    const DevMesh &m = h->m;
-    const HostMesh &hm = h->hm;
-    const int64_t nelt = m.nelt, n3 = (int64_t)m.n1 * m.n1 * m.n1, fs = m.fs;
-    const int *kf_ed = kf_of_ed();
-    // the layers: `thick` elements inward from every PML face, along the
-    // face's axis (march_faces on a conforming axis-aligned mesh)
-    std::vector<int32_t> tag(nelt, 0);
-    for (int64_t e = 0; e < nelt; ++e)
-        for (int f = 1; f <= 6; ++f) {
-            if (!pml_face[e * 6 + f - 1]) continue;
-            const int kf = kf_ed[f];
-            if (hm.nbr[e * 6 + kf] >= 0) return fail(NKC_ERR_ARG, "PML face %d of element %lld is not a wall", f, (long long)e + 1);
-            int64_t cur = e;
-            for (int l = 0; l < thick; ++l) {
-                tag[cur] |= 1 << kf;
-                if ((tag[cur] >> (kf ^ 1)) & 1) return fail(NKC_ERR_ARG, "PML layers collide in element %lld", (long long)cur + 1);
-                if (l + 1 == thick) break;
-                const int32_t nb = hm.nbr[cur * 6 + (kf ^ 1)];
-                if (nb < 0) return fail(NKC_ERR_ARG, "the PML is thicker than the mesh at element %lld", (long long)cur + 1);
-                cur = nb;
-            }
-        }
-    // pml_extent_and_tags (src/cem_maxwell_pml.F:261-431): inner / outer
-    // coordinate of each of the six layers
-    const double inf = 1e20;
-    double inner[6] = {-inf, inf, -inf, inf, -inf, inf}, outer[6] = {inf, -inf, inf, -inf, inf, -inf};
+    // pml_errchk, the layers, their extents and sigma (mesh_tables.hpp)
     std::vector<int64_t> elems;
-    for (int64_t e = 0; e < nelt; ++e) {
-        if (!tag[e]) continue;
-        elems.push_back(e);
-        for (int f = 0; f < 6; ++f) {
-            if (!((tag[e] >> f) & 1)) continue;
-            const int ax = f / 2;
-            double lo = inf, hi = -inf;
-            for (int64_t v = 0; v < n3; ++v) {
-                const double x = h->hxyz[ax * fs + e * n3 + v];
-                lo = std::min(lo, x);
-                hi = std::max(hi, x);
-            }
-            if (f & 1) { inner[f] = std::min(inner[f], lo); outer[f] = std::max(outer[f], hi); }
-            else       { inner[f] = std::max(inner[f], hi); outer[f] = std::min(outer[f], lo); }
-        }
-    }
-    // pml_calc_sigma (src/cem_maxwell_pml.F:433-505): sigma_axis =
-    // sigmamax ((x - inner) / (outer - inner))^order, sigmamax =
-    // -(order + 1) log(referr) / (2 eta width), eta = sqrt(mu / eps)
-    std::vector<double> sig(elems.size() * 3 * n3, 0.0), P(elems.size() * 6 * n3, 0.0);
-    for (size_t l = 0; l < elems.size(); ++l) {
-        const int64_t e = elems[l];
-        for (int f = 0; f < 6; ++f) {
-            if (!((tag[e] >> f) & 1)) continue;
-            const int ax = f / 2;
-            const double width = std::fabs(outer[f] - inner[f]);
-            for (int64_t v = 0; v < n3; ++v) {
-                const int64_t g = e * n3 + v;
-                const size_t mi = hm.mat ? (size_t)g : (size_t)e;
-                const double eta = std::sqrt((1.0 / hm.imu[mi]) / (1.0 / hm.ieps[mi]));
-                const double smax = -(order + 1.0) * std::log(referr) / (2.0 * eta * width);
-                const double z01 = (h->hxyz[ax * fs + g] - inner[f]) / (outer[f] - inner[f]);
-                sig[(l * 3 + ax) * n3 + v] = smax * std::pow(z01, order);
-            }
-        }
-    }
+    std::vector<double> sig;
+    double ext[12];
+    std::string err;
+    if ((rc = pml_tables(h->hm, h->hxyz.data(), h->m.fs, pml_face, thick, order, referr, elems, sig, ext, err)))
+        return fail(rc, "%s", err.c_str());
     if (extents)
-        for (int f = 0; f < 6; ++f) { extents[f] = inner[f]; extents[6 + f] = outer[f]; }
+        for (int f = 0; f < 12; ++f) extents[f] = ext[f];
     if (elems.empty()) return NKC_OK;
+    const std::vector<double> P(elems.size() * 6 * (size_t)h->m.n1 * h->m.n1 * h->m.n1, 0.0);
     return install_pml(h, elems, sig, P);
 }
 
@@ -1915,14 +1856,15 @@ int32_t alloc_adjoint(nkc_handle *h) {
     }
     A.S = S;
     A.nsub_max = (A.every + S - 1) / S;
-    // whole segments fit: keep as many of the last ones as 80 % of the free
-    // memory holds, so the backward sweep recomputes only the others
+    // whole segments fit: keep as many of the last ones as half of the free
+    // memory holds, so the backward sweep recomputes only the others (the
+    // other half stays free for co-resident allocations)
     int64_t K = 1;
     if (S == A.every && h->adj_store_cap == 0) {
         size_t fr = 0, tot = 0;
         HIPCHK(hipMemGetInfo(&fr, &tot));
         const size_t win = per_step * S + sizeof(double) * blk;
-        K = std::max<int64_t>(1, std::min<int64_t>(A.nseg, (int64_t)((0.8 * (double)fr) / (double)win)));
+        K = std::max<int64_t>(1, std::min<int64_t>(A.nseg, (int64_t)((0.5 * (double)fr) / (double)win)));
     }
     A.K = K;
     A.slot_id.assign(K, -1);
@@ -2056,6 +1998,13 @@ int32_t nkc_adjoint_set_store(nkc_handle *h, int64_t max_steps) {
     if (!h) return fail(NKC_ERR_ARG, "null handle");
     if (max_steps < 0) return fail(NKC_ERR_ARG, "max_steps < 0");
     h->adj_store_cap = max_steps;
+    return NKC_OK;
+}
+
+int32_t nkc_adjoint_release(nkc_handle *h) {
+    if (!h) return fail(NKC_ERR_ARG, "null handle");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    free_adjoint(h);
     return NKC_OK;
 }
 

@@ -527,6 +527,10 @@ class MaxwellSolver:
         """Cap the stage-store window of the adjoint (0: from free memory)."""
         _lib.check(self._L.nkc_adjoint_set_store(self._h, int(max_steps)))
 
+    def adjoint_release(self):
+        """Free the adjoint's checkpoints, stage store and gradients."""
+        _lib.check(self._L.nkc_adjoint_release(self._h))
+
     def adjoint_set_grads(self, material=True):
         """Whether the backward sweep computes dL/deps, dL/dmu (default) or
         skips them (Drude gradients only; half the stage store)."""

@@ -1,0 +1,293 @@
+// sanitize_host_tables.cpp -- CPU driver for the host-side table builders
+// of nekcem_amd/csrc/mesh_tables.hpp, built by tests/test_sanitize.py with
+// g++ -fsanitize=address,undefined -fno-sanitize-recover=all and run on:
+//
+//   ref_<i>.bin  the reference-layout arrays of a box (what cem_maxwell_init
+//                leaves in COMMON, written by nekcem_amd/mesh.py) plus the
+//                box description: compact_reference() on the arrays must give
+//                build_box()'s tables for the same box;
+//   hex_<i>.bin  element corners and boundary codes as NekCEM's readers fill
+//                them (nekcem_amd/rea.py on the reference's case files):
+//                build_hex();
+//   boxes built here for every order 3..10, periodic / PEC mixes and
+//                one-element boxes, the PML set-up (pml_tables) on their
+//                walls, and malformed inputs that must be refused with an
+//                error code, not read out of bounds.
+//
+// Prints one line per case and "ALL OK" at the end; any sanitizer report
+// aborts with a nonzero status.
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../nekcem_amd/csrc/mesh_tables.hpp"
+
+using namespace nkc;
+
+namespace {
+
+struct Reader {
+    std::vector<char> buf;
+    size_t pos = 0;
+    bool open(const std::string &path) {
+        FILE *f = std::fopen(path.c_str(), "rb");
+        if (!f) return false;
+        std::fseek(f, 0, SEEK_END);
+        const long n = std::ftell(f);
+        std::fseek(f, 0, SEEK_SET);
+        buf.resize((size_t)n);
+        const size_t got = std::fread(buf.data(), 1, buf.size(), f);
+        std::fclose(f);
+        return got == buf.size();
+    }
+    template <class T>
+    std::vector<T> take(size_t n) {
+        if (pos + n * sizeof(T) > buf.size()) {
+            std::fprintf(stderr, "truncated input\n");
+            std::exit(2);
+        }
+        std::vector<T> v(n);
+        if (n) std::memcpy(v.data(), buf.data() + pos, n * sizeof(T));
+        pos += n * sizeof(T);
+        return v;
+    }
+};
+
+double maxdiff(const std::vector<double> &a, const std::vector<double> &b) {
+    if (a.size() != b.size()) return 1e300;
+    double m = 0;
+    for (size_t i = 0; i < a.size(); ++i) m = std::max(m, std::fabs(a[i] - b[i]));
+    return m;
+}
+
+int fails = 0;
+
+void expect(bool ok, const char *what) {
+    std::printf("%s %s\n", ok ? "ok  " : "FAIL", what);
+    if (!ok) ++fails;
+}
+
+// Node coordinates of a box mesh (GLL lines per element, element-major).
+std::vector<double> box_xyz(const nkc_box_desc &d, const HostMesh &m, int64_t &fs) {
+    const int n1 = m.n1, n2 = n1 * n1, n3 = n2 * n1;
+    std::vector<double> z, w, D;
+    gll_basis(n1, z, w, D);
+    const int64_t npts = m.nelt * n3;
+    fs = npts;
+    std::vector<double> xyz(3 * npts);
+    for (int64_t p = 0; p < npts; ++p) {
+        const int64_t e = p / n3, r = p % n3;
+        const int64_t ex[3] = {e % d.nel[0], (e / d.nel[0]) % d.nel[1], e / ((int64_t)d.nel[0] * d.nel[1])};
+        const int ii[3] = {(int)(r % n1), (int)((r / n1) % n1), (int)(r / n2)};
+        for (int a = 0; a < 3; ++a) {
+            const double len = (d.hi[a] - d.lo[a]) / d.nel[a];
+            const double v0 = d.lo[a] + len * ex[a];
+            xyz[a * npts + p] = v0 + len * 0.5 * (1.0 + z[ii[a]]);
+        }
+    }
+    return xyz;
+}
+
+void ref_case(const std::string &path) {
+    Reader r;
+    if (!r.open(path)) { expect(false, ("read " + path).c_str()); return; }
+    auto hdr = r.take<int64_t>(10);   // N, ifupwind, nelt, ncempec, nel[3], pec[3]
+    auto ext = r.take<double>(6);     // lo[3], hi[3]
+    const int N = (int)hdr[0], n1 = N + 1, n2 = n1 * n1, n3 = n2 * n1;
+    const int64_t nelt = hdr[2], ncempec = hdr[3], npts = nelt * n3, nxzfl = 6 * n2 * nelt;
+    auto eps_e = r.take<double>(nelt);
+    auto dxm1 = r.take<double>(n2);
+    auto wxm1 = r.take<double>(n1);
+    std::vector<std::vector<double>> met;
+    for (int i = 0; i < 11; ++i) met.push_back(r.take<double>(npts));   // rx..tz, jac, bm
+    std::vector<std::vector<double>> fa;
+    for (int i = 0; i < 4; ++i) fa.push_back(r.take<double>(nxzfl));    // unx, uny, unz, area
+    auto epsn = r.take<double>(npts), mun = r.take<double>(npts);
+    std::vector<std::vector<double>> zy;
+    for (int i = 0; i < 4; ++i) zy.push_back(r.take<double>(nxzfl));    // Z0, Z1, Y0, Y1
+    auto cemface = r.take<int32_t>(nxzfl), partner = r.take<int32_t>(nxzfl);
+    auto cempec = r.take<int32_t>(ncempec);
+    nkc_desc d{};
+    d.N = N;
+    d.ifupwind = (int32_t)hdr[1];
+    d.nelt = nelt;
+    d.dxm1 = dxm1.data();
+    d.wxm1 = wxm1.data();
+    const double **mp[11] = {&d.rxmn, &d.rymn, &d.rzmn, &d.sxmn, &d.symn, &d.szmn, &d.txmn, &d.tymn, &d.tzmn,
+                             &d.jacm, &d.bmn};
+    for (int i = 0; i < 11; ++i) *mp[i] = met[i].data();
+    d.unxm = fa[0].data();
+    d.unym = fa[1].data();
+    d.unzm = fa[2].data();
+    d.aream = fa[3].data();
+    d.cemface = cemface.data();
+    d.face_partner = partner.data();
+    d.cempec = ncempec ? cempec.data() : nullptr;
+    d.ncempec = ncempec;
+    d.permittivity = epsn.data();
+    d.permeability = mun.data();
+    d.Z_0 = zy[0].data();
+    d.Z_1 = zy[1].data();
+    d.Y_0 = zy[2].data();
+    d.Y_1 = zy[3].data();
+    HostMesh a;
+    const int32_t rc = compact_reference(&d, a);
+    expect(rc == NKC_OK, (path + " compact_reference").c_str());
+    if (rc) { std::printf("     %s\n", a.err.c_str()); return; }
+    nkc_box_desc b{};
+    b.N = N;
+    b.ifupwind = d.ifupwind;
+    for (int k = 0; k < 3; ++k) {
+        b.nel[k] = (int32_t)hdr[4 + k];
+        b.pec[k] = (int32_t)hdr[7 + k];
+        b.lo[k] = ext[k];
+        b.hi[k] = ext[3 + k];
+    }
+    b.eps_elem = eps_e.data();
+    HostMesh c;
+    expect(build_box(&b, c) == NKC_OK, (path + " build_box").c_str());
+    const bool same = a.nbr == c.nbr && a.mat == c.mat && maxdiff(a.geo, c.geo) <= 1e-10 * 64 &&
+                      maxdiff(a.fco, c.fco) <= 1e-12 && maxdiff(a.ieps, c.ieps) <= 1e-14 &&
+                      maxdiff(a.jac, c.jac) <= 1e-12;
+    expect(same, (path + " reference arrays == box description").c_str());
+    // a corrupted cemface and an out-of-range PEC slot are refused
+    auto bad = cemface;
+    bad[nxzfl / 2] = (int32_t)npts + 5;
+    d.cemface = bad.data();
+    HostMesh x;
+    expect(compact_reference(&d, x) != NKC_OK, (path + " corrupted cemface refused").c_str());
+    d.cemface = cemface.data();
+    auto badp = partner;
+    badp[3] = (int32_t)nxzfl + 7;
+    d.face_partner = badp.data();
+    HostMesh y;
+    expect(compact_reference(&d, y) != NKC_OK, (path + " out-of-range partner refused").c_str());
+    d.face_partner = partner.data();
+    if (ncempec) {
+        auto bp = cempec;
+        bp[0] = (int32_t)nxzfl + 1;
+        d.cempec = bp.data();
+        HostMesh z;
+        expect(compact_reference(&d, z) != NKC_OK, (path + " out-of-range cempec refused").c_str());
+    }
+}
+
+void hex_case(const std::string &path) {
+    Reader r;
+    if (!r.open(path)) { expect(false, ("read " + path).c_str()); return; }
+    auto hdr = r.take<int64_t>(3);   // N, ifupwind, nelt
+    const int64_t nelt = hdr[2];
+    auto xc = r.take<double>(8 * nelt), yc = r.take<double>(8 * nelt), zc = r.take<double>(8 * nelt);
+    auto bc = r.take<double>(30 * nelt);
+    auto cbc = r.take<char>(18 * nelt);
+    HostMesh m;
+    const int32_t rc = build_hex((int)hdr[0] + 1, (int)hdr[1], nelt, xc.data(), yc.data(), zc.data(), cbc.data(),
+                                 bc.data(), nullptr, nullptr, m);
+    expect(rc == NKC_OK, (path + " build_hex").c_str());
+    if (rc) { std::printf("     %s\n", m.err.c_str()); return; }
+    int64_t paired = 0;
+    for (int32_t v : m.nbr) paired += v >= 0;
+    std::printf("     %lld elements, %lld paired faces\n", (long long)nelt, (long long)paired);
+    // a neighbour link past the element count and an unknown code are refused
+    for (int64_t q = 0; q < 6 * nelt; ++q) {
+        const std::string code(cbc.data() + 3 * q, 3);
+        if (code[0] == 'E' || code[0] == 'P') {
+            if (code == "PEC" || code == "PML") continue;
+            auto b2 = bc;
+            b2[5 * q] = (double)(nelt + 3);
+            HostMesh x;
+            expect(build_hex((int)hdr[0] + 1, 1, nelt, xc.data(), yc.data(), zc.data(), cbc.data(), b2.data(),
+                             nullptr, nullptr, x) != NKC_OK,
+                   (path + " link past the element count refused").c_str());
+            break;
+        }
+    }
+    auto c2 = cbc;
+    c2[0] = 'Z';
+    c2[1] = 'Z';
+    c2[2] = 'Z';
+    HostMesh x;
+    expect(build_hex((int)hdr[0] + 1, 1, nelt, xc.data(), yc.data(), zc.data(), c2.data(), bc.data(), nullptr,
+                     nullptr, x) != NKC_OK,
+           (path + " unknown boundary code refused").c_str());
+}
+
+void box_and_pml_cases() {
+    const int nels[4][3] = {{1, 1, 1}, {2, 3, 1}, {3, 2, 2}, {4, 4, 4}};
+    for (int N = 3; N <= 10; ++N)
+        for (const auto &ne : nels)
+            for (int pecmask = 0; pecmask < 8; pecmask += 3) {
+                nkc_box_desc d{};
+                d.N = N;
+                d.ifupwind = 1;
+                for (int a = 0; a < 3; ++a) {
+                    d.nel[a] = ne[a];
+                    d.pec[a] = (pecmask >> a) & 1;
+                    d.lo[a] = -0.5 * (a + 1);
+                    d.hi[a] = 1.0 + a;
+                }
+                std::vector<double> eps(ne[0] * ne[1] * ne[2]);
+                for (size_t e = 0; e < eps.size(); ++e) eps[e] = 1.0 + 0.25 * (double)(e % 7);
+                d.eps_elem = eps.data();
+                HostMesh m;
+                char what[96];
+                std::snprintf(what, sizeof what, "build_box N=%d nel=%dx%dx%d pec=%d", N, ne[0], ne[1], ne[2], pecmask);
+                const bool ok = build_box(&d, m) == NKC_OK && (int64_t)m.nbr.size() == 6 * m.nelt &&
+                                (int64_t)m.geo.size() == 9 * m.nelt;
+                expect(ok, what);
+                if (!ok || !d.pec[1]) continue;
+                // PML on the two y walls, as thick as the mesh allows
+                std::vector<int8_t> pf(6 * m.nelt, 0);
+                for (int64_t e = 0; e < m.nelt; ++e) {
+                    const int64_t ey = (e / ne[0]) % ne[1];
+                    if (ey == 0) pf[6 * e + 0] = 1;                   // reference face 1: s = -1
+                    if (ey == ne[1] - 1) pf[6 * e + 2] = 1;           // face 3: s = +1
+                }
+                int64_t fs = 0;
+                const std::vector<double> xyz = box_xyz(d, m, fs);
+                std::vector<int64_t> elems;
+                std::vector<double> sig;
+                double ext[12];
+                std::string err;
+                const int thick_ok = ne[1] / 2 > 0 ? ne[1] / 2 : 1;
+                const int32_t rc = pml_tables(m, xyz.data(), fs, pf.data(), thick_ok, 3.0, 1e-6, elems, sig, ext, err);
+                const bool collide = ne[1] == 1;   // both walls on one layer of elements
+                std::snprintf(what, sizeof what, "pml_tables N=%d nel=%dx%dx%d thick=%d", N, ne[0], ne[1], ne[2], thick_ok);
+                expect(collide ? rc != NKC_OK : (rc == NKC_OK && sig.size() == elems.size() * 3 * (size_t)m.n1 * m.n1 * m.n1),
+                       what);
+                // too thick for the mesh, and out-of-range parameters
+                expect(pml_tables(m, xyz.data(), fs, pf.data(), ne[1] + 1, 3.0, 1e-6, elems, sig, ext, err) != NKC_OK,
+                       "pml_tables thicker than the mesh refused");
+                expect(pml_tables(m, xyz.data(), fs, pf.data(), 1, 0.5, 1e-6, elems, sig, ext, err) != NKC_OK,
+                       "pml_tables order out of range refused");
+            }
+    // malformed boxes
+    nkc_box_desc d{};
+    d.N = 5;
+    d.nel[0] = 2; d.nel[1] = 0; d.nel[2] = 2;
+    d.hi[0] = d.hi[1] = d.hi[2] = 1.0;
+    HostMesh m;
+    expect(build_box(&d, m) != NKC_OK, "build_box zero elements refused");
+    d.nel[1] = 2;
+    d.hi[2] = -1.0;
+    expect(build_box(&d, m) != NKC_OK, "build_box negative extent refused");
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+    for (int i = 1; i < argc; ++i) {
+        const std::string a = argv[i];
+        if (a.find("ref_") != std::string::npos) ref_case(a);
+        else if (a.find("hex_") != std::string::npos) hex_case(a);
+    }
+    box_and_pml_cases();
+    if (fails) {
+        std::printf("%d FAILED\n", fails);
+        return 1;
+    }
+    std::printf("ALL OK\n");
+    return 0;
+}
