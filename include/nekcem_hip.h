@@ -341,6 +341,24 @@ int32_t nkc_cem_error_fields(nkc_handle *h, const double *hx, const double *ex, 
  * (from several threads or streams) race. */
 int32_t nkc_cem_error_noalloc(nkc_handle *h, int32_t sol, double time, double *out);
 
+/* ---- restart files ---------------------------------------------------
+ * nkc_restart_write writes the current state as cem_restart_out does
+ * (src/io.F:411-490 with src/io_co.c:115-554, one rank): a big-endian
+ * legacy-VTK binary unstructured grid -- the header line "istep dumpno nx1
+ * np time dt" (time and dt with six decimals, as %f prints them), the node
+ * coordinates as float32, the (N)^3 GLL sub-hexahedra of each element, then
+ * E and H as VECTORS of doubles (single = 0, param(87) = 0) or floats.
+ * Needs the node coordinates (box handles have them; others
+ * nkc_set_coords).  nkc_restart_read is restart_swap (src/io.F:637-788):
+ * E and H of a file written for this mesh become the state (k := 0) and
+ * the header's istep, dumpno, time and dt come back (restart_swap adds the
+ * time read to the run's time).  A file of another order or point count
+ * is refused. */
+int32_t nkc_restart_write(nkc_handle *h, const char *path, int32_t istep, int32_t dumpno, double time, double dt,
+                          int32_t single);
+int32_t nkc_restart_read(nkc_handle *h, const char *path, int32_t *istep, int32_t *dumpno, double *time,
+                         double *dt);
+
 /* chk_blowup (src/cem_common.F:1150-1165): *maxabs = max |u| over the six
  * fields; returns NKC_ERR_BLOWUP if a value is non-finite or above 1e20. */
 int32_t nkc_check_finite(nkc_handle *h, double *maxabs);

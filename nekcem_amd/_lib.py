@@ -104,6 +104,10 @@ SIGNATURES = [
     ("nkc_cem_error", C.c_int32, [C.c_void_p, C.c_int32, C.c_double, _dp]),
     ("nkc_cem_error_fields", C.c_int32, [C.c_void_p, _dp, _dp, C.c_int64, _dp]),
     ("nkc_cem_error_noalloc", C.c_int32, [C.c_void_p, C.c_int32, C.c_double, _dp]),
+    ("nkc_restart_write", C.c_int32, [C.c_void_p, C.c_char_p, C.c_int32, C.c_int32, C.c_double, C.c_double,
+                                      C.c_int32]),
+    ("nkc_restart_read", C.c_int32, [C.c_void_p, C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), _dp,
+                                     _dp]),
     ("nkc_check_finite", C.c_int32, [C.c_void_p, _dp]),
     ("nkc_sync", C.c_int32, [C.c_void_p]),
     ("nkc_stream", C.c_int32, [C.c_void_p, C.POINTER(C.c_void_p)]),

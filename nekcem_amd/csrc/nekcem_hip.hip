@@ -25,6 +25,7 @@
 #include "diag_kernels.hpp"
 #include "mesh_tables.hpp"
 #include "order_launch.hpp"
+#include "restart_io.hpp"
 #include "stage_kernel.hpp"
 
 using namespace nkc;
@@ -1678,6 +1679,48 @@ int32_t nkc_set_incident(nkc_handle *h, int64_t nface, const int32_t *faces, con
     }
     h->inc_om = omega;
     h->ninc = nface;
+    return NKC_OK;
+}
+
+int32_t nkc_restart_write(nkc_handle *h, const char *path, int32_t istep, int32_t dumpno, double time, double dt,
+                          int32_t single) {
+    if (!h || !path) return fail(NKC_ERR_ARG, "null handle or path");
+    int32_t rc;
+    if ((rc = ensure_coords(h))) return rc;
+    const DevMesh &m = h->m;
+    std::vector<double> u(6 * (size_t)m.fs);
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipMemcpy(u.data(), m.u[h->cur], sizeof(double) * u.size(), hipMemcpyDeviceToHost));
+    RestartHeader hd;
+    hd.istep = istep;
+    hd.dumpno = dumpno;
+    hd.nx1 = m.n1;
+    hd.np = 1;
+    hd.time = time;
+    hd.dt = dt;
+    std::string err;
+    if ((rc = restart_write(path, hd, m.n1, m.nelt, h->hxyz.data(), m.fs, u.data() + 3 * m.fs, u.data(), m.fs,
+                            single == 0, err)))
+        return fail(rc, "%s", err.c_str());
+    return NKC_OK;
+}
+
+int32_t nkc_restart_read(nkc_handle *h, const char *path, int32_t *istep, int32_t *dumpno, double *time, double *dt) {
+    if (!h || !path) return fail(NKC_ERR_ARG, "null handle or path");
+    const DevMesh &m = h->m;
+    std::vector<double> u(6 * (size_t)m.fs, 0.0);
+    RestartHeader hd;
+    std::string err;
+    int32_t rc;
+    if ((rc = restart_read(path, m.n1, m.nelt, hd, u.data() + 3 * m.fs, u.data(), m.fs, err)))
+        return fail(rc, "%s", err.c_str());
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipMemcpy(m.u[h->cur], u.data(), sizeof(double) * u.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(m.k, 0, sizeof(double) * u.size()));
+    if (istep) *istep = hd.istep;
+    if (dumpno) *dumpno = hd.dumpno;
+    if (time) *time = hd.time;
+    if (dt) *dt = hd.dt;
     return NKC_OK;
 }
 

@@ -16,6 +16,8 @@ c       cem_maxwell_hip_update_host the !$ACC UPDATE HOST of userchk and
 c                                   cem_out (src/io.F:193-195)
 c       cem_maxwell_hip_pml         the PML arrays of cem_maxwell_init,
 c                                   once, when ifpml
+c       cem_maxwell_hip_restart_out cem_restart_out (src/io.F:411-490)
+c       cem_maxwell_hip_restart_swap restart_swap (src/io.F:637-788)
 c       cem_maxwell_hip_end         acc_copy_all_out (src/cem_drive.F:482)
 c
 c     Any nonzero status ends the run through nkc_check (error stop 1),
@@ -149,6 +151,51 @@ c     with userini's pmlbn / pmldn (src/cem_drive.F:444-448).
       call nkc_check(nkc_set_pml(hnkc, int(maxpml, c_int64_t), pmlptr,
      $     pmlsigma, cem_hip_dptr(pmlbn), cem_hip_dptr(pmldn),
      $     int(lpts1, c_int64_t)), 'nkc_set_pml')
+      return
+      end
+
+c     cem_restart_out (src/io.F:411-490) for the device state: the file
+c     restart-mpi-binary-N1-t<dumpno>.vtk of getfilename_restart
+c     (src/io_mpiutil.c:41-60) in the vtk directory, doubles unless
+c     param(87) asks for floats.
+      subroutine cem_maxwell_hip_restart_out(dumpno)
+      use iso_c_binding
+      use nekcem_hip
+      use cem_hip_state
+      implicit none
+      include 'SIZE'
+      include 'TOTAL'
+      integer dumpno
+      character(len=64) fname
+      integer(c_int32_t) single
+      write(fname,'(a,i5.5,a)') 'vtk/restart-mpi-binary-N1-t',
+     $     dumpno,'.vtk'
+      single = 0
+      if (nint(param(87)).ne.0) single = 1
+      call nkc_check(nkc_restart_write(hnkc, trim(fname)//c_null_char,
+     $     int(istep, c_int32_t), int(dumpno, c_int32_t), time, dt,
+     $     single), 'nkc_restart_write')
+      return
+      end
+
+c     restart_swap (src/io.F:637-788): the state from the restart file of
+c     dump param(84); the header's step goes to irststep (INPUT) and its
+c     time is added to the run's time.
+      subroutine cem_maxwell_hip_restart_swap
+      use iso_c_binding
+      use nekcem_hip
+      use cem_hip_state
+      implicit none
+      include 'SIZE'
+      include 'TOTAL'
+      character(len=64) fname
+      integer(c_int32_t) idump
+      real(c_double) rtime, rdt
+      write(fname,'(a,i5.5,a)') 'vtk/restart-mpi-binary-N1-t',
+     $     nint(param(84)),'.vtk'
+      call nkc_check(nkc_restart_read(hnkc, trim(fname)//c_null_char,
+     $     irststep, idump, rtime, rdt), 'nkc_restart_read')
+      time = time + rtime
       return
       end
 

@@ -39,7 +39,8 @@ module nekcem_hip
        nkc_set_graphene, nkc_get_graphene, &
        nkc_adjoint_forward, nkc_adjoint_backward, nkc_adjoint_get_drude, &
        nkc_check_finite, nkc_sync, nkc_stream, nkc_info, &
-       nkc_set_graph, nkc_set_keep_k, nkc_last_error, nkc_check, nkc_error_message
+       nkc_set_graph, nkc_set_keep_k, nkc_restart_write, nkc_restart_read, &
+       nkc_last_error, nkc_check, nkc_error_message
 
   interface
      integer(c_int32_t) function nkc_create(d, h) bind(C, name='nkc_create')
@@ -262,6 +263,25 @@ module nekcem_hip
        type(c_ptr), value :: h
        integer(c_int32_t), value :: enable
      end function nkc_set_graph
+
+     ! restart file of cem_restart_out / restart_swap; path is NUL-terminated
+     integer(c_int32_t) function nkc_restart_write(h, path, istep, dumpno, time, dt, single) &
+          bind(C, name='nkc_restart_write')
+       import :: c_int32_t, c_ptr, c_double, c_char
+       type(c_ptr), value :: h
+       character(kind=c_char), intent(in) :: path(*)
+       integer(c_int32_t), value :: istep, dumpno, single
+       real(c_double), value :: time, dt
+     end function nkc_restart_write
+
+     integer(c_int32_t) function nkc_restart_read(h, path, istep, dumpno, time, dt) &
+          bind(C, name='nkc_restart_read')
+       import :: c_int32_t, c_ptr, c_double, c_char
+       type(c_ptr), value :: h
+       character(kind=c_char), intent(in) :: path(*)
+       integer(c_int32_t), intent(out) :: istep, dumpno
+       real(c_double), intent(out) :: time, dt
+     end function nkc_restart_read
 
      ! 1: khn/ken after nkc_step are rk4_upd's register (last stage writes it)
      integer(c_int32_t) function nkc_set_keep_k(h, keep) bind(C, name='nkc_set_keep_k')

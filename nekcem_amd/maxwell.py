@@ -506,6 +506,24 @@ class MaxwellSolver:
         _lib.check(self._L.nkc_cem_error_fields(self._h, _d(H), _d(E), n, _d(out)))
         return out[:6], out[6:]
 
+    def restart_out(self, path, dumpno=0, single=False):
+        """cem_restart_out: write the state to a restart file at path, with
+        the solver's istep, time and dt in its header."""
+        self._ensure_coords()
+        _lib.check(self._L.nkc_restart_write(self._h, str(path).encode(), int(self.istep), int(dumpno),
+                                             float(self.time), float(self.dt or 0.0), 1 if single else 0))
+
+    def restart_swap(self, path):
+        """restart_swap: the state from a restart file (k := 0); the header's
+        time is added to the solver's time as the reference does.  Returns
+        dict(istep, dumpno, time, dt) of the header."""
+        istep, dumpno = C.c_int32(), C.c_int32()
+        t, dt = C.c_double(), C.c_double()
+        _lib.check(self._L.nkc_restart_read(self._h, str(path).encode(), C.byref(istep), C.byref(dumpno),
+                                            C.byref(t), C.byref(dt)))
+        self.time = self.time + t.value
+        return dict(istep=istep.value, dumpno=dumpno.value, time=t.value, dt=dt.value)
+
     def set_layout(self, layout):
         """Force the stage-kernel layout (0..NLAYOUT-1), or -1 to re-tune."""
         _lib.check(self._L.nkc_set_layout(self._h, int(layout)))

@@ -684,3 +684,62 @@ def set_num_threads(n):
 
 def num_threads():
     return int(lib().orc_num_threads())
+
+
+# ---------------------------------------------------------------------------
+# Restart file (parity with the reference's writer unpinned: it needs MPI-IO
+# and is not run here; the layout is restated from the source and checked by
+# round trips and by the reference's own tests/restart semantics).
+# ---------------------------------------------------------------------------
+
+def restart_cells(n1, nelt):
+    """(ncell, 8) 0-based node ids of the GLL sub-hexahedra, in the order of
+    vtk_dump_header_swap (src/io_dumpvtk.F:660-681): per element eg, level
+    j = 1..n1-1 from nfcnt = (j-1) n1^2 + (eg-1) n1^3, the (n1-1)^2 squares
+    of the level stepping nfcnt by one and over each row's last node."""
+    n2, n3 = n1 * n1, n1 ** 3
+    out = []
+    for eg in range(1, nelt + 1):
+        for j in range(1, n1):
+            nfcnt = (j - 1) * n2 + (eg - 1) * n3
+            for _ in range((n1 - 1) * (n1 - 1)):
+                a = [nfcnt, nfcnt + 1, nfcnt + n1 + 1, nfcnt + n1]
+                out.append(a + [v + n2 for v in a])
+                nfcnt += 1
+                if (nfcnt + 1) % n1 == 0:
+                    nfcnt += 1
+    return np.array(out, dtype=np.int64).reshape(-1, 8)
+
+
+def restart_bytes(istep, dumpno, n1, time, dt, xyz, E, H, double=True, np_ranks=1):
+    """The bytes cem_restart_out writes for one rank (src/io.F:411-490;
+    writeheader4 src/io_co.c:115-148, writenodes4 :152-227, writecells4
+    :238-405, writefield4 / writefield4_double :412-554): header lines
+    printed with %d and %f, big-endian float32 node coordinates, 9 int32 per
+    cell (8 and the node ids), int32 cell types 12, then the E and H vectors
+    (big-endian float64, or float32), the last field's trailing " \\n" left
+    in the writer's buffer.  xyz, E, H: (3, npts)."""
+    xyz, E, H = (np.asarray(a, dtype=np.float64) for a in (xyz, E, H))
+    npts = xyz.shape[1]
+    cells = restart_cells(n1, npts // n1 ** 3)
+    ncell = len(cells)
+    b = bytearray()
+    b += b"# vtk DataFile Version 3.0 \n"
+    b += b"%d %d %d %d %f %f Electromagnetic Field  \n" % (istep, dumpno, n1, np_ranks, time, dt)
+    b += b"BINARY \n"
+    b += b"DATASET UNSTRUCTURED_GRID \n"
+    b += b"POINTS  %d  float  \n" % npts
+    b += xyz.T.astype(">f4").tobytes()
+    b += b"\nCELLS %d  %d \n" % (ncell, 9 * ncell)
+    conn = np.concatenate([np.full((ncell, 1), 8), cells], axis=1)
+    b += conn.astype(">i4").tobytes()
+    b += b"\nCELL_TYPES %d \n" % ncell
+    b += np.full(ncell, 12).astype(">i4").tobytes()
+    b += b"\nPOINT_DATA %d \n" % npts
+    kind, fmt = (b"double", ">f8") if double else (b"float", ">f4")
+    b += b"VECTORS %s  %s \n" % (b"E ", kind)
+    b += E.T.astype(fmt).tobytes()
+    b += b" \n"
+    b += b"VECTORS %s  %s \n" % (b"H ", kind)
+    b += H.T.astype(fmt).tobytes()
+    return bytes(b)

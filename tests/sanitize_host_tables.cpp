@@ -9,6 +9,11 @@
 //   hex_<i>.bin  element corners and boundary codes as NekCEM's readers fill
 //                them (nekcem_amd/rea.py on the reference's case files):
 //                build_hex();
+//   rst_<i>.bin  a state, node coordinates and header of a restart file,
+//                beside rst_<i>.vtk, the oracle's bytes for them
+//                (oracle/oracle.py restart_bytes): restart_write must give
+//                the same bytes, restart_read must give back the state, and
+//                truncated or mismatched files must be refused;
 //   boxes built here for every order 3..10, periodic / PEC mixes and
 //                one-element boxes, the PML set-up (pml_tables) on their
 //                walls, and malformed inputs that must be refused with an
@@ -23,6 +28,7 @@
 #include <vector>
 
 #include "../nekcem_amd/csrc/mesh_tables.hpp"
+#include "../nekcem_amd/csrc/restart_io.hpp"
 
 using namespace nkc;
 
@@ -214,6 +220,56 @@ void hex_case(const std::string &path) {
            (path + " unknown boundary code refused").c_str());
 }
 
+std::vector<char> slurp(const std::string &path) {
+    Reader r;
+    if (!r.open(path)) return {};
+    return r.buf;
+}
+
+void restart_case(const std::string &path) {
+    Reader r;
+    if (!r.open(path)) { expect(false, ("read " + path).c_str()); return; }
+    auto hdr = r.take<int64_t>(5);   // n1, nelt, istep, dumpno, double
+    auto td = r.take<double>(2);
+    const int n1 = (int)hdr[0];
+    const int64_t nelt = hdr[1], npts = (int64_t)n1 * n1 * n1 * nelt;
+    auto xyz = r.take<double>(3 * npts), E = r.take<double>(3 * npts), H = r.take<double>(3 * npts);
+    const std::string ref = path.substr(0, path.size() - 4) + ".vtk", out = path + ".out.vtk";
+    RestartHeader hd;
+    hd.istep = (int32_t)hdr[2];
+    hd.dumpno = (int32_t)hdr[3];
+    hd.nx1 = n1;
+    hd.time = td[0];
+    hd.dt = td[1];
+    std::string err;
+    expect(restart_write(out.c_str(), hd, n1, nelt, xyz.data(), npts, E.data(), H.data(), npts, hdr[4] != 0, err) ==
+               NKC_OK,
+           (path + " restart_write").c_str());
+    const std::vector<char> a = slurp(out), b = slurp(ref);
+    expect(!a.empty() && a == b, (path + " bytes equal the oracle's").c_str());
+    std::vector<double> E2(3 * npts, -1.0), H2(3 * npts, -1.0);
+    RestartHeader h2;
+    expect(restart_read(ref.c_str(), n1, nelt, h2, E2.data(), H2.data(), npts, err) == NKC_OK,
+           (path + " restart_read").c_str());
+    bool same = h2.istep == hd.istep && h2.dumpno == hd.dumpno && h2.nx1 == n1;
+    for (int64_t i = 0; i < 3 * npts && same; ++i) {
+        const double e = hdr[4] ? E[i] : (double)(float)E[i], h = hdr[4] ? H[i] : (double)(float)H[i];
+        same = E2[i] == e && H2[i] == h;
+    }
+    expect(same, (path + " read back equals the state").c_str());
+    // another order, and a truncated file, are refused
+    expect(restart_read(ref.c_str(), n1 + 1, nelt, h2, E2.data(), H2.data(), npts, err) != NKC_OK,
+           (path + " other order refused").c_str());
+    {
+        FILE *f = std::fopen(out.c_str(), "wb");
+        std::fwrite(b.data(), 1, b.size() / 2, f);
+        std::fclose(f);
+    }
+    expect(restart_read(out.c_str(), n1, nelt, h2, E2.data(), H2.data(), npts, err) != NKC_OK,
+           (path + " truncated file refused").c_str());
+    std::remove(out.c_str());
+}
+
 void box_and_pml_cases() {
     const int nels[4][3] = {{1, 1, 1}, {2, 3, 1}, {3, 2, 2}, {4, 4, 4}};
     for (int N = 3; N <= 10; ++N)
@@ -282,6 +338,7 @@ int main(int argc, char **argv) {
         const std::string a = argv[i];
         if (a.find("ref_") != std::string::npos) ref_case(a);
         else if (a.find("hex_") != std::string::npos) hex_case(a);
+        else if (a.find("rst_") != std::string::npos) restart_case(a);
     }
     box_and_pml_cases();
     if (fails) {

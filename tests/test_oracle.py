@@ -735,3 +735,48 @@ def test_3dgraphene_fails_with_planted_mistakes(oracle, broken):
     for t, l2, li in _graphene_errors(c, sol, G0, 100):
         bad |= bool(np.any(l2 > l2tol) or np.any(li > litol))
     assert bad
+
+
+def test_restart_cells_cover_the_gll_subcubes(oracle):
+    """restart_cells (vtk_dump_header_swap's connectivity): every cell's 8
+    nodes are the corners of one unit cube of the element's (i, j, k) GLL
+    index grid, in VTK hexahedron order (bottom face counter-clockwise, then
+    the top face above it), and the cells cover each of the N^3 sub-cubes of
+    every element exactly once."""
+    for n1, nelt in ((2, 1), (3, 2), (5, 3), (8, 1)):
+        c = oracle.restart_cells(n1, nelt)
+        assert c.shape == ((n1 - 1) ** 3 * nelt, 8)
+        e, r = np.divmod(c, n1 ** 3)
+        i, j, k = r % n1, (r // n1) % n1, r // (n1 * n1)
+        assert np.all(e == e[:, :1])
+        d = np.stack([i - i[:, :1], j - j[:, :1], k - k[:, :1]], axis=2)
+        corners = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0],
+                            [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]])
+        assert np.all(d == corners[None])
+        base = (e[:, 0] * n1 ** 3 + r[:, 0])
+        assert len(np.unique(base)) == len(base)
+        assert np.all((i[:, 0] < n1 - 1) & (j[:, 0] < n1 - 1) & (k[:, 0] < n1 - 1))
+
+
+def test_restart_bytes_layout(oracle):
+    """restart_bytes: header lines as writeheader4 prints them (%f: six
+    decimals), block sizes from the counts, big-endian data."""
+    rng = np.random.default_rng(0)
+    n1, nelt = 3, 2
+    npts = nelt * n1 ** 3
+    xyz, E, H = (rng.standard_normal((3, npts)) for _ in range(3))
+    b = oracle.restart_bytes(5, 2, n1, 1.23456789, 1e-3, xyz, E, H)
+    lines = b.split(b"\n", 5)
+    assert lines[0] == b"# vtk DataFile Version 3.0 "
+    assert lines[1] == b"5 2 3 1 1.234568 0.001000 Electromagnetic Field  "
+    assert lines[2] == b"BINARY " and lines[3] == b"DATASET UNSTRUCTURED_GRID "
+    assert lines[4] == b"POINTS  %d  float  " % npts
+    ncell = 8 * nelt
+    size = (sum(len(l) + 1 for l in lines[:5]) + 12 * npts + len(b"\nCELLS %d  %d \n" % (ncell, 9 * ncell))
+            + 36 * ncell + len(b"\nCELL_TYPES %d \n" % ncell) + 4 * ncell + len(b"\nPOINT_DATA %d \n" % npts)
+            + 2 * len(b"VECTORS E   double \n") + 2 + 2 * 24 * npts)
+    assert len(b) == size
+    off = sum(len(l) + 1 for l in lines[:5])
+    x = np.frombuffer(b[off:off + 12 * npts], dtype=">f4").reshape(npts, 3).T
+    assert np.array_equal(x, xyz.astype(np.float32))
+    assert b.endswith(H.T.astype(">f8").tobytes())

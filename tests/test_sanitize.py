@@ -5,6 +5,8 @@ build_hex, pml_tables) and is built with g++ -fsanitize=address,undefined
 -fno-sanitize-recover=all.  It runs on the reference-layout arrays of four
 boxes (written here from nekcem_amd/mesh.py), on the element corners and
 boundary codes of the reference's own case files (read by nekcem_amd/rea.py),
+on restart files (restart_io.hpp's writer against the oracle's restatement
+of the layout, byte for byte, and its reader),
 on boxes of every order 3..10 with the PML set-up on their walls, and on
 malformed inputs that must be refused; any out-of-bounds access, leak or
 undefined behaviour aborts the run."""
@@ -18,6 +20,7 @@ import pytest
 
 from nekcem_amd import rea
 from nekcem_amd.mesh import BoxMesh
+from oracle import oracle as O
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CASES = os.path.join(HERE, "golden", "cases")
@@ -67,6 +70,20 @@ def _write_hex(path, case):
         f.write(codes)
 
 
+def _write_restart(base, n1, nelt, double, seed):
+    rng = np.random.default_rng(seed)
+    npts = nelt * n1 ** 3
+    xyz, E, H = (rng.standard_normal((3, npts)) for _ in range(3))
+    istep, dumpno, t, dt = 1234, 7, 0.1234567891, 2.5e-4
+    with open(base + ".bin", "wb") as f:
+        f.write(np.array([n1, nelt, istep, dumpno, 1 if double else 0], dtype=np.int64).tobytes())
+        f.write(np.array([t, dt], dtype=np.float64).tobytes())
+        for a in (xyz, E, H):
+            f.write(np.ascontiguousarray(a).tobytes())
+    with open(base + ".vtk", "wb") as f:
+        f.write(O.restart_bytes(istep, dumpno, n1, t, dt, xyz, E, H, double=double))
+
+
 @pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
 def test_host_tables_under_asan_ubsan():
     with tempfile.TemporaryDirectory() as tmp:
@@ -81,13 +98,16 @@ def test_host_tables_under_asan_ubsan():
             p = os.path.join(tmp, f"ref_{i}.bin")
             _write_ref(p, case, i)
             args.append(p)
+        for i, (n1, nelt, dbl) in enumerate(((4, 3, True), (8, 2, True), (6, 1, False))):
+            p = os.path.join(tmp, f"rst_{i}")
+            _write_restart(p, n1, nelt, dbl, i)
+            args.append(p + ".bin")
         for case in HEX_CASES:
             p = os.path.join(tmp, f"hex_{case}.bin")
             _write_hex(p, case)
             args.append(p)
         env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=1",
                    UBSAN_OPTIONS="print_stacktrace=1:halt_on_error=1")
-        env.pop("LD_PRELOAD", None)
         r = subprocess.run([exe] + args, capture_output=True, text=True, env=env, timeout=600)
         out = r.stdout + r.stderr
         assert r.returncode == 0, out[-4000:]
