@@ -427,6 +427,8 @@ def main():
     ap.add_argument("--cpu-steps-1core", type=int, default=1,
                     help="steps of the one-thread CPU oracle sample (0 = skip)")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--layout", type=int, default=-1,
+                    help="stage-kernel layout of the main workload (-1: the autotuner picks)")
     ap.add_argument("--batched-steps", type=int, default=50,
                     help="steps of the batched-cavity speedup measurement (0 = skip)")
     ap.add_argument("--batched-only", action="store_true")
@@ -475,6 +477,8 @@ def main():
     print(f"[bench] setup {setup_s:.1f}s npts={npts} nelt={mesh.nelt} dt={dt:.3e} "
           f"info={solver.info()}", file=sys.stderr, flush=True)
 
+    if args.layout >= 0:
+        solver.set_layout(args.layout)
     solver.step(dt, args.warmup)
     solver.sync()
     layout = solver.layout()

@@ -41,7 +41,7 @@ def hipcc():
 
 
 def sources():
-    return [SRC] + [os.path.join(CSRC, f"order_n{n}.hip") for n in ORDERS]
+    return [SRC, os.path.join(CSRC, "geo_inst.hip")] + [os.path.join(CSRC, f"order_n{n}.hip") for n in ORDERS]
 
 
 def _compile(src, extra):

@@ -48,11 +48,12 @@ constexpr int DIAG_BS = 256;
 constexpr int DIAG_NV = 13;      // 6 sums of bm err^2, 6 maxima of |err|, sum of bm
 
 // Per-workgroup partial norms (grid-stride over the nodes, a fixed-order
-// tree in LDS), so the final sums do not depend on timing.
+// tree in LDS), so the final sums do not depend on timing.  bm1 = J w3 with
+// the Jacobian per element, or per node (jac_node) on a general mesh.
 __global__ void __launch_bounds__(DIAG_BS)
 error_partials(int sol, double t, const double *__restrict__ u, const double *__restrict__ exact, int64_t fs,
                int64_t npts, int n1, const double *__restrict__ xyz, const double *__restrict__ jac,
-               const double *__restrict__ w, double *__restrict__ part) {
+               const double *__restrict__ w, int jac_node, double *__restrict__ part) {
     __shared__ double red[DIAG_NV][DIAG_BS];
     const int n2 = n1 * n1, n3 = n2 * n1;
     double acc[DIAG_NV];
@@ -62,7 +63,7 @@ error_partials(int sol, double t, const double *__restrict__ u, const double *__
         const int64_t e = p / n3;
         const int r = (int)(p - e * n3);
         const int i = r % n1, j = (r / n1) % n1, k = r / n2;
-        const double bm = jac[e] * (w[i] * w[j] * w[k]);
+        const double bm = jac[jac_node ? p : e] * (w[i] * w[j] * w[k]);
         double ex[6];
         if (sol == SOL_FIELDS) {
             for (int c = 0; c < 6; ++c) ex[c] = exact[c * fs + p];

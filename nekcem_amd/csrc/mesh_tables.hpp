@@ -41,7 +41,18 @@ struct HostMesh {
     std::vector<int32_t> nbr;        // nelt*6 (kernel face order)
     std::vector<double> fco;         // nelt*6*4 or nelt*6*n2*4
     std::vector<double> ieps, imu;   // nelt or npts
-    std::vector<double> jac;         // nelt (element Jacobian)
+    std::vector<double> jac;         // nelt (element Jacobian); per node on a general mesh
+    // General (curved, rotated or sheared) hexahedra, read by the
+    // per-node-geometry kernel: the nine metric quotients G_ab = (cofactor
+    // ab)/J per node (rx, ry, rz, sx, sy, sz, tx, ty, tz over J, component
+    // major), per face node in the reference's face order the unit normal
+    // and the lift factor area/(J w3) of its volume node (4 doubles), the
+    // partner's volume node (0-based; HOST_FACE_PEC / HOST_FACE_NONE), and
+    // bm1 = J w3 per node.  fco then holds each face node's flux
+    // coefficients in the reference's face order, ieps / imu per node.
+    bool general = false;
+    std::vector<double> gmet, fgeo, bmw;
+    std::vector<int32_t> fnb;
     std::string err;
 };
 
@@ -75,7 +86,7 @@ inline void flux_coef(double Y0, double Y1, double Z0, double Z1, double C0, dou
 // ---------------------------------------------------------------------------
 // from the reference arrays
 // ---------------------------------------------------------------------------
-inline int32_t compact_reference(const nkc_desc *d, HostMesh &m) {
+inline int32_t compact_box(const nkc_desc *d, HostMesh &m) {
     const int n1 = d->N + 1, n2 = n1 * n1, n3 = n2 * n1;
     const int64_t nelt = d->nelt, npts = (int64_t)n3 * nelt, nxzfl = (int64_t)6 * n2 * nelt;
     const double C0 = d->ifupwind ? 1.0 : 0.0;
@@ -223,6 +234,125 @@ inline int32_t compact_reference(const nkc_desc *d, HostMesh &m) {
             m.ieps[p] = 1.0 / d->permittivity[p];
             m.imu[p] = 1.0 / d->permeability[p];
         }
+    return NKC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// General hexahedra from the reference arrays: any element geometry the
+// reference's GLMAPM1 / AREA3 produce (curved, rotated, sheared) and any
+// conforming face pairing gsh_face describes (face_partner may match a face
+// node to any node of the facing face).  Per node the metric quotients and
+// bm1, per face node (reference face order) the normal, the lift factor
+// area/bm1 of its volume node, the partner's volume node and the flux
+// coefficients of cem_maxwell_flux3d from the Y / Z arrays.
+// ---------------------------------------------------------------------------
+inline int32_t compact_general(const nkc_desc *d, HostMesh &m) {
+    const int n1 = d->N + 1, n2 = n1 * n1, n3 = n2 * n1;
+    const int64_t nelt = d->nelt, npts = (int64_t)n3 * nelt, nxzfl = (int64_t)6 * n2 * nelt;
+    const double C0 = d->ifupwind ? 1.0 : 0.0;
+    char buf[320];
+    auto bad = [&](int32_t code, const char *msg) { m.err = msg; return code; };
+    std::vector<int8_t> pecslot(nxzfl, 0);
+    for (int64_t l = 0; l < d->ncempec; ++l) {
+        const int64_t s = (int64_t)d->cempec[l] - 1;
+        if (s < 0 || s >= nxzfl) {
+            snprintf(buf, sizeof buf, "cempec[%lld]=%d out of range", (long long)l, d->cempec[l]);
+            return bad(NKC_ERR_ARG, buf);
+        }
+        pecslot[s] = 1;
+    }
+    m.n1 = n1;
+    m.nelt = nelt;
+    m.mat = 1;
+    m.general = true;
+    m.D.assign(d->dxm1, d->dxm1 + n2);
+    m.w.assign(d->wxm1, d->wxm1 + n1);
+    m.geo.assign(nelt * 9, 0.0);
+    m.nbr.assign(nelt * 6, HOST_FACE_NONE);
+    m.gmet.assign((size_t)9 * npts, 0.0);
+    m.bmw.assign(npts, 0.0);
+    m.jac.assign(npts, 0.0);
+    m.ieps.assign(npts, 0.0);
+    m.imu.assign(npts, 0.0);
+    const double *cof[9] = {d->rxmn, d->rymn, d->rzmn, d->sxmn, d->symn, d->szmn, d->txmn, d->tymn, d->tzmn};
+    for (int64_t p = 0; p < npts; ++p) {
+        const double J = d->jacm[p];
+        if (!(J > 0.0) || !(d->bmn[p] > 0.0)) {
+            snprintf(buf, sizeof buf, "node %lld: Jacobian %g or mass %g not positive", (long long)p, J, d->bmn[p]);
+            return bad(NKC_ERR_ARG, buf);
+        }
+        for (int c = 0; c < 9; ++c) m.gmet[(size_t)c * npts + p] = cof[c][p] / J;
+        m.bmw[p] = d->bmn[p];
+        m.jac[p] = J;
+        m.ieps[p] = 1.0 / d->permittivity[p];
+        m.imu[p] = 1.0 / d->permeability[p];
+    }
+    m.fgeo.assign((size_t)4 * nxzfl, 0.0);
+    m.fnb.assign(nxzfl, HOST_FACE_NONE);
+    m.fco.assign((size_t)4 * nxzfl, 0.0);
+    const int *kf_ed = kf_of_ed();
+    for (int64_t e = 0; e < nelt; ++e)
+        for (int ef = 1; ef <= 6; ++ef) {
+            const int kf = kf_ed[ef];
+            const int64_t s0 = e * 6 * n2 + (int64_t)(ef - 1) * n2;
+            bool any_partner = false;
+            for (int q = 0; q < n2; ++q) {
+                const int64_t s = s0 + q;
+                const int64_t v = (int64_t)d->cemface[s] - 1;
+                if (v != e * n3 + face_node(n1, kf, q)) {
+                    snprintf(buf, sizeof buf, "cemface[%lld]=%d does not follow the skpdat face order",
+                             (long long)s, d->cemface[s]);
+                    return bad(NKC_ERR_ARG, buf);
+                }
+                const double nv[3] = {d->unxm[s], d->unym[s], d->unzm[s]};
+                const double nn = std::sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+                if (!(std::fabs(nn - 1.0) < 1e-8)) {
+                    snprintf(buf, sizeof buf, "face node %lld: normal is not a unit vector", (long long)s);
+                    return bad(NKC_ERR_ARG, buf);
+                }
+                for (int c = 0; c < 3; ++c) m.fgeo[(size_t)4 * s + c] = nv[c];
+                m.fgeo[(size_t)4 * s + 3] = d->aream[s] / d->bmn[v];
+                const int64_t pr = (int64_t)d->face_partner[s] - 1;
+                int32_t nb;
+                if (pr >= 0) {
+                    if (pr >= nxzfl || pecslot[s]) {
+                        snprintf(buf, sizeof buf, "face node %lld: partner %lld out of range or on a PEC face",
+                                 (long long)s, (long long)pr);
+                        return bad(NKC_ERR_ARG, buf);
+                    }
+                    if ((int64_t)d->face_partner[pr] - 1 != s) {
+                        snprintf(buf, sizeof buf, "face node %lld: partner %lld does not pair back", (long long)s,
+                                 (long long)pr);
+                        return bad(NKC_ERR_ARG, buf);
+                    }
+                    nb = d->cemface[pr] - 1;
+                    any_partner = true;
+                    // the element across: the first paired node names it
+                    if (m.nbr[e * 6 + kf] == HOST_FACE_NONE) m.nbr[e * 6 + kf] = (int32_t)(pr / (6 * n2));
+                } else {
+                    nb = pecslot[s] ? HOST_FACE_PEC : HOST_FACE_NONE;
+                }
+                m.fnb[s] = nb;
+                flux_coef(d->Y_0[s], d->Y_1[s], d->Z_0[s], d->Z_1[s], C0, &m.fco[(size_t)4 * s]);
+            }
+            if (!any_partner && pecslot[s0]) m.nbr[e * 6 + kf] = HOST_FACE_PEC;
+        }
+    return NKC_OK;
+}
+
+// The reference arrays of nkc_create: axis-aligned affine boxes paired
+// face to face go to the compact per-element tables of the tuned kernels,
+// anything else to the per-node tables of the general kernel.
+inline int32_t compact_reference(const nkc_desc *d, HostMesh &m) {
+    const int32_t rc = compact_box(d, m);
+    if (rc != NKC_ERR_UNSUPPORTED) return rc;
+    HostMesh g;
+    const int32_t rg = compact_general(d, g);
+    if (rg) {
+        m.err = g.err;
+        return rg;
+    }
+    m = std::move(g);
     return NKC_OK;
 }
 

@@ -23,6 +23,8 @@
 #include "../../include/nekcem_hip.h"
 #include "adjoint_kernel.hpp"
 #include "diag_kernels.hpp"
+#include "geo_kernel.hpp"
+#include "geo_launch.hpp"
 #include "mesh_tables.hpp"
 #include "order_launch.hpp"
 #include "restart_io.hpp"
@@ -71,6 +73,8 @@ const double RK4C[5] = {0.0,
                         2802321613138.0 / 2924317926251.0};
 
 bool supported_n1(int n1) { return n1 >= 4 && n1 <= 10; }
+// Orders the general-hexahedra kernel takes (nkc_create only).
+bool supported_n1_general(int n1) { return n1 >= GEO_MIN_N1 && n1 <= GEO_MAX_N1; }
 
 int32_t dev_alloc(double **p, size_t n) {
     HIPCHK(hipMalloc((void **)p, sizeof(double) * n));
@@ -103,6 +107,11 @@ struct DevMesh {
     int32_t *nbr = nullptr;
     int layout = -1;              // stage-kernel layout (PACK | OCC << 1); -1: not tuned yet
     std::vector<double> jac, w;   // host copies (source weights)
+    // general hexahedra (geo_kernel.hpp): per-node metric quotients and
+    // bm1, per-face-node normal + lift, partner node; fco per face node
+    bool general = false;
+    double *gmet = nullptr, *fgeo = nullptr, *bmw = nullptr;
+    int32_t *fnb = nullptr;
 };
 
 int32_t dev_mesh_upload(const HostMesh &m, DevMesh &dm) {
@@ -127,13 +136,22 @@ int32_t dev_mesh_upload(const HostMesh &m, DevMesh &dm) {
     if ((rc = upload(&dm.fco, m.fco))) return rc;
     if ((rc = upload(&dm.ieps, m.ieps))) return rc;
     if ((rc = upload(&dm.imu, m.imu))) return rc;
+    dm.general = m.general;
+    if (m.general) {
+        if ((rc = upload(&dm.gmet, m.gmet))) return rc;
+        if ((rc = upload(&dm.fgeo, m.fgeo))) return rc;
+        if ((rc = upload(&dm.bmw, m.bmw))) return rc;
+        if ((rc = upload(&dm.fnb, m.fnb))) return rc;
+        dm.layout = 0;            // one layout: nothing to tune
+    }
     return NKC_OK;
 }
 
 void dev_mesh_free(DevMesh &dm) {
-    double *p[] = {dm.u[0], dm.u[1], dm.k, dm.D, dm.geo, dm.fco, dm.ieps, dm.imu, dm.dt_e};
+    double *p[] = {dm.u[0], dm.u[1], dm.k, dm.D, dm.geo, dm.fco, dm.ieps, dm.imu, dm.dt_e, dm.gmet, dm.fgeo, dm.bmw};
     for (double *q : p) (void)hipFree(q);
     (void)hipFree(dm.nbr);
+    (void)hipFree(dm.fnb);
     dm = DevMesh();
 }
 
@@ -156,7 +174,45 @@ int stage_block(int n1, int layout) {
     return one * layout_epw(layout);
 }
 
+}  // namespace
+struct nkc_handle;
+namespace {
+// Entry points whose physics the general-hexahedra kernel does not carry.
+int32_t general_refuses(const nkc_handle *, const char *what) {
+    return fail(NKC_ERR_UNSUPPORTED, "%s is not built for general (non-box) hexahedra", what);
+}
+
+// One stage on a general mesh: the per-node-geometry kernel.
+int32_t launch_geo_stage(const DevMesh &dm, const StageArgs &a, int gmode, hipStream_t s) {
+    GeoArgs g{};
+    g.uin = a.uin;
+    g.uout = a.uout;
+    g.kbuf = a.kbuf;
+    g.res = a.res;
+    g.D = dm.D;
+    g.gmet = dm.gmet;
+    g.bmw = dm.bmw;
+    g.fgeo = dm.fgeo;
+    g.fnb = dm.fnb;
+    g.fco = dm.fco;
+    g.ieps = dm.ieps;
+    g.imu = dm.imu;
+    g.fs = dm.fs;
+    g.npts = dm.npts;
+    g.nelt = (int32_t)dm.nelt;
+    g.per_xcd = (int)((dm.nelt + 7) / 8);
+    g.rka = a.rka;
+    g.rkb = a.rkb;
+    g.dt = a.dt;
+    g.kflags = a.kflags;
+    if (!launch_geo(g, dm.n1, gmode, 8 * g.per_xcd, s))
+        return fail(NKC_ERR_UNSUPPORTED, "polynomial order N=%d not built for general hexahedra", dm.n1 - 1);
+    HIPCHK(hipGetLastError());
+    return NKC_OK;
+}
+
 int32_t launch_stage(const DevMesh &dm, const StageArgs &a, int mode, hipStream_t s, int layout = -2) {
+    if (dm.general) return launch_geo_stage(dm, a, mode == MODE_RK ? GMODE_RK : GMODE_RESIDUAL, s);
     const int pk0 = layout == -2 ? (dm.layout < 0 ? 0 : dm.layout) : layout;
     // the residual entry and orders without the layout run layout 0
     const int pk = (mode == MODE_RK && layout_built(dm.n1, pk0)) ? pk0 : 0;
@@ -212,6 +268,10 @@ int32_t stage_kflags(int s) {
 int32_t tune_stage(DevMesh &dm, const StageArgs *feat, size_t jsz, size_t lsz, size_t psz, size_t gsz,
                    const double *uin, hipStream_t s, double us[NLAYOUT]) {
     for (int l = 0; l < NLAYOUT; ++l) us[l] = NAN;
+    if (dm.general) {
+        dm.layout = 0;            // the general kernel has one layout
+        return NKC_OK;
+    }
     // not timed: a layout not built for this order, and a packed one-element
     // layout whose workgroup equals its unpacked twin
     auto skip = [&](int p) {
@@ -870,7 +930,7 @@ int32_t nkc_create(const nkc_desc *d, nkc_handle **out) {
     if (!d || !out) return fail(NKC_ERR_ARG, "null descriptor or output handle");
     *out = nullptr;
     const int n1 = d->N + 1;
-    if (!supported_n1(n1)) return fail(NKC_ERR_UNSUPPORTED, "polynomial order N=%d outside 3..9", d->N);
+    if (!supported_n1_general(n1)) return fail(NKC_ERR_UNSUPPORTED, "polynomial order N=%d outside 3..11", d->N);
     if (d->nelt <= 0 || d->nelt > (int64_t)1 << 30) return fail(NKC_ERR_ARG, "nelt=%lld out of range", (long long)d->nelt);
     const void *req[] = {d->dxm1, d->wxm1, d->rxmn, d->rymn, d->rzmn, d->sxmn, d->symn, d->szmn,
                          d->txmn, d->tymn, d->tzmn, d->jacm, d->bmn, d->unxm, d->unym, d->unzm,
@@ -884,6 +944,8 @@ int32_t nkc_create(const nkc_desc *d, nkc_handle **out) {
     HostMesh hm;
     const int32_t rc = compact_reference(d, hm);
     if (rc) return fail(rc, "%s", hm.err.c_str());
+    if (!hm.general && !supported_n1(n1))
+        return fail(NKC_ERR_UNSUPPORTED, "polynomial order N=%d: box meshes are built for N = 3..9", d->N);
     return make_handle(hm, d->ifupwind ? 1.0 : 0.0, out);
 }
 
@@ -1029,6 +1091,14 @@ int32_t nkc_curl(nkc_handle *h, double *reshn, double *resen, int64_t ld) {
     if ((rc = ensure_diag(h))) return rc;
     if (!h->res) HIPCHK(hipMalloc((void **)&h->res, sizeof(double) * 6 * m.fs));
     HIPCHK(hipStreamSynchronize(h->stream));
+    if (m.general) {
+        StageArgs a = mesh_args(m);
+        a.uin = m.u[h->cur];
+        a.res = h->res;
+        if ((rc = launch_geo_stage(m, a, GMODE_CURL, h->stream))) return rc;
+        HIPCHK(hipStreamSynchronize(h->stream));
+        return copy_fields_d2h(h->res, m.fs, 0, reshn, resen, m.npts, ld);
+    }
     const int bs = ((m.n1 * m.n1 * m.n1 + 63) / 64) * 64;
     const dim3 grid((unsigned)m.nelt), block((unsigned)bs);
     const double *u = m.u[h->cur];
@@ -1049,6 +1119,7 @@ int32_t nkc_curl(nkc_handle *h, double *reshn, double *resen, int64_t ld) {
 
 int32_t nkc_set_source(nkc_handle *h, const double *amp, const double dir[3], double t0, double tw) {
     if (!h) return fail(NKC_ERR_ARG, "null handle");
+    if (h->m.general && (amp != nullptr)) return general_refuses(h, "a source");
     HIPCHK(hipStreamSynchronize(h->stream));
     free_adjoint(h);                      // the stored trajectory has the old source
     h->m.layout = -1;                     // the step's kernel instance may change: tune again
@@ -1102,6 +1173,7 @@ int32_t nkc_set_source_time(nkc_handle *h, int32_t kind, double omega, double ph
 int32_t nkc_set_drude(nkc_handle *h, int64_t ndrude, const int32_t *dindex, const double *params,
                       const double *J, const double *kJ) {
     if (!h) return fail(NKC_ERR_ARG, "null handle");
+    if (h->m.general && (ndrude > 0)) return general_refuses(h, "a Drude region");
     HIPCHK(hipStreamSynchronize(h->stream));
     (void)hipFree(h->drude_off); h->drude_off = nullptr;
     (void)hipFree(h->dJ); h->dJ = nullptr;
@@ -1214,7 +1286,7 @@ int32_t error_launch(nkc_handle *h, int sol, double time, const double *exact, d
     const int nb = error_blocks(h);
     hipLaunchKernelGGL(error_partials, dim3(nb), dim3(DIAG_BS), 0, h->stream, sol, time, (const double *)m.u[h->cur],
                        exact, m.fs, m.npts, m.n1, (const double *)h->xyz, (const double *)h->djac,
-                       (const double *)h->dw, part);
+                       (const double *)h->dw, (int)m.general, part);
     hipLaunchKernelGGL(error_final, dim3(1), dim3(64), 0, h->stream, (const double *)part, nb, dout);
     HIPCHK(hipGetLastError());
     double v[DIAG_NV];
@@ -1319,6 +1391,7 @@ int32_t nkc_cem_error_noalloc(nkc_handle *h, int32_t sol, double time, double *o
 int32_t nkc_set_lorentz(nkc_handle *h, int64_t nlor, const int32_t *lindex, const double *params,
                         const double *L, const double *kL) {
     if (!h) return fail(NKC_ERR_ARG, "null handle");
+    if (h->m.general && (nlor > 0)) return general_refuses(h, "a Lorentz region");
     HIPCHK(hipStreamSynchronize(h->stream));
     free_adjoint(h);
     h->m.layout = -1;                     // the step's kernel instance may change: tune again
@@ -1426,6 +1499,7 @@ int32_t reset_pml(nkc_handle *h) {
 int32_t nkc_set_pml(nkc_handle *h, int64_t maxpml, const int32_t *pmlptr, const double *sigma, const double *bn,
                     const double *dn, int64_t ld) {
     if (!h) return fail(NKC_ERR_ARG, "null handle");
+    if (h->m.general && (maxpml > 0)) return general_refuses(h, "a PML");
     int32_t rc = reset_pml(h);
     if (rc || maxpml == 0) return rc;
     const DevMesh &m = h->m;
@@ -1454,6 +1528,7 @@ int32_t nkc_set_pml(nkc_handle *h, int64_t maxpml, const int32_t *pmlptr, const 
 int32_t nkc_setup_pml(nkc_handle *h, const int8_t *pml_face, int32_t thick, double order, double referr,
                       double *extents) {
     if (!h) return fail(NKC_ERR_ARG, "null handle");
+    if (h->m.general && (pml_face != nullptr)) return general_refuses(h, "a PML");
     int32_t rc = reset_pml(h);
     if (rc || !pml_face) return rc;
     if ((rc = ensure_coords(h))) return rc;
@@ -1558,6 +1633,7 @@ void free_sheet(nkc_handle *h) {
 int32_t nkc_set_graphene(nkc_handle *h, int64_t ngraph, const int32_t *graphindex, const double *graphparams,
                          const double *fjn, int64_t ld) {
     if (!h) return fail(NKC_ERR_ARG, "null handle");
+    if (h->m.general && (ngraph > 0)) return general_refuses(h, "a graphene sheet");
     HIPCHK(hipStreamSynchronize(h->stream));
     free_adjoint(h);
     h->m.layout = -1;                     // the step's kernel instance may change: tune again
@@ -1647,6 +1723,7 @@ int32_t nkc_get_graphene(nkc_handle *h, int32_t which, double *fjn, int64_t ld) 
 int32_t nkc_set_incident(nkc_handle *h, int64_t nface, const int32_t *faces, const double k[3], double omega,
                          const double hamp[3], const double eamp[3]) {
     if (!h) return fail(NKC_ERR_ARG, "null handle");
+    if (h->m.general && (nface > 0)) return general_refuses(h, "an incident field");
     HIPCHK(hipStreamSynchronize(h->stream));
     free_adjoint(h);
     h->m.layout = -1;                     // the step's kernel instance may change: tune again
@@ -1758,8 +1835,9 @@ int32_t nkc_info(nkc_handle *h, int64_t out[8]) {
     out[0] = h->m.npts;
     out[1] = h->m.mat;
     out[2] = 1;
-    out[3] = stage_block(n1, h->m.layout < 0 ? 0 : h->m.layout);
-    out[4] = (int64_t)sizeof(double) * (6 * n1 * n1 * n1 + 25 * n1 * n1);
+    out[3] = h->m.general ? GEO_BS : stage_block(n1, h->m.layout < 0 ? 0 : h->m.layout);
+    out[4] = h->m.general ? (int64_t)sizeof(double) * (6 * n1 * n1 * n1 + 37 * n1 * n1)
+                          : (int64_t)sizeof(double) * (6 * n1 * n1 * n1 + 25 * n1 * n1);
     out[5] = h->use_graph ? 1 : 0;
     out[6] = h->m.fs;
     out[7] = h->m.nelt;
@@ -1784,6 +1862,7 @@ int32_t nkc_set_keep_k(nkc_handle *h, int32_t keep) {
 
 int32_t nkc_set_layout(nkc_handle *h, int32_t layout) {
     if (!h) return fail(NKC_ERR_ARG, "null handle");
+    if (h->m.general && (layout > 0)) return general_refuses(h, "a stage layout other than 0");
     if (layout < -1 || layout >= NLAYOUT) return fail(NKC_ERR_ARG, "layout %d not in -1..%d", layout, NLAYOUT - 1);
     if (layout >= 0 && !layout_built(h->m.n1, layout))
         return fail(NKC_ERR_ARG, "layout %d is not built for N=%d", layout, h->m.n1 - 1);
@@ -1816,6 +1895,7 @@ int32_t nkc_tune(nkc_handle *h, double *out) {
 
 int32_t nkc_set_materials(nkc_handle *h, const double *eps, const double *mu) {
     if (!h || !eps || !mu) return fail(NKC_ERR_ARG, "null handle or material array");
+    if (h->m.general && (true)) return general_refuses(h, "replacing the materials");
     const int64_t npts = h->m.npts;
     for (int64_t p = 0; p < npts; ++p)
         if (!(eps[p] > 0.0) || !(mu[p] > 0.0)) return fail(NKC_ERR_ARG, "material not positive at node %lld", (long long)p);
@@ -2070,6 +2150,7 @@ int32_t nkc_adjoint_info(nkc_handle *h, int64_t *out) {
 
 int32_t nkc_adjoint_forward(nkc_handle *h, double *time, double dt, int64_t nsteps, int64_t every) {
     if (!h) return fail(NKC_ERR_ARG, "null handle");
+    if (h->m.general && (true)) return general_refuses(h, "the adjoint");
     if (nsteps <= 0 || every <= 0 || !std::isfinite(dt)) return fail(NKC_ERR_ARG, "bad adjoint forward arguments");
     if (h->nlor) return fail(NKC_ERR_UNSUPPORTED, "the adjoint of a Lorentz region is not built");
     if (h->npml) return fail(NKC_ERR_UNSUPPORTED, "the adjoint of a PML is not built");
@@ -2216,6 +2297,7 @@ int32_t nkc_adjoint_get_drude(nkc_handle *h, double *ga, double *gb) {
 
 int32_t nkc_adjoint_apply(nkc_handle *h, const double *kH, const double *kE, double *aH, double *aE, int64_t ld) {
     if (!h || !kH || !kE || !aH || !aE) return fail(NKC_ERR_ARG, "null argument");
+    if (h->m.general && (true)) return general_refuses(h, "the adjoint");
     DevMesh &m = h->m;
     if (ld < m.npts) return fail(NKC_ERR_ARG, "ld < npts");
     const size_t blk = 6 * (size_t)m.fs;

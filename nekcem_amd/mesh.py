@@ -117,6 +117,61 @@ def box_dxmin(nel, N, lo=(0.0, 0.0, 0.0), hi=(2 * np.pi,) * 3):
     return float(np.sqrt(d2) / 2.0)
 
 
+def face_geometry(n1, nelt, wxm1, deriv):
+    """cemface (1-based) and the unit outward normals and area weights of
+    every face node (AREA3 + UNITVEC, src/nek5_coef.F:1150-1237) in the
+    reference's face order, from the nine coordinate derivatives xr, xs, xt,
+    yr, ys, yt, zr, zs, zt (flat, reference volume order)."""
+    n2, n3 = n1 * n1, n1 ** 3
+    fn = skp_face_nodes(n1)                                     # (6, n2)
+    base = (np.arange(nelt) * n3)[:, None, None]
+    vol = base + fn[None, :, :]                                 # (nelt, 6, n2)
+    cemface = (vol.reshape(-1) + 1).astype(np.int32)
+    xr, xs, xt, yr, ys, yt, zr, zs, zt = (a[vol] for a in deriv)
+    unx, uny, unz, area = (np.zeros((nelt, 6, n2)) for _ in range(4))
+    wgt = np.tile(wxm1, n1) * np.repeat(wxm1, n1)
+    for f, (ax, side) in enumerate(ED_FACES):
+        if ax == 0:   u, v = (xs, ys, zs), (xt, yt, zt)      # "R": x_s x x_t
+        elif ax == 1: u, v = (xr, yr, zr), (xt, yt, zt)      # "S": x_r x x_t
+        else:         u, v = (xr, yr, zr), (xs, ys, zs)      # "T": x_r x x_s
+        A = u[1][:, f] * v[2][:, f] - u[2][:, f] * v[1][:, f]
+        B = u[2][:, f] * v[0][:, f] - u[0][:, f] * v[2][:, f]
+        Cc = u[0][:, f] * v[1][:, f] - u[1][:, f] * v[0][:, f]
+        # outward sign per AREA3: R and T faces +A on the + side, S faces
+        # +A on the - side
+        sgn = (1.0 if side == 1 else -1.0) * (-1.0 if ax == 1 else 1.0)
+        area[:, f] = np.sqrt(A * A + B * B + Cc * Cc) * wgt
+        unx[:, f], uny[:, f], unz[:, f] = sgn * A, sgn * B, sgn * Cc
+    s = np.sqrt(unx * unx + uny * uny + unz * unz)
+    return cemface, (unx / s).reshape(-1), (uny / s).reshape(-1), (unz / s).reshape(-1), area.reshape(-1)
+
+
+def face_materials(mesh, eps=None, mu=None):
+    """cem_maxwell_materials (src/cem_maxwell.F:262-325) on any mesh with
+    cemface, face_partner and cempec: per-node eps, mu -> (eps, mu, Z_0,
+    Z_1, Y_0, Y_1); the face sums are the gs sums over partners, doubled
+    on PEC faces."""
+    npts = mesh.npts
+    eps = np.ones(npts) if eps is None else np.ascontiguousarray(eps, dtype=np.float64)
+    mu = np.ones(npts) if mu is None else np.ascontiguousarray(mu, dtype=np.float64)
+    vol = mesh.cemface.astype(np.int64) - 1
+    zimp = np.sqrt(mu / eps)[vol]
+    ycon = np.sqrt(eps / mu)[vol]
+    p = mesh.face_partner.astype(np.int64) - 1
+    has = p >= 0
+    Z0 = zimp.copy()
+    Y0 = ycon.copy()
+    Z0[has] += zimp[p[has]]          # gs_op_fields(gsh_face, Z_0, add)
+    Y0[has] += ycon[p[has]]
+    pecmask = np.zeros(mesh.nxzfl, dtype=bool)
+    pecmask[mesh.cempec - 1] = True
+    Z0[pecmask] *= 2.0
+    Y0[pecmask] *= 2.0
+    Z1 = Z0 - zimp
+    Y1 = Y0 - ycon
+    return eps, mu, 0.5 * Z0, Z1, 0.5 * Y0, Y1
+
+
 class BoxMesh:
     """Reference-layout arrays for a box mesh.
 
@@ -214,32 +269,8 @@ class BoxMesh:
     def _faces(self):
         n1, nelt = self.n1, self.nelt
         n2, n3 = n1 * n1, n1 ** 3
-        fn = skp_face_nodes(n1)                                     # (6, n2)
-        base = (np.arange(nelt) * n3)[:, None, None]
-        vol = base + fn[None, :, :]                                 # (nelt, 6, n2)
-        self.cemface = (vol.reshape(-1) + 1).astype(np.int32)
-        xr, xs, xt, yr, ys, yt, zr, zs, zt = (a[vol] for a in self._deriv)
-        unx, uny, unz, area = (np.zeros((nelt, 6, n2)) for _ in range(4))
-        wa = np.tile(self.wxm1, n1)
-        wb = np.repeat(self.wxm1, n1)
-        wgt = wa * wb
-        for f, (ax, side) in enumerate(ED_FACES):
-            if ax == 0:   u, v = (xs, ys, zs), (xt, yt, zt)      # "R": x_s x x_t
-            elif ax == 1: u, v = (xr, yr, zr), (xt, yt, zt)      # "S": x_r x x_t
-            else:         u, v = (xr, yr, zr), (xs, ys, zs)      # "T": x_r x x_s
-            A = u[1][:, f] * v[2][:, f] - u[2][:, f] * v[1][:, f]
-            B = u[2][:, f] * v[0][:, f] - u[0][:, f] * v[2][:, f]
-            Cc = u[0][:, f] * v[1][:, f] - u[1][:, f] * v[0][:, f]
-            # outward sign per AREA3: R and T faces +A on the + side, S faces
-            # +A on the - side
-            sgn = (1.0 if side == 1 else -1.0) * (-1.0 if ax == 1 else 1.0)
-            area[:, f] = np.sqrt(A * A + B * B + Cc * Cc) * wgt
-            unx[:, f], uny[:, f], unz[:, f] = sgn * A, sgn * B, sgn * Cc
-        s = np.sqrt(unx * unx + uny * uny + unz * unz)
-        self.unxm = (unx / s).reshape(-1)
-        self.unym = (uny / s).reshape(-1)
-        self.unzm = (unz / s).reshape(-1)
-        self.aream = area.reshape(-1)
+        self.cemface, self.unxm, self.unym, self.unzm, self.aream = face_geometry(n1, nelt, self.wxm1,
+                                                                                  self._deriv)
         del self._deriv
 
         # partner map (1-based, 0 = none) and PEC list
@@ -270,29 +301,7 @@ class BoxMesh:
     # -- materials (cem_maxwell_materials) --------------------------------
     def materials(self, eps=None, mu=None):
         """Per-node permittivity/permeability -> (eps, mu, Z_0, Z_1, Y_0, Y_1)."""
-        npts = self.npts
-        eps = np.ones(npts) if eps is None else np.ascontiguousarray(eps, dtype=np.float64)
-        mu = np.ones(npts) if mu is None else np.ascontiguousarray(mu, dtype=np.float64)
-        vol = self.cemface.astype(np.int64) - 1
-        zimp = np.sqrt(mu / eps)[vol]
-        ycon = np.sqrt(eps / mu)[vol]
-        p = self.face_partner.astype(np.int64) - 1
-        has = p >= 0
-        Z0 = zimp.copy()
-        Y0 = ycon.copy()
-        Z0[has] += zimp[p[has]]          # gs_op_fields(gsh_face, Z_0, add)
-        Y0[has] += ycon[p[has]]
-        pecmask = np.zeros(self.nxzfl, dtype=bool)
-        pecmask[self.cempec - 1] = True
-        Z1 = np.zeros_like(Z0)
-        Y1 = np.zeros_like(Y0)
-        Z0[pecmask] *= 2.0
-        Y0[pecmask] *= 2.0
-        Z1 = Z0 - zimp
-        Y1 = Y0 - ycon
-        Z0 = 0.5 * Z0
-        Y0 = 0.5 * Y0
-        return eps, mu, Z0, Z1, Y0, Y1
+        return face_materials(self, eps, mu)
 
     # -- time step --------------------------------------------------------
     def dxmin(self):

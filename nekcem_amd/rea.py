@@ -9,7 +9,10 @@ Formats:
 * .rea: "N PARAMETERS FOLLOW" then one value per line; "**MESH DATA**"
   then "NEL NDIM NELV" (NEL < 0: the mesh is in the .re2) and per element a
   header line plus 6 lines of 4 numbers (x, y, z of corners 1-4, then of
-  5-8); "CURVED SIDE DATA" (must be empty here); "FLUID BOUNDARY
+  5-8); "CURVED SIDE DATA": a count, then per curved edge or face
+  "IEDGE IEL CURVE(1..5) CCURVE" (edge 1..8 or face 1..6, 1-based element,
+  5 parameters, a 1-character type such as 'C', a circular arc of radius
+  CURVE(1)); "FLUID BOUNDARY
   CONDITIONS" with one line per element face: code (columns 2-4), element,
   face, 5 parameters.
 * .box (genbox2 / getbox / makebox, src/nek5_genbox.F:2-555; used when
@@ -22,7 +25,8 @@ Formats:
   faces are 'E' with their neighbour, periodic walls 'P' with theirs.
 * .re2 (v002): an 80-byte header "#v002 nelgt ndim nelgv", a float32
   6.54321 byte-order mark, per element 25 float64 (group, x[8], y[8],
-  z[8]), then ncurve (float64) and its 64-byte records, then nbc (float64)
+  z[8]), then ncurve (float64) and its 64-byte records (element, side,
+  5 parameters as float64, type as 8 characters), then nbc (float64)
   and 64-byte records (element, face, 5 parameters as float64, code as 8
   characters).  Faces the .re2 does not list are interior ('E').
 """
@@ -38,8 +42,9 @@ def _floats(line):
 
 
 def read_rea(path):
-    """dict(param (1-based list), ndim, nel, xc, yc, zc, cbc, bc, in_re2,
-    in_box)."""
+    """dict(param (1-based list), ndim, nel, xc, yc, zc, cbc, bc, curves,
+    in_re2, in_box); curves is a list of (side 1..8, element 0-based,
+    parameters (5,), type)."""
     with open(path) as f:
         lines = f.read().splitlines()
     out = {}
@@ -71,8 +76,14 @@ def read_rea(path):
         while "CURVED SIDE DATA" not in lines[i]:
             i += 1
         ncurve = int(lines[i + 1].split()[0])
-        if ncurve:
-            raise ValueError(f"{path}: {ncurve} curved sides; the HIP path takes straight-sided boxes")
+        curves = []
+        for k in range(ncurve):
+            # rdcurve (src/nek5_connect2.F): fixed columns for nel < 1000
+            line = lines[i + 2 + k]
+            side, iel = int(line[0:3]), int(line[3:6])
+            curves.append((side, iel - 1, np.array(_floats(line[6:76])[:5]), line[76:].strip()[:1] or line.split()[-1]))
+        out["curves"] = curves
+        i += 2 + ncurve
         while "FLUID" not in lines[i] or "BOUNDARY" not in lines[i]:
             i += 1
         i += 1
@@ -107,8 +118,12 @@ def read_re2(path):
     xc, yc, zc = m[:, 1:9].copy(), m[:, 9:17].copy(), m[:, 17:25].copy()
     (ncurve,) = struct.unpack(end + "d", b[off:off + 8])
     off += 8
-    if int(ncurve):
-        raise ValueError(f"{path}: {int(ncurve)} curved sides; the HIP path takes straight-sided boxes")
+    curves = []
+    for _ in range(int(ncurve)):
+        v = struct.unpack(end + "7d", b[off:off + 56])
+        kind = b[off + 56:off + 64].decode("ascii", "replace").strip()[:1]
+        curves.append((int(v[1]), int(v[0]) - 1, np.array(v[2:7]), kind))
+        off += 64
     (nbc,) = struct.unpack(end + "d", b[off:off + 8])
     off += 8
     cbc = np.full((nel, 6), "E  ", dtype="<U3")
@@ -120,7 +135,7 @@ def read_re2(path):
         cbc[e - 1, f - 1] = code
         bc[e - 1, f - 1] = v[2:7]
         off += 64
-    return dict(nel=nel, xc=xc, yc=yc, zc=zc, cbc=cbc, bc=bc)
+    return dict(nel=nel, xc=xc, yc=yc, zc=zc, cbc=cbc, bc=bc, curves=curves)
 
 
 def _box_vertices(tokens, n):

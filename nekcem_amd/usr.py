@@ -328,3 +328,59 @@ def graphene_solution(t, y, eps, mu, upper_nodes, pml_nodes, inner, outer, order
     E[2] = np.where(lo, (p["trantm"] * uu).real, E[2])
     H[0] = np.where(lo, -(p["trantm"] * uu / eta).real, H[0])
     return H, E
+
+
+# ---------------------------------------------------------------------------
+# tests/cylwave: the TM_01 mode of a circular PEC waveguide, periodic in z
+# (tests/cylwave/cylwave.usr)
+# ---------------------------------------------------------------------------
+CYLWAVE_L2TOL = (5e-9,) * 6
+CYLWAVE_LINFTOL = (5e-8,) * 6
+
+
+def xy_radius(xc, yc):
+    """geom_xyradius (src/cem_common.F:817-843): the largest corner radius
+    in the x-y plane."""
+    return float(np.sqrt(np.max(np.asarray(xc) ** 2 + np.asarray(yc) ** 2)))
+
+
+def cylwave_usrdat(xc, yc, zc):
+    """cylwave's usrdat: corners within 1e-4 (in r^2) of the radius, rounded
+    down to a tenth, are moved onto that circle."""
+    xc, yc = np.array(xc, dtype=np.float64), np.array(yc, dtype=np.float64)
+    radius = int(10 * xy_radius(xc, yc) + 0.1) / 10.0
+    rr = xc * xc + yc * yc
+    on = rr > radius * radius - 1e-4
+    rn = radius / np.sqrt(rr[on])
+    xc[on] *= rn
+    yc[on] *= rn
+    return xc, yc, np.array(zc, dtype=np.float64)
+
+
+def cylwave_usrdat2(x, y, z):
+    """cylwave's usrdat2: z scaled onto [0, 2 pi xmax]."""
+    zmin, zmax = float(np.min(z)), float(np.max(z))
+    sz = 2 * np.pi * float(np.max(x)) / (zmax - zmin)
+    return x, y, sz * (z - zmin)
+
+
+def cylwave_solution(t, x, y, z, radius):
+    """cylwave's usersol with m = 0, the first root of J_0, one wavelength
+    along z: E_z = Re(a J_0(K rho)), E_rho = Re(-i a kz/K J_0'(K rho)),
+    H_phi = Re(-i a omega/K J_0'(K rho)), a = exp(-i kz z) exp(i omega t),
+    K = j_01 / radius, omega = sqrt(kz^2 + K^2)."""
+    from scipy.special import jn_zeros, jv, jvp
+    zsize = float(np.max(z) - np.min(z))
+    kz = 2 * np.pi / zsize if abs(zsize) > 1e-20 else 0.0
+    K = float(jn_zeros(0, 1)[0]) / radius
+    om = np.sqrt(kz * kz + K * K)
+    rho = np.sqrt(x * x + y * y)
+    phi = np.arctan2(y, x)
+    a = np.exp(-1j * kz * z) * np.exp(1j * om * t)
+    ezz = np.real(a * jv(0, K * rho))
+    erho = np.real(a * (-1j) * kz / K * jvp(0, K * rho))
+    hphi = np.real(a * (-1j) * om / K * jvp(0, K * rho))
+    zero = np.zeros_like(x)
+    H = np.stack([-np.sin(phi) * hphi, np.cos(phi) * hphi, zero])
+    E = np.stack([np.cos(phi) * erho, np.sin(phi) * erho, ezz])
+    return H, E

@@ -743,3 +743,83 @@ def restart_bytes(istep, dumpno, n1, time, dt, xyz, E, H, double=True, np_ranks=
     b += b"VECTORS %s  %s \n" % (b"H ", kind)
     b += H.T.astype(fmt).tobytes()
     return bytes(b)
+
+
+class GeneralCase(BoxCase):
+    """A reference-layout case on a general hexahedral mesh (curved,
+    rotated or sheared elements).  The node coordinates are given; the rest
+    is derived here as the reference's setup does: geometry by orc_geom
+    (GLMAPM1 / AREA3 from the coordinates), cemface by orc_set_fc_ptr, the
+    gs pairing as face nodes at equal coordinates on faces whose centroids
+    coincide, directly or after one of the given periodic translations
+    (the pairing setup_dgds2 produces from the global vertex numbering,
+    src/nek5_connect11.F:2179-2347), and the PEC list from pec_face
+    (nelt, 6) in the reference's face order (cem_maxwell_pec_init)."""
+
+    def __init__(self, N, xm1, ym1, zm1, pec_face, periods=(), upwind=True, eps=None, mu=None):
+        L = lib()
+        n1 = N + 1
+        self.n1 = n1
+        n2, n3 = n1 * n1, n1 ** 3
+        self.xm1, self.ym1, self.zm1 = (np.ascontiguousarray(a, dtype=np.float64).reshape(-1) for a in (xm1, ym1, zm1))
+        self.npts = npts = len(self.xm1)
+        self.nelt = nelt = npts // n3
+        self.nxzfl = nxzfl = 6 * n2 * nelt
+        self.spec = BoxSpec(nel=(nelt, 1, 1), N=N, bc=("PEC",) * 3, upwind=upwind)
+        self.z, self.w = zwgll(n1)
+        self.D = dgll(self.z)
+        self._Dflat = np.ascontiguousarray(self.D.reshape(-1, order="F"))
+        g = {k: np.zeros(npts) for k in ("rx", "ry", "rz", "sx", "sy", "sz", "tx", "ty", "tz", "jac", "bm1")}
+        self.w3 = np.zeros(n3)
+        fa = {k: np.zeros(nxzfl) for k in ("unx", "uny", "unz", "area")}
+        L.orc_geom(n1, nelt, _p(self._Dflat), _p(self.w), _p(self.xm1), _p(self.ym1), _p(self.zm1),
+                   *[_p(g[k]) for k in ("rx", "ry", "rz", "sx", "sy", "sz", "tx", "ty", "tz", "jac", "bm1")],
+                   _p(self.w3), _p(fa["unx"]), _p(fa["uny"]), _p(fa["unz"]), _p(fa["area"]))
+        self.geom, self.face, self.bm1 = g, fa, g["bm1"]
+        self.cemface = np.zeros(nxzfl, dtype=np.int32)
+        L.orc_set_fc_ptr(n1, nelt, _p(self.cemface))
+        self.partner = self._partner_by_position(np.asarray(periods, dtype=np.float64).reshape(-1, 3))
+        self.pec_face = np.ascontiguousarray(np.asarray(pec_face, dtype=np.int8).reshape(-1))
+        self.cempec = np.zeros(nxzfl, dtype=np.int32)
+        self.ncempec = int(L.orc_pec_init(n1, nelt, _p(self.pec_face), _p(self.cempec)))
+        self.set_materials(eps, mu)
+        self.dxmin = float(L.orc_get_dxmin(n1, nelt, _p(self.xm1), _p(self.ym1), _p(self.zm1)))
+        self.srflx = np.zeros(6 * nxzfl)
+        self.src_amp = None
+        self.src_dir, self.src_t0, self.src_tw = (0.0, 0.0, 1.0), 0.0, 1.0
+        self.dindex = self.drude_a = self.drude_b = None
+        self.lindex = self.lor_par = None
+        self.src_kind, self.src_omega, self.src_phase = 0, 0.0, 0.0
+        self.pmlptr = self.pmlsigma = None
+        self.pmltag = np.zeros(nelt, dtype=np.int64)
+        self.inc_on = self.gindex = self.inc = None
+
+    def _partner_by_position(self, periods):
+        n1, nelt, nxzfl = self.n1, self.nelt, self.nxzfl
+        n2 = n1 * n1
+        vol = self.cemface.astype(np.int64)                                      # 0-based here
+        P = np.stack([self.xm1[vol], self.ym1[vol], self.zm1[vol]], axis=1)      # (nxzfl, 3)
+        cen = P.reshape(nelt * 6, n2, 3).mean(axis=1)
+        scale = max(1.0, float(np.max(np.abs(P))))
+        q = 1e-7 * scale
+
+        def key(c):
+            return tuple(np.floor(c / q + 0.5).astype(np.int64))
+
+        faces = {}
+        for fi in range(6 * nelt):
+            faces.setdefault(key(cen[fi]), []).append(fi)
+        partner = np.full(nxzfl, -1, dtype=np.int32)
+        shifts = [np.zeros(3)] + [s for T in periods for s in (T, -T)]
+        for fi in range(6 * nelt):
+            for T in shifts:
+                other = [fj for fj in faces.get(key(cen[fi] + T), []) if fj != fi]
+                if other:
+                    fj = other[0]
+                    A, B = P[fi * n2:(fi + 1) * n2] + T, P[fj * n2:(fj + 1) * n2]
+                    d = np.linalg.norm(A[:, None, :] - B[None, :, :], axis=2)
+                    b = np.argmin(d, axis=1)
+                    assert np.max(d[np.arange(n2), b]) < 1e-6 * scale
+                    partner[fi * n2:(fi + 1) * n2] = fj * n2 + b
+                    break
+        return partner

@@ -3,8 +3,8 @@ of tests/tests.json (kept as tests/golden/maxwell3d_tests.json) is run from
 its case files with the entry's .rea parameter overrides applied as the
 reference's runner applies them (bin/runtests rewrites the .rea parameters,
 then the case's userchk decides pass or fail), and must pass the case's
-userchk tolerances.  cylwave needs curved elements at N = 11 and is skipped
-with that reason."""
+userchk tolerances.  cylwave (curved elements, N = 11) runs on the
+general-hexahedra kernel."""
 import json
 import os
 
@@ -88,7 +88,32 @@ def _boxpml(params):
     s.close()
 
 
+def _cylwave(params):
+    """tests/cylwave: the mesh from its .rea with the case's usrdat and
+    usrdat2 (curved elements, N = 11), the TM_01 mode as userini, CFL
+    param(12), param(11) steps, userchk at istep <= 10 and every iocomm
+    against usersol at L2 <= 5e-9, Linf <= 5e-8 per field."""
+    from nekcem_amd.maxwell import MaxwellSolver
+    from tests.meshes_general import cylwave_mesh
+    mesh, R, _, param = cylwave_mesh(11)
+    p = list(param)
+    for k, v in params.items():
+        p[int(k)] = float(v)
+    s = MaxwellSolver(mesh)
+    x, y, z = mesh.coords()
+    s.set_state(*usr.cylwave_solution(0.0, x, y, z, R))
+    dt = mesh.dt(p[12])
+    l2tol, litol = np.array(usr.CYLWAVE_L2TOL), np.array(usr.CYLWAVE_LINFTOL)
+    for step in range(1, int(p[11]) + 1):
+        s.step(dt, 1)
+        if step <= 10 or step % int(p[13]) == 0:
+            l2, li = s.cem_error_fields(*usr.cylwave_solution(s.time, x, y, z, R))
+            assert np.all(l2 <= l2tol) and np.all(li <= litol), (step, l2, li)
+    s.close()
+
+
 RUNNERS = {
+    "cylwave": _cylwave,
     "3dboxper": lambda p: _analytic("3dboxper", p, (0.0, 2 * np.pi)),
     "3dboxpec": lambda p: _analytic("3dboxpec", p, (-1.0, 1.0)),
     "3ddielectric": _dielectric,
@@ -102,7 +127,4 @@ def test_reference_registry_entry(name):
     entry = MATRIX[name]
     assert entry["app"] == "maxwell" and "acc" in entry["parallelization"]
     print(f"[case] {name} params={entry['params']}", flush=True)
-    if entry["dir"] == "cylwave":
-        pytest.skip("cylwave: 80 circular-arc edges (curved elements) at N = 11; "
-                    "the kernels are built for affine elements at N <= 9")
     RUNNERS[entry["dir"]](entry["params"])

@@ -22,8 +22,16 @@ own tolerances:
                   Carpenter & Kennedy); 3dboxper and 3dboxpec analytic
                   errors within the reference's userchk tolerances
   Drude / source  uniform-field ODEs with closed-form solutions
+  GeneralCase     (general hexahedra: geometry from the coordinates, gs
+                  pairing by position) equals BoxCase on a box's
+                  coordinates and on a box with rotated element frames; the
+                  reference's own cylwave (curved elements, N = 11) within
+                  its userchk tolerances over 1000 steps
+  restart_bytes   the legacy-VTK layout: header lines, block sizes, cells
+                  covering every GLL sub-cube once
 """
 import math
+import os
 
 import numpy as np
 import pytest
@@ -780,3 +788,77 @@ def test_restart_bytes_layout(oracle):
     x = np.frombuffer(b[off:off + 12 * npts], dtype=">f4").reshape(npts, 3).T
     assert np.array_equal(x, xyz.astype(np.float32))
     assert b.endswith(H.T.astype(">f8").tobytes())
+
+
+def test_general_case_on_a_box_equals_box_case(oracle):
+    """GeneralCase (geometry from the coordinates, pairing by position) on
+    a box's own coordinates gives BoxCase's residual: its pairing equals the
+    lattice ids' and its geometry the box's."""
+    from nekcem_amd.mesh import box_coords
+    rng = np.random.default_rng(1)
+    nel, N, lo, hi, bc = (3, 2, 2), 4, (0, 0, 0), (1.0, 1.3, 0.8), ("P", "PEC", "P")
+    box = oracle.BoxCase(oracle.BoxSpec(nel=nel, N=N, lo=lo, hi=hi, bc=bc))
+    x, y, z = box_coords(nel, N, lo, hi)
+    pec = box.pec_face.reshape(-1, 6)
+    gen = oracle.GeneralCase(N, x, y, z, pec, periods=[(1.0, 0, 0), (0, 0, 0.8)])
+    assert np.array_equal(gen.partner, box.partner)
+    H, E = rng.standard_normal((3, box.npts)), rng.standard_normal((3, box.npts))
+    for a, b in zip(gen.residual(H, E), box.residual(H, E)):
+        assert np.max(np.abs(a - b)) <= 1e-12 * np.max(np.abs(b))
+
+
+def test_general_case_rotated_elements(oracle):
+    """Box elements renumbered in rotated local frames (other axes, faces
+    and in-face orientations; pairing by position) give the plain box's
+    residual at the same physical nodes, to round-off."""
+    from tests.meshes_general import rotated_box
+    rng = np.random.default_rng(2)
+    nel, N, lo, hi, bc = (2, 2, 2), 5, (0, 0, 0), (1.0, 1.0, 1.0), ("P", "P", "PEC")
+    mesh, idx, periods = rotated_box(nel, N, lo, hi, bc, which=[0, 3, 5, 6], seed=3)
+    box = oracle.BoxCase(oracle.BoxSpec(nel=nel, N=N, lo=lo, hi=hi, bc=bc))
+    x, y, z = mesh.coords()
+    gen = oracle.GeneralCase(N, x, y, z, mesh.pec_face, periods=periods)
+    assert gen.ncempec == box.ncempec
+    H, E = rng.standard_normal((3, box.npts)), rng.standard_normal((3, box.npts))
+    gH, gE = gen.residual(H[:, idx], E[:, idx])
+    bH, bE = box.residual(H, E)
+    for a, b in ((gH, bH[:, idx]), (gE, bE[:, idx])):
+        assert np.max(np.abs(a - b)) <= 1e-11 * np.max(np.abs(b))
+
+
+def test_general_case_cylwave_userchk(oracle):
+    """The reference's own tests/cylwave (50 curved elements, 80 circular-arc
+    edges, PEC cylinder wall, periodic in z, N = 11, CFL 0.25, 1000 steps)
+    on GeneralCase, checked like its userchk (istep <= 10 and every iocomm =
+    100) against the TM_01 mode at L2 <= 5e-9 and Linf <= 5e-8 per field.
+    The same check fails with the arcs left out (straight-sided elements)."""
+    from nekcem_amd import usr
+    from nekcem_amd.hexmesh import HexMesh
+    from tests.meshes_general import cylwave_mesh
+    mesh, R, zp, param = cylwave_mesh(11)
+    x, y, z = mesh.coords()
+    ref = oracle.GeneralCase(11, x, y, z, mesh.pec_face, periods=[(0, 0, zp)])
+    assert np.array_equal(ref.partner + 1, mesh.face_partner)
+    dt = mesh.dt(param[12])
+    assert abs(dt - 0.25 * ref.dxmin) < 1e-15
+    H, E = usr.cylwave_solution(0.0, x, y, z, R)
+    t = 0.0
+    for step in range(1, int(param[11]) + 1):
+        out = ref.run(H, E, dt, 1, t=t)
+        H, E, t = out["H"], out["E"], out["time"]
+        if step <= 10 or step % int(param[13]) == 0:
+            Hs, Es = usr.cylwave_solution(t, x, y, z, R)
+            for u, s in zip(list(H) + list(E), list(Hs) + list(Es)):
+                l2, li = ref.cem_error(u, s)
+                assert l2 <= 5e-9 and li <= 5e-8, (step, l2, li)
+    # straight-sided elements: the mode misses the wall by far more
+    from nekcem_amd import rea
+    r = rea.read_case(os.path.join(os.path.dirname(__file__), "golden", "cases", "cylwave.rea"))
+    xc, yc, zc = usr.cylwave_usrdat(r["xc"], r["yc"], r["zc"])
+    flat = HexMesh.from_corners(11, xc, yc, zc, r["cbc"], r["bc"], (), usrdat2=usr.cylwave_usrdat2)
+    x, y, z = flat.coords()
+    ref2 = oracle.GeneralCase(11, x, y, z, flat.pec_face, periods=[(0, 0, zp)])
+    H, E = usr.cylwave_solution(0.0, x, y, z, R)
+    out = ref2.run(H, E, flat.dt(param[12]), 10)
+    Hs, Es = usr.cylwave_solution(out["time"], x, y, z, R)
+    assert max(ref2.cem_error(u, s)[0] for u, s in zip(out["E"], Es)) > 1e-6
