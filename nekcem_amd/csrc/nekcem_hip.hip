@@ -942,10 +942,9 @@ int32_t nkc_create(const nkc_desc *d, nkc_handle **out) {
     const int64_t npts = (int64_t)n1 * n1 * n1 * d->nelt;
     if (npts > ((int64_t)1 << 31) - 1) return fail(NKC_ERR_ARG, "npts exceeds int32 indexing");
     HostMesh hm;
-    const int32_t rc = compact_reference(d, hm);
+    // box meshes above the box kernels' orders run on the general kernel
+    const int32_t rc = supported_n1(n1) ? compact_reference(d, hm) : compact_general(d, hm);
     if (rc) return fail(rc, "%s", hm.err.c_str());
-    if (!hm.general && !supported_n1(n1))
-        return fail(NKC_ERR_UNSUPPORTED, "polynomial order N=%d: box meshes are built for N = 3..9", d->N);
     return make_handle(hm, d->ifupwind ? 1.0 : 0.0, out);
 }
 

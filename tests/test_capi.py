@@ -60,16 +60,28 @@ def test_create_rejects_unsupported_order():
     assert L.nkc_create(C.byref(d), C.byref(h)) == _lib.NKC_ERR_UNSUPPORTED
 
 
-def test_create_rejects_non_box_geometry():
-    """A sheared element (nonzero off-diagonal cofactor) is refused, not run."""
+def test_create_takes_non_box_geometry_to_the_general_tables():
+    """A sheared element (nonzero off-diagonal cofactor) passes the host
+    set-up on the general-hexahedra tables (without a GPU nkc_create then
+    stops at the device: NKC_ERR_NODEVICE); a face partner that does not
+    pair back is refused there."""
     L = _lib.load()
     mesh = BoxMesh((2, 2, 2), 4)
     ry = mesh.rymn.copy()
     ry[: mesh.n1 ** 3] = 0.3 * mesh.rxmn[0]
     d, keep = _desc(mesh, rymn=ry)
     h = C.c_void_p()
-    assert L.nkc_create(C.byref(d), C.byref(h)) == _lib.NKC_ERR_UNSUPPORTED
-    assert b"axis-aligned" in L.nkc_last_error()
+    rc = L.nkc_create(C.byref(d), C.byref(h))
+    assert rc in (_lib.NKC_OK, _lib.NKC_ERR_NODEVICE), L.nkc_last_error()
+    if rc == _lib.NKC_OK:
+        L.nkc_destroy(h)
+    fp = mesh.face_partner.copy()
+    s = int(np.nonzero(fp)[0][0])
+    fp[s] = fp[s] + 1 if fp[s] < len(fp) else fp[s] - 1     # the next face node: it pairs elsewhere
+    mesh.face_partner = fp
+    d, keep = _desc(mesh, rymn=ry)
+    assert L.nkc_create(C.byref(d), C.byref(h)) == _lib.NKC_ERR_ARG
+    assert b"pair back" in L.nkc_last_error() or b"range" in L.nkc_last_error()
 
 
 def test_create_rejects_bad_cemface():
