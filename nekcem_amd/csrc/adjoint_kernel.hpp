@@ -265,7 +265,7 @@ adjoint_kernel(AdjArgs a) {
                 const int64_t p = (fo + c) * fs + node;
                 lm[c][kk] = ok ? a.lam_in[p] : 0.0;
                 kp[c][kk] = ok ? a.kap_in[p] : 0.0;
-                rr[c][kk] = (GM && ok) ? ld_stream(a.rs + p) : 0.0;
+                rr[c][kk] = (GM && ok) ? ld_stream<nt_streams<N1>()>(a.rs + p) : 0.0;
             }
             gg[kk] = (GM && ok) ? g[node] : 0.0;
         }
@@ -280,7 +280,7 @@ adjoint_kernel(AdjArgs a) {
                 const double kt = kp[c][kk] + a.rkb * lm[c][kk];
                 s_v[(fo + c) * N3 + S::idx(i, j, k)] = pm[kk] * kt;
                 kr += kt * rr[c][kk];
-                st_stream(a.kap_out + (fo + c) * fs + node, a.rka * kt);
+                st_stream<nt_streams<N1>()>(a.kap_out + (fo + c) * fs + node, a.rka * kt);
             }
             if (GM) s_g[grp * N3 + k * N2 + q] = gg[kk] - a.dt * kr * pm[kk];
         }
@@ -467,7 +467,7 @@ adjoint_kernel(AdjArgs a) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const int64_t p = (fo + c) * fs + node;
-            st_stream(a.lam_out + p, lm[c][kk] + a.dt * w[c]);
+            st_stream<nt_streams<N1>()>(a.lam_out + p, lm[c][kk] + a.dt * w[c]);
         }
     }
 }

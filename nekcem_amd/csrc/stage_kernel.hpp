@@ -28,7 +28,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-// Build-time tuning knobs (the defaults are the measured best on MI355X).
+// Build-time switches (0 turns the non-temporal forms off at every order).
 #ifndef NKC_NT_STORE
 #define NKC_NT_STORE 1      // non-temporal stores for the streamed outputs
 #endif
@@ -38,22 +38,26 @@
 
 namespace nkc {
 
-template <class T>
+// Whether the streamed outputs and the RK register go past L2 with
+// non-temporal stores and loads, per order.  At N = 7 a role's level is 64
+// doubles, whole cache lines from one wave, and streaming them keeps L2 for
+// the neighbour traces (32^3: 619 against 639-648 us per stage with plain
+// accesses); at the other orders a level's (N+1)^2 doubles straddle lines
+// and the plain forms measured faster (N = 4 670 -> 534 us, N = 5 555 ->
+// 495, N = 6 538 -> 498-504, N = 8 410 -> 391, N = 9 level).
+template <int N1>
+constexpr bool nt_streams() { return N1 == 8; }
+
+template <bool NT, class T>
 __device__ __forceinline__ void st_stream(T *p, T v) {
-#if NKC_NT_STORE
-    __builtin_nontemporal_store(v, p);
-#else
-    *p = v;
-#endif
+    if constexpr (NT && NKC_NT_STORE) __builtin_nontemporal_store(v, p);
+    else *p = v;
 }
 
-template <class T>
+template <bool NT, class T>
 __device__ __forceinline__ T ld_stream(const T *p) {
-#if NKC_NT_LOAD_K
-    return __builtin_nontemporal_load(p);
-#else
-    return *p;
-#endif
+    if constexpr (NT && NKC_NT_LOAD_K) return __builtin_nontemporal_load(p);
+    else return *p;
 }
 
 // Kernel face order (symmetric): 0 x-, 1 x+, 2 y-, 3 y+, 4 z-, 5 z+.
@@ -334,6 +338,7 @@ __global__ void __launch_bounds__((Shape<N1, PACK>::BS * EPW)) __attribute__((am
 stage_kernel(StageArgs a) {
     using S = Shape<N1, PACK>;
     constexpr int N2 = S::N2, N3 = S::N3, KH = S::KH, BS = S::BS, LANES = S::LANES;
+    constexpr bool NT = nt_streams<N1>();
     __shared__ double s_u_all[EPW][6 * N3];   // element, swizzled
     // ZF: the z-face fluxes are formed in phase 1 as well (roles 0 and 3,
     // once per face) and read back from LDS at the end levels, so phase 2
@@ -391,7 +396,7 @@ stage_kernel(StageArgs a) {
 #pragma unroll
             for (int kk = 0; kk < KH; ++kk) {
                 const int k = k0 + kk;
-                kp[c][KPRE ? kk : 0] = (k < N1) ? ld_stream(a.kbuf + (3 * grp + c) * fs + eo + k * N2 + q) : 0.0;
+                kp[c][KPRE ? kk : 0] = (k < N1) ? ld_stream<NT>(a.kbuf + (3 * grp + c) * fs + eo + k * N2 + q) : 0.0;
             }
     }
     // source amplitude of this role's levels (E roles), with the HBM
@@ -535,7 +540,7 @@ stage_kernel(StageArgs a) {
         for (int c = 0; c < 3; ++c)
             kold[c] = (KPRE || MODE != MODE_RK || !(a.kflags & 1))
                           ? kp[c][KPRE ? kk : 0]
-                          : ld_stream(a.kbuf + (3 * grp + c) * fs + eo + vloc);
+                          : ld_stream<NT>(a.kbuf + (3 * grp + c) * fs + eo + vloc);
         double S1_r = 0, S2_r = 0, S0_s = 0, S2_s = 0, S0_t = 0, S1_t = 0;
 #pragma unroll
         for (int l = 0; l < N1; ++l) {
@@ -675,18 +680,18 @@ stage_kernel(StageArgs a) {
         for (int c = 0; c < 3; ++c) r[c] *= minv;
         if (MODE == MODE_RESIDUAL) {
 #pragma unroll
-            for (int c = 0; c < 3; ++c) st_stream(a.res + (co + c) * fs + node, r[c]);
+            for (int c = 0; c < 3; ++c) st_stream<NT>(a.res + (co + c) * fs + node, r[c]);
         } else {
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 const int64_t p = (co + c) * fs + node;
                 const double kk2 = a.rka * kold[c] + dt * r[c];
-                if (a.kflags & 2) st_stream(a.kbuf + p, kk2);
-                st_stream(a.uout + p, s_u[(co + c) * N3 + own] + a.rkb * kk2);
+                if (a.kflags & 2) st_stream<NT>(a.kbuf + p, kk2);
+                st_stream<NT>(a.uout + p, s_u[(co + c) * N3 + own] + a.rkb * kk2);
             }
             if (a.st_r) {
 #pragma unroll
-                for (int c = 0; c < 3; ++c) st_stream(a.st_r + (co + c) * fs + node, r[c]);
+                for (int c = 0; c < 3; ++c) st_stream<NT>(a.st_r + (co + c) * fs + node, r[c]);
             }
         }
     }

@@ -156,6 +156,22 @@ class _HexMeshInfo:
         return abs(p12) if p12 < 0 else p12 * self.dxmin()
 
 
+def _axis_aligned(xc, yc, zc, tol=1e-9):
+    """Whether every element's 8 corners are those of an axis-aligned box
+    (corners 1-4 on t = -1 counter-clockwise from (r, s) = (-1, -1), 5-8
+    above them, as nkc_create_hex reads them)."""
+    side = np.array([[0, 1, 1, 0, 0, 1, 1, 0], [0, 0, 1, 1, 0, 0, 1, 1], [0, 0, 0, 0, 1, 1, 1, 1]], dtype=bool)
+    for a, c in enumerate((xc, yc, zc)):
+        c = np.asarray(c, dtype=np.float64).reshape(-1, 8)
+        lo = c[:, ~side[a]].mean(axis=1)
+        hi = c[:, side[a]].mean(axis=1)
+        scale = np.maximum(1.0, np.abs(hi - lo))
+        if np.any(np.abs(c[:, ~side[a]] - lo[:, None]) > tol * scale[:, None]) or \
+                np.any(np.abs(c[:, side[a]] - hi[:, None]) > tol * scale[:, None]):
+            return False
+    return True
+
+
 class MaxwellSolver:
     """One nkc handle: the device-resident Maxwell state of a mesh."""
 
@@ -186,17 +202,27 @@ class MaxwellSolver:
         return self
 
     @classmethod
-    def from_rea(cls, rea_path, N, usrdat2=None, re2_path=None, **kw):
+    def from_rea(cls, rea_path, N, usrdat2=None, re2_path=None, usrdat2_nodes=None, **kw):
         """Handle for a NekCEM case file (.rea, with its .re2 when the mesh is
         there).  usrdat2(xc, yc, zc) -> (xc, yc, zc) may move the corners as
-        the case's usrdat2 moves the mesh.  The .rea parameters are kept in
-        ``param`` (1-based: param[11] = nsteps, param[12] = dt or CFL)."""
+        the case's usrdat / usrdat2 move the mesh; usrdat2_nodes(x, y, z) ->
+        (x, y, z) the node coordinates.  Axis-aligned box elements go to
+        nkc_create_hex; curved sides or other element shapes to the general
+        hexahedra (nekcem_amd.hexmesh, nkc_create).  The .rea parameters are
+        kept in ``param`` (1-based: param[11] = nsteps, param[12] = dt or
+        CFL)."""
         from .rea import read_case
         r = read_case(rea_path, re2_path)
         xc, yc, zc = r["xc"], r["yc"], r["zc"]
         if usrdat2 is not None:
             xc, yc, zc = usrdat2(xc, yc, zc)
-        self = cls.from_corners(N, xc, yc, zc, r["cbc"], r["bc"], **kw)
+        curves = [c for c in r.get("curves", ()) if c[3] not in ("", " ")]
+        if curves or usrdat2_nodes is not None or not _axis_aligned(xc, yc, zc):
+            from .hexmesh import HexMesh
+            mesh = HexMesh.from_corners(N, xc, yc, zc, r["cbc"], r["bc"], curves, usrdat2=usrdat2_nodes)
+            self = cls(mesh, ifupwind=kw.get("ifupwind", True), graph=kw.get("graph", True))
+        else:
+            self = cls.from_corners(N, xc, yc, zc, r["cbc"], r["bc"], **kw)
         self.param = r["param"]
         return self
 

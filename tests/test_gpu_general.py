@@ -157,3 +157,27 @@ def test_general_refuses_box_only_physics(oracle):
         assert ei.value.code == _lib.NKC_ERR_UNSUPPORTED
     g.set_layout(0)
     g.close()
+
+
+def test_cylwave_from_its_rea():
+    """MaxwellSolver.from_rea on the reference's cylwave.rea (its usrdat on
+    the corners, usrdat2 on the nodes) takes the general path and steps
+    exactly like the mesh built directly."""
+    import os
+    from nekcem_amd import usr
+    print("[case] cylwave from_rea", flush=True)
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "cases", "cylwave.rea")
+    s = MaxwellSolver.from_rea(path, 11, usrdat2=usr.cylwave_usrdat, usrdat2_nodes=usr.cylwave_usrdat2)
+    mesh, R, _, param = cylwave_mesh(11)
+    assert s.npts == mesh.npts and s.param[11] == param[11]
+    x, y, z = mesh.coords()
+    H, E = usr.cylwave_solution(0.0, x, y, z, R)
+    g = MaxwellSolver(mesh)
+    dt = mesh.dt(param[12])
+    for h in (s, g):
+        h.set_state(H, E)
+        h.step(dt, 5)
+    a, b = s.get_state(), g.get_state()
+    assert all(np.array_equal(u, v) for u, v in zip(a, b))
+    s.close()
+    g.close()
