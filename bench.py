@@ -171,6 +171,9 @@ def batched(ncav=256, steps=50, warmup=3, seed=3, log=sys.stderr):
             "t_batched_s": t_batch, "t_sequential_s": t_seq, "cavities": ncav, "npts": npts,
             "steps": steps, "batched_G_per_s": npts * 30 * steps / t_batch / 1e9,
             "sequential_G_per_s": npts * 30 * steps / t_seq / 1e9,
+            # both sides stream at least B_ALG bytes per unit: no schedule beats
+            # HBM_PEAK_GBS / B_ALG G/s, so the batch is at most this much faster
+            "hbm_bound_x": (HBM_PEAK_GBS / B_ALG) / (npts * 30 * steps / t_seq / 1e9),
             "bitwise_identical_to_sequential": bool(identical),
             "workload": f"{ncav} PEC cavities on [0,1]^3, n^3 elements n~U{{2..16}}, N~U{{5..9}}, "
                         f"per-element eps U[1,4], Gaussian Ez (width 0.2), own CFL-0.25 dt, seed {seed}"}

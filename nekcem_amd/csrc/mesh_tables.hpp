@@ -1,18 +1,26 @@
 // mesh_tables.hpp -- host-side compact tables the stage kernel reads.
 //
-// Two producers fill the same HostMesh:
-//   compact_reference()  from the reference-layout arrays of nkc_desc (what
-//                        NekCEM's cem_maxwell_init leaves in COMMON), checking
-//                        that the mesh is made of axis-aligned affine
-//                        hexahedra paired face to face;
+// Producers of a HostMesh:
+//   compact_box()        from the reference-layout arrays of nkc_desc (what
+//                        NekCEM's cem_maxwell_init leaves in COMMON), when
+//                        the mesh is made of axis-aligned affine hexahedra
+//                        paired face to face;
 //   build_box()          directly from a box description (nkc_box_desc), the
 //                        genbox2 semantics of src/nek5_genbox.F:2-145 without
-//                        materialising per-node arrays.
-// Both give, per element: g = (rx, sy, tz)/J, the six lift factors
-// area/(J w3), the six neighbours (or PEC / unpaired), the upwind/central
-// flux coefficients (Y02, C02Y, Z02, C02Z) of cem_maxwell_flux3d
-// (src/cem_maxwell.F:970-975) from the impedance averages of
-// cem_maxwell_materials (src/cem_maxwell.F:262-325), and 1/eps, 1/mu.
+//                        materialising per-node arrays;
+//   build_hex()          from element corners and boundary codes (.rea);
+//   compact_general()    from the reference-layout arrays of any conforming
+//                        hexahedral mesh (curved, rotated, sheared elements).
+// compact_reference() tries compact_box() and falls back to
+// compact_general().  The box tables hold, per element: g = (rx, sy, tz)/J,
+// the six lift factors area/(J w3), the six neighbours (or PEC / unpaired),
+// the upwind/central flux coefficients (Y02, C02Y, Z02, C02Z) of
+// cem_maxwell_flux3d (src/cem_maxwell.F:970-975) from the impedance averages
+// of cem_maxwell_materials (src/cem_maxwell.F:262-325), and 1/eps, 1/mu.
+// The general tables hold the same per node (nine cofactor/J quotients,
+// bm1, 1/eps, 1/mu) and per face node (unit normal, area/bm1, the partner's
+// volume node, the four flux coefficients).  pml_tables() is the PML set-up
+// of src/cem_maxwell_pml.F on the box tables.
 #pragma once
 #include <algorithm>
 #include <cmath>
