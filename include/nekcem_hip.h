@@ -385,7 +385,13 @@ int32_t nkc_set_graph(nkc_handle *h, int32_t enable);
  * back in fewer waves (1).  Bit 1: the register allocator is left free (0)
  * or held to 128 VGPRs for 4 waves per SIMD (1).  Layouts 4 and 5 take two
  * elements per workgroup, packed, with the free / 128-VGPR register budget;
- * they are built for N <= 5 only.  The first nkc_step times the layouts on
+ * they are built for N <= 5 only.  Layouts 6 and 7 split an element six
+ * ways (the two update groups x three runs of levels, packed; every lane
+ * forms one face flux), with the free / occupancy-bound register budget;
+ * they are built for N >= 7 and carry sources, Drude and Lorentz regions
+ * but no PML, incident field or graphene sheet (nkc_set_layout refuses
+ * them on such a handle with NKC_ERR_UNSUPPORTED, and the tuner does not
+ * time them there).  The first nkc_step times the layouts on
  * the handle's mesh and keeps the fastest (an autotuner; a packed
  * one-element layout with the same workgroup as the unpacked one, and a
  * layout not built for the order, are not timed).  nkc_set_layout forces
@@ -393,7 +399,7 @@ int32_t nkc_set_graph(nkc_handle *h, int32_t enable);
  * next step; nkc_tune tunes now: out[NKC_NLAYOUT + 1] = {us per stage for
  * each layout (NaN: not timed), chosen layout} (out may be NULL).  All
  * layouts compute the same arithmetic: results are bit-identical. */
-#define NKC_NLAYOUT 6
+#define NKC_NLAYOUT 8
 int32_t nkc_set_layout(nkc_handle *h, int32_t layout);
 int32_t nkc_tune(nkc_handle *h, double *out);
 /* The current layout and the last tuner times without tuning:

@@ -50,13 +50,6 @@ struct GeoArgs {
 
 constexpr int GEO_BS = 256;
 
-// n x v
-__device__ __forceinline__ void gcross(const double n[3], const double v[3], double o[3]) {
-    o[0] = n[1] * v[2] - n[2] * v[1];
-    o[1] = n[2] * v[0] - n[0] * v[2];
-    o[2] = n[0] * v[1] - n[1] * v[0];
-}
-
 // In-element volume index of in-face node q = a + n1 b on the reference's
 // face ef0 + 1 (0: y-, 1: x+, 2: y+, 3: x-, 4: z-, 5: z+), skpdat order
 // (src/nek5_connect11.F:1483-1528).

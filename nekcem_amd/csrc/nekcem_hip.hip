@@ -158,18 +158,31 @@ void dev_mesh_free(DevMesh &dm) {
 // Stage-kernel layouts: 0..3 one element per workgroup, bit 0 = PACK (lane
 // roles packed), bit 1 = OCC (4 waves/SIMD register budget); 4, 5 two
 // elements per workgroup, packed, without / with the OCC budget (N <= 5
-// only).  The residual test entry uses layout 0.
+// only); 6, 7 the six-role kernel without / with the budget of its
+// LDS-bound occupancy (N >= 7 only; no PML, incident field or graphene
+// sheet).  The residual test entry uses layout 0.
 constexpr int NLAYOUT = NKC_NLAYOUT;
 
 // Elements per workgroup of a layout.
-int layout_epw(int layout) { return layout >= 4 ? 2 : 1; }
+int layout_epw(int layout) { return (layout == 4 || layout == 5) ? 2 : 1; }
 
 // Whether a layout is built for this order.
-bool layout_built(int n1, int layout) { return layout < 4 || n1 <= EPW_MAX_N1; }
+bool layout_built(int n1, int layout) {
+    if (layout < 4) return true;
+    if (layout < 6) return n1 <= EPW_MAX_N1;
+    return n1 >= S6_MIN_N1;
+}
+
+// Whether a built layout carries the physics of these stage arguments (the
+// six-role kernel has no PML, incident field or graphene sheet).
+bool layout_runs(int n1, int layout, const StageArgs &a) {
+    return layout_built(n1, layout) && !(layout >= 6 && (a.pml_off || a.inc_mask || a.gr_off));
+}
 
 // Lanes a workgroup of the given layout has.
 int stage_block(int n1, int layout) {
     const int n2 = n1 * n1;
+    if (layout >= 6) return (6 * n2 + 63) / 64 * 64;
     const int one = (layout & 1) ? (4 * n2 + 63) / 64 * 64 : 4 * ((n2 + 63) / 64 * 64);
     return one * layout_epw(layout);
 }
@@ -215,7 +228,7 @@ int32_t launch_stage(const DevMesh &dm, const StageArgs &a, int mode, hipStream_
     if (dm.general) return launch_geo_stage(dm, a, mode == MODE_RK ? GMODE_RK : GMODE_RESIDUAL, s);
     const int pk0 = layout == -2 ? (dm.layout < 0 ? 0 : dm.layout) : layout;
     // the residual entry and orders without the layout run layout 0
-    const int pk = (mode == MODE_RK && layout_built(dm.n1, pk0)) ? pk0 : 0;
+    const int pk = (mode == MODE_RK && layout_runs(dm.n1, pk0, a)) ? pk0 : 0;
     const int64_t nwg = (dm.nelt + layout_epw(pk) - 1) / layout_epw(pk);
     const int per_xcd = (int)((nwg + 7) / 8);
     const int grid = 8 * per_xcd;
@@ -275,7 +288,8 @@ int32_t tune_stage(DevMesh &dm, const StageArgs *feat, size_t jsz, size_t lsz, s
     // not timed: a layout not built for this order, and a packed one-element
     // layout whose workgroup equals its unpacked twin
     auto skip = [&](int p) {
-        return !layout_built(dm.n1, p) || (p < 4 && (p & 1) && stage_block(dm.n1, p) == stage_block(dm.n1, p - 1));
+        return !layout_built(dm.n1, p) || (p < 4 && (p & 1) && stage_block(dm.n1, p) == stage_block(dm.n1, p - 1)) ||
+               (p >= 6 && feat && (feat->pml_off || feat->inc_mask || feat->gr_off));
     };
     const size_t blk = 6 * (size_t)dm.fs;
     const size_t jx = (feat && feat->drude_off) ? jsz : 0;
@@ -460,7 +474,7 @@ struct nkc_handle {
     double *yz = nullptr;                 // (sqrt(eps/mu), sqrt(mu/eps)) per ieps entry, adjoint only
     int64_t adj_store_cap = 0;            // cap on the adjoint's stage-store window (0: from free memory)
     int32_t adj_grads = 3;                // bit 0: eps/mu gradients, bit 1: Drude gradients
-    double tune_us[NLAYOUT] = {NAN, NAN, NAN, NAN, NAN, NAN};   // tuner's stage times per layout
+    double tune_us[NLAYOUT] = {NAN, NAN, NAN, NAN, NAN, NAN, NAN, NAN};   // tuner's stage times per layout
     // on-device diagnostics: node coordinates (3 x fs; from the box or
     // nkc_set_coords), element Jacobians and GLL weights (bm1 = jac w3),
     // partial norms, the 13 final values
@@ -1865,6 +1879,8 @@ int32_t nkc_set_layout(nkc_handle *h, int32_t layout) {
     if (layout < -1 || layout >= NLAYOUT) return fail(NKC_ERR_ARG, "layout %d not in -1..%d", layout, NLAYOUT - 1);
     if (layout >= 0 && !layout_built(h->m.n1, layout))
         return fail(NKC_ERR_ARG, "layout %d is not built for N=%d", layout, h->m.n1 - 1);
+    if (layout >= 0 && !layout_runs(h->m.n1, layout, base_args(h)))
+        return fail(NKC_ERR_UNSUPPORTED, "layout %d carries no PML, incident field or graphene sheet", layout);
     HIPCHK(hipStreamSynchronize(h->stream));
     h->m.layout = layout;
     return drop_graphs(h);

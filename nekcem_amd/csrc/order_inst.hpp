@@ -35,6 +35,25 @@ void launch_p(const StageArgs &a, int mat, int mode, int grid, hipStream_t s) {
     else launch_m<N1, MAT_NODE, PACK, OCC, EPW>(a, mode, grid, s);
 }
 
+// The six-role kernel (layouts 6, 7): RK stages with at most a source,
+// Drude or Lorentz region; anything else runs layout 0.
+template <int N1, int OCC>
+void launch_s6(const StageArgs &a, int mat, int mode, int grid, hipStream_t s) {
+    if (mode != MODE_RK || a.pml_off || a.inc_mask || a.gr_off) {
+        launch_p<N1, 0, 0, 1>(a, mat, mode, grid, s);
+        return;
+    }
+    const dim3 bs(Shape6<N1>::BS);
+    const bool feat = a.src_off || a.drude_off || a.lor_off;
+    if (mat == MAT_ELEM) {
+        if (feat) hipLaunchKernelGGL((stage6_kernel<N1, MAT_ELEM, MODE_RK, OCC, 1>), dim3(grid), bs, 0, s, a);
+        else hipLaunchKernelGGL((stage6_kernel<N1, MAT_ELEM, MODE_RK, OCC, 0>), dim3(grid), bs, 0, s, a);
+    } else {
+        if (feat) hipLaunchKernelGGL((stage6_kernel<N1, MAT_NODE, MODE_RK, OCC, 1>), dim3(grid), bs, 0, s, a);
+        else hipLaunchKernelGGL((stage6_kernel<N1, MAT_NODE, MODE_RK, OCC, 0>), dim3(grid), bs, 0, s, a);
+    }
+}
+
 template <int N1>
 void launch_n(const StageArgs &a, int mat, int mode, int layout, int grid, hipStream_t s) {
     switch (layout) {
@@ -46,6 +65,12 @@ void launch_n(const StageArgs &a, int mat, int mode, int layout, int grid, hipSt
         break;
     case 5:
         if constexpr (N1 <= EPW_MAX_N1) launch_p<N1, 1, 1, 2>(a, mat, mode, grid, s);
+        break;
+    case 6:
+        if constexpr (N1 >= S6_MIN_N1) launch_s6<N1, 0>(a, mat, mode, grid, s);
+        break;
+    case 7:
+        if constexpr (N1 >= S6_MIN_N1) launch_s6<N1, 1>(a, mat, mode, grid, s);
         break;
     default: launch_p<N1, 0, 0, 1>(a, mat, mode, grid, s); break;
     }

@@ -8,12 +8,15 @@
 #include <hip/hip_runtime.h>
 
 #include "adjoint_kernel.hpp"
+#include "stage6_kernel.hpp"
 #include "stage_kernel.hpp"
 
 namespace nkc {
 
 // Orders that have the two-element-per-workgroup layouts 4 and 5.
 constexpr int EPW_MAX_N1 = 6;
+// Orders that have the six-role layouts 6 and 7.
+constexpr int S6_MIN_N1 = 8;
 
 #define NKC_DECL_ORDER(n)                                                                          \
     void launch_stage_n##n(const StageArgs &a, int mat, int mode, int layout, int grid, hipStream_t s); \

@@ -159,6 +159,13 @@ __device__ __forceinline__ void add_incident(const StageArgs &a, int64_t p, doub
     }
 }
 
+// n x v for a general n.
+__device__ __forceinline__ void gcross(const double n[3], const double v[3], double o[3]) {
+    o[0] = n[1] * v[2] - n[2] * v[1];
+    o[1] = n[2] * v[0] - n[0] * v[2];
+    o[2] = n[0] * v[1] - n[1] * v[0];
+}
+
 // n x v for n = sg * e_AX.
 template <int AX>
 __device__ __forceinline__ void ncross(double sg, const double v[3], double o[3]) {

@@ -32,7 +32,7 @@ from .mesh import BoxMesh
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
 
-NLAYOUT = 6   # stage-kernel layouts (NKC_NLAYOUT)
+NLAYOUT = 8   # stage-kernel layouts (NKC_NLAYOUT)
 
 
 def _d(a):

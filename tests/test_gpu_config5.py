@@ -9,7 +9,8 @@ differences of the GPU forward.
 N = 8 and 9 form the z-face fluxes in phase 1 (N = 7 in phase 2); layouts
 2 and 3 read the RK register level by level, 0 and 1 up front; layouts 1
 and 3 pack the four lane roles (at N = 7 the packed workgroup equals the
-unpacked one).  Tolerances as in the other parity tests: 1e-11 of the
+unpacked one); layouts 6 and 7 split the element six ways (free and
+occupancy-bound register budgets).  Tolerances as in the other parity tests: 1e-11 of the
 field's max for one residual, 1e-11 relative L2 (bm1-weighted) per field
 after the steps; the adjoint's gradients within 1e-9 of the oracle's.
 """
@@ -104,7 +105,7 @@ def _steps_reference(oracle, N):
     return _STEPS_REF[N]
 
 
-@pytest.mark.parametrize("layout", [0, 1, 2, 3])
+@pytest.mark.parametrize("layout", [0, 1, 2, 3, 6, 7])
 @pytest.mark.parametrize("N", ORDERS)
 def test_source_drude_steps_every_layout(oracle, N, layout):
     """40 RK steps (graph replay) with the pulse and the Drude block under

@@ -215,8 +215,8 @@ def test_graphene_layouts_bit_identical(oracle, N):
         s, _, _, _, _, _ = _planes(oracle, N=N)
         try:
             s.set_layout(layout)
-        except Exception as err:     # a layout not built for this order
-            assert "not built" in str(err)
+        except Exception as err:     # a layout not built for this order, or without the sheet
+            assert "not built" in str(err) or "carries no" in str(err)
             s.close()
             continue
         s.step(0.002, 4)

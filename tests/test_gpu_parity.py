@@ -329,13 +329,14 @@ def test_lorentz_residual_and_steps_parity(oracle):
     gpu.close()
 
 
-@pytest.mark.parametrize("N", [4, 5, 6, 8, 9])
+@pytest.mark.parametrize("N", [4, 5, 6, 7, 8, 9])
 def test_lane_layouts_bit_identical_and_tuner(N):
     """All stage-kernel layouts built for the order give the same bits
     (graph replay, 3 steps, per-element eps, periodic x, PEC y/z): the four
-    one-element layouts, and at N <= 5 the two two-element ones on an odd
-    element count (the last workgroup's second block is empty); the tuner
-    times them and picks one; a layout not built for the order is refused."""
+    one-element layouts, at N <= 5 the two two-element ones on an odd
+    element count (the last workgroup's second block is empty), at N >= 7
+    the two six-role ones; the tuner times them and picks one; a layout not
+    built for the order is refused."""
     print(f"[case] lane layouts N={N}", flush=True)
     from nekcem_amd._lib import NkcError
     from nekcem_amd.maxwell import MaxwellSolver, NLAYOUT
@@ -347,7 +348,7 @@ def test_lane_layouts_bit_identical_and_tuner(N):
     eps = np.repeat(rng.uniform(1, 4, mesh.nelt), mesh.n1 ** 3)
     H, E = rng.standard_normal((3, n)), rng.standard_normal((3, n))
     dt = mesh.dt(0.25)
-    layouts = range(NLAYOUT) if N <= 5 else range(4)
+    layouts = list(range(6)) if N <= 5 else [0, 1, 2, 3] + ([6, 7] if N >= 7 else [])
     out = []
     for layout in layouts:
         s = MaxwellSolver(mesh, eps=eps)
@@ -359,12 +360,12 @@ def test_lane_layouts_bit_identical_and_tuner(N):
             t = s.tune()
             assert np.isfinite(t["us"][0]) and np.isfinite(t["us"][2]) and t["layout"] in layouts
             assert all(np.isfinite(t["us"][p]) == (N <= 5) for p in (4, 5))
+            assert all(np.isfinite(t["us"][p]) == (N >= 7) for p in (6, 7))
             assert s.get_state()[1].tobytes() == out[-1][1].tobytes()   # tuning leaves the state
-            if N > 5:
-                with pytest.raises(NkcError, match="not built"):
-                    s.set_layout(4)
+            with pytest.raises(NkcError, match="not built"):
+                s.set_layout(4 if N > 5 else 6)
         s.close()
-    for layout, o in enumerate(out[1:], start=1):
+    for layout, o in zip(layouts[1:], out[1:]):
         for a, b in zip(out[0], o):
             assert np.array_equal(a, b), (layout, float(np.max(np.abs(a - b))))
 
@@ -416,4 +417,4 @@ def test_one_element_steps_every_layout(oracle):
         assert parity(gH, gE, out["H"], out["E"], mesh.bmn) <= 1e-11, layout
         gpu.close()
         done += 1
-    assert done == NLAYOUT      # N = 5 builds every layout
+    assert done == 6            # N = 5 builds layouts 0..5

@@ -208,6 +208,14 @@ def test_pml_layouts_bit_identical(oracle):
     for (H, E), P in outs[1:]:
         assert np.array_equal(H, outs[0][0][0]) and np.array_equal(E, outs[0][0][1])
         assert np.array_equal(P, outs[0][1])
+    # the six-role layouts (built at this order) carry no PML: refused
+    from nekcem_amd._lib import NkcError, NKC_ERR_UNSUPPORTED
+    s, _, _, _ = _dielectric(oracle, True, with_oracle=False)
+    for layout in (6, 7):
+        with pytest.raises(NkcError, match="carries no") as ei:
+            s.set_layout(layout)
+        assert ei.value.code == NKC_ERR_UNSUPPORTED
+    s.close()
 
 
 def test_adjoint_refuses_pml(oracle):
