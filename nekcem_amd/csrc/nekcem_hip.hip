@@ -106,6 +106,7 @@ struct DevMesh {
     double *dt_e = nullptr;       // per-element dt (batches), or null
     int32_t *nbr = nullptr;
     int layout = -1;              // stage-kernel layout (PACK | OCC << 1); -1: not tuned yet
+    std::vector<int32_t> xcd_lo;  // 9 element bounds of equal-work XCD ranges, or empty
     std::vector<double> jac, w;   // host copies (source weights)
     // general hexahedra (geo_kernel.hpp): per-node metric quotients and
     // bm1, per-face-node normal + lift, partner node; fco per face node
@@ -162,6 +163,11 @@ void dev_mesh_free(DevMesh &dm) {
 // LDS-bound occupancy (N >= 7 only; no PML, incident field or graphene
 // sheet).  The residual test entry uses layout 0.
 constexpr int NLAYOUT = NKC_NLAYOUT;
+
+// Work of a Drude element's stage relative to a plain one, for the XCD split
+// (E lanes of a Drude element spent 33 against 18 us in the level phase at
+// N = 9, 24^3, a workgroup's life ~1.6 times the others').
+constexpr double DRUDE_WORK = 1.6;
 
 // Elements per workgroup of a layout.
 int layout_epw(int layout) { return (layout == 4 || layout == 5) ? 2 : 1; }
@@ -231,9 +237,20 @@ int32_t launch_stage(const DevMesh &dm, const StageArgs &a, int mode, hipStream_
     const int pk = (mode == MODE_RK && layout_runs(dm.n1, pk0, a)) ? pk0 : 0;
     const int64_t nwg = (dm.nelt + layout_epw(pk) - 1) / layout_epw(pk);
     const int per_xcd = (int)((nwg + 7) / 8);
-    const int grid = 8 * per_xcd;
     StageArgs b = a;
     b.per_xcd = per_xcd;
+    int grid = 8 * per_xcd;
+    b.xcd_bal = 0;
+    if (dm.xcd_lo.size() == 9 && layout_epw(pk) == 1) {
+        int len = 0;
+        for (int x = 0; x < 8; ++x) {
+            b.xcd_lo[x] = dm.xcd_lo[x];
+            len = std::max(len, dm.xcd_lo[x + 1] - dm.xcd_lo[x]);
+        }
+        b.xcd_lo[8] = dm.xcd_lo[8];
+        b.xcd_bal = 1;
+        grid = 8 * std::max(len, 1);
+    }
     switch (dm.n1) {
     case 4: launch_stage_n4(b, dm.mat, mode, pk, grid, s); break;
     case 5: launch_stage_n5(b, dm.mat, mode, pk, grid, s); break;
@@ -1183,6 +1200,24 @@ int32_t nkc_set_source_time(nkc_handle *h, int32_t kind, double omega, double ph
     return drop_graphs(h);                // the captured stages carry the old factor
 }
 
+// Element bounds of 8 contiguous ranges of equal work, one per XCD, when the
+// elements with off[e] >= 0 cost w times the others (elements in order,
+// so each XCD still takes a compact run of the mesh).
+std::vector<int32_t> xcd_ranges(const std::vector<int32_t> &off, double w) {
+    const int64_t n = (int64_t)off.size();
+    double total = 0.0;
+    for (int64_t e = 0; e < n; ++e) total += off[e] >= 0 ? w : 1.0;
+    std::vector<int32_t> lo(9, (int32_t)n);
+    lo[0] = 0;
+    double acc = 0.0;
+    int x = 1;
+    for (int64_t e = 0; e < n && x < 8; ++e) {
+        acc += off[e] >= 0 ? w : 1.0;
+        while (x < 8 && acc >= total * x / 8.0) lo[x++] = (int32_t)(e + 1);
+    }
+    return lo;
+}
+
 int32_t nkc_set_drude(nkc_handle *h, int64_t ndrude, const int32_t *dindex, const double *params,
                       const double *J, const double *kJ) {
     if (!h) return fail(NKC_ERR_ARG, "null handle");
@@ -1198,6 +1233,7 @@ int32_t nkc_set_drude(nkc_handle *h, int64_t ndrude, const int32_t *dindex, cons
     h->ndrude = 0;
     h->ndelem = 0;
     h->drude_slot.clear();
+    h->m.xcd_lo.clear();
     int32_t rc = drop_graphs(h);
     if (rc || ndrude == 0) return rc;
     if (ndrude < 0 || !dindex || !params) return fail(NKC_ERR_ARG, "bad Drude node list");
@@ -1231,6 +1267,7 @@ int32_t nkc_set_drude(nkc_handle *h, int64_t ndrude, const int32_t *dindex, cons
     if ((rc = upload(&h->drude_off, off))) return rc;   // last: switches the term on
     h->ndrude = ndrude;
     h->ndelem = nde;
+    h->m.xcd_lo = xcd_ranges(off, DRUDE_WORK);
     return NKC_OK;
 }
 

@@ -99,7 +99,7 @@ stage6_kernel(StageArgs a) {
     __shared__ double s_f[36 * N2];     // [face][comp 0..5][a + n1 b]
     __shared__ double s_D[N2];
 
-    const int e = remap_element(blockIdx.x, a.nelt, a.per_xcd);
+    const int e = stage_element(a, blockIdx.x, a.nelt);
     if (e >= a.nelt) return;
     const int t = threadIdx.x;
     const int role = t / N2, q = t % N2;
@@ -121,6 +121,9 @@ stage6_kernel(StageArgs a) {
         ld[n] = (v < 6 * N3) ? a.uin[(v / N3) * fs + eo + (v % N3)] : 0.0;
     }
     constexpr bool KPRE = !OCC;
+    // KPF (the occupancy-bound budget without in-stage physics): the first
+    // level's register up front, each next one a level ahead
+    constexpr bool KPF = !KPRE && FEAT == 0;
     double kp[3][KPRE ? KP : 1];
     if (MODE == MODE_RK && act) {
 #pragma unroll
@@ -128,7 +131,7 @@ stage6_kernel(StageArgs a) {
 #pragma unroll
             for (int kk = 0; kk < (KPRE ? KP : 1); ++kk) {
                 const int k = k0 + kk;
-                kp[c][kk] = (KPRE && (a.kflags & 1) && kk < kn)
+                kp[c][kk] = ((KPRE || KPF) && (a.kflags & 1) && kk < kn)
                                 ? ld_stream<NT>(a.kbuf + (3 * grp + c) * fs + eo + k * N2 + q) : 0.0;
             }
     }
@@ -206,6 +209,12 @@ stage6_kernel(StageArgs a) {
     const int dd = (FEAT && grp == 1 && a.drude_off) ? a.drude_off[e] : -1;
     const int lo = (FEAT && grp == 1 && a.lor_off) ? a.lor_off[e] : -1;
     const int co = 3 * grp;
+    // the x face (0 x-, 1 x+) and the y face (2 y-, 3 y+) this lane's nodes
+    // lie on, or -1, and the LDS offsets of their fluxes (level 0)
+    const int xface = (i == 0) ? 0 : (i == N1 - 1 ? 1 : -1);
+    const int yface = (j == 0) ? 2 : (j == N1 - 1 ? 3 : -1);
+    const int xbase = ((xface < 0 ? 0 : xface) * 6 + 3 * grp) * N2 + j;
+    const int ybase = ((yface < 0 ? 2 : yface) * 6 + 3 * grp) * N2 + i;
 #pragma unroll
     for (int kk = 0; kk < KP; ++kk) {
         if (kk >= kn) break;
@@ -213,10 +222,17 @@ stage6_kernel(StageArgs a) {
         const int vloc = k * N2 + q;
         double kold[3];
 #pragma unroll
-        for (int c = 0; c < 3; ++c)
-            kold[c] = (KPRE || MODE != MODE_RK || !(a.kflags & 1))
-                          ? kp[c][KPRE ? kk : 0]
-                          : ld_stream<NT>(a.kbuf + (3 * grp + c) * fs + eo + vloc);
+        for (int c = 0; c < 3; ++c) kold[c] = kp[c][KPRE ? kk : 0];
+        if (KPF && MODE == MODE_RK && (a.kflags & 1) && kk + 1 < kn) {
+            // the next level's RK register, issued before this level's
+            // stores (a load issued after them is waited on together with
+            // them: one in-order counter)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) kp[c][0] = ld_stream<NT>(a.kbuf + (3 * grp + c) * fs + eo + vloc + N2);
+        } else if (!KPRE && !KPF && MODE == MODE_RK && (a.kflags & 1)) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) kold[c] = ld_stream<NT>(a.kbuf + (3 * grp + c) * fs + eo + vloc);
+        }
         double S1_r = 0, S2_r = 0, S0_s = 0, S2_s = 0, S0_t = 0, S1_t = 0;
 #pragma unroll
         for (int l = 0; l < N1; ++l) {
@@ -246,21 +262,15 @@ stage6_kernel(StageArgs a) {
 #pragma unroll
             for (int c = 0; c < 3; ++c) r[c] += s_f[(5 * 6 + co + c) * N2 + q];
         }
-        if (i == 0) {
+        // a node lies on at most one x face and one y face: one lift each,
+        // x before y as in the face order
+        if (xface >= 0) {
 #pragma unroll
-            for (int c = 0; c < 3; ++c) r[c] += s_f[(0 * 6 + co + c) * N2 + j + N1 * k];
+            for (int c = 0; c < 3; ++c) r[c] += s_f[xbase + c * N2 + N1 * k];
         }
-        if (i == N1 - 1) {
+        if (yface >= 0) {
 #pragma unroll
-            for (int c = 0; c < 3; ++c) r[c] += s_f[(1 * 6 + co + c) * N2 + j + N1 * k];
-        }
-        if (j == 0) {
-#pragma unroll
-            for (int c = 0; c < 3; ++c) r[c] += s_f[(2 * 6 + co + c) * N2 + i + N1 * k];
-        }
-        if (j == N1 - 1) {
-#pragma unroll
-            for (int c = 0; c < 3; ++c) r[c] += s_f[(3 * 6 + co + c) * N2 + i + N1 * k];
+            for (int c = 0; c < 3; ++c) r[c] += s_f[ybase + c * N2 + N1 * k];
         }
         const int64_t node = eo + k * N2 + q;
         if (so >= 0) {

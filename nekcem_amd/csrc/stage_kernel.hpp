@@ -145,6 +145,11 @@ struct StageArgs {
     const double *__restrict__ gA;
     double *__restrict__ gAn;
     double *__restrict__ gK;
+    // xcd_bal: XCD x takes elements xcd_lo[x] .. xcd_lo[x + 1] - 1 (ranges
+    // of equal work when some elements cost more, e.g. a Drude region)
+    // instead of equal runs of per_xcd elements (one element per workgroup)
+    int32_t xcd_bal;
+    int32_t xcd_lo[9];
 };
 
 // Incident plane wave at volume node p (global index), time t, added to a
@@ -294,6 +299,16 @@ __device__ __forceinline__ int remap_element(int b, int nelt, int per_xcd) {
     return xcd * per_xcd + slot;
 }
 
+// The element of workgroup b of a one-element-per-workgroup stage launch
+// (nwg when the workgroup has none).
+__device__ __forceinline__ int stage_element(const StageArgs &a, int b, int nwg) {
+    if (a.xcd_bal) {
+        const int x = b & 7, e = a.xcd_lo[x] + (b >> 3);
+        return e < a.xcd_lo[x + 1] ? e : nwg;
+    }
+    return remap_element(b, nwg, a.per_xcd);
+}
+
 template <int N1, int MAT>
 __device__ __forceinline__ void load_coef(const StageArgs &a, int e, int f, int q, double c4[4]) {
     const double *p = (MAT == MAT_ELEM) ? a.fco + ((int64_t)e * 6 + f) * 4
@@ -358,7 +373,8 @@ stage_kernel(StageArgs a) {
     __shared__ double s_D[N2];          // D(i,l) at [i + n1 l]
 
     const int sub = (EPW > 1) ? (int)threadIdx.x / BS : 0;
-    const int e = remap_element(blockIdx.x, (a.nelt + EPW - 1) / EPW, a.per_xcd) * EPW + sub;
+    const int e = (EPW == 1 ? stage_element(a, blockIdx.x, a.nelt)
+                            : remap_element(blockIdx.x, (a.nelt + EPW - 1) / EPW, a.per_xcd) * EPW) + sub;
     if (e - sub >= a.nelt) return;   // whole workgroup leaves together
     // with EPW > 1 the last workgroup's spare blocks stay for the barriers
     const bool valid = EPW == 1 || e < a.nelt;
@@ -390,20 +406,23 @@ stage_kernel(StageArgs a) {
     // the 4-waves/SIMD layouts, level by level in phase 2 (3 KH fewer live
     // doubles, which the 128-VGPR budget would otherwise spill)
     constexpr bool KPRE = !OCC;
+    // KPF (the 4-waves/SIMD layouts without in-stage physics): the first
+    // level's register up front, each next one a level ahead
+    constexpr bool KPF = !KPRE && FEAT == 0;
     double kp[3][KPRE ? KH : 1];
-    if (MODE == MODE_RK && act && !(KPRE && (a.kflags & 1))) {
+    if (MODE == MODE_RK && act) {
 #pragma unroll
         for (int c = 0; c < 3; ++c)
 #pragma unroll
             for (int kk = 0; kk < (KPRE ? KH : 1); ++kk) kp[c][kk] = 0.0;
     }
-    if (KPRE && MODE == MODE_RK && act && (a.kflags & 1)) {
+    if ((KPRE || KPF) && MODE == MODE_RK && act && (a.kflags & 1)) {
 #pragma unroll
         for (int c = 0; c < 3; ++c)
 #pragma unroll
-            for (int kk = 0; kk < KH; ++kk) {
+            for (int kk = 0; kk < (KPRE ? KH : 1); ++kk) {
                 const int k = k0 + kk;
-                kp[c][KPRE ? kk : 0] = (k < N1) ? ld_stream<NT>(a.kbuf + (3 * grp + c) * fs + eo + k * N2 + q) : 0.0;
+                kp[c][kk] = (k < N1) ? ld_stream<NT>(a.kbuf + (3 * grp + c) * fs + eo + k * N2 + q) : 0.0;
             }
     }
     // source amplitude of this role's levels (E roles), with the HBM
@@ -537,6 +556,12 @@ stage_kernel(StageArgs a) {
     const int dd = (FEAT && grp == 1 && a.drude_off) ? a.drude_off[e] : -1;
     const int lo = (FEAT && grp == 1 && a.lor_off) ? a.lor_off[e] : -1;
     const int po = (FEAT == 2 && a.pml_off) ? a.pml_off[e] : -1;
+    // the x face (0 x-, 1 x+) and the y face (2 y-, 3 y+) this lane's nodes
+    // lie on, or -1, and the LDS offsets of their fluxes (level 0)
+    const int xface = (i == 0) ? 0 : (i == N1 - 1 ? 1 : -1);
+    const int yface = (j == 0) ? 2 : (j == N1 - 1 ? 3 : -1);
+    const int xbase = ((xface < 0 ? 0 : xface) * 6 + 3 * grp) * N2 + j;
+    const int ybase = ((yface < 0 ? 2 : yface) * 6 + 3 * grp) * N2 + i;
 #pragma unroll
     for (int kk = 0; kk < KH; ++kk) {
         const int k = k0 + kk;
@@ -544,10 +569,18 @@ stage_kernel(StageArgs a) {
         const int vloc = k * N2 + q;
         double kold[3];
 #pragma unroll
-        for (int c = 0; c < 3; ++c)
-            kold[c] = (KPRE || MODE != MODE_RK || !(a.kflags & 1))
-                          ? kp[c][KPRE ? kk : 0]
-                          : ld_stream<NT>(a.kbuf + (3 * grp + c) * fs + eo + vloc);
+        for (int c = 0; c < 3; ++c) kold[c] = kp[c][KPRE ? kk : 0];
+        if (KPF && MODE == MODE_RK && (a.kflags & 1) && kk + 1 < KH && k + 1 < N1) {
+            // the next level's RK register, issued before this level's
+            // stores: a load issued after stores is waited on together with
+            // them (one in-order counter), so per-level loads would wait
+            // for every level's write acknowledgements
+#pragma unroll
+            for (int c = 0; c < 3; ++c) kp[c][0] = ld_stream<NT>(a.kbuf + (3 * grp + c) * fs + eo + vloc + N2);
+        } else if (!KPRE && !KPF && MODE == MODE_RK && (a.kflags & 1)) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) kold[c] = ld_stream<NT>(a.kbuf + (3 * grp + c) * fs + eo + vloc);
+        }
         double S1_r = 0, S2_r = 0, S0_s = 0, S2_s = 0, S0_t = 0, S1_t = 0;
 #pragma unroll
         for (int l = 0; l < N1; ++l) {
@@ -587,21 +620,15 @@ stage_kernel(StageArgs a) {
             for (int c = 0; c < 3; ++c) r[c] += grp ? FE[c] : FH[c];
         }
         const int co = 3 * grp;
-        if (i == 0) {
+        // a node lies on at most one x face and one y face: one lift each,
+        // x before y as in the face order
+        if (xface >= 0) {
 #pragma unroll
-            for (int c = 0; c < 3; ++c) r[c] += s_f[(0 * 6 + co + c) * N2 + j + N1 * k];
+            for (int c = 0; c < 3; ++c) r[c] += s_f[xbase + c * N2 + N1 * k];
         }
-        if (i == N1 - 1) {
+        if (yface >= 0) {
 #pragma unroll
-            for (int c = 0; c < 3; ++c) r[c] += s_f[(1 * 6 + co + c) * N2 + j + N1 * k];
-        }
-        if (j == 0) {
-#pragma unroll
-            for (int c = 0; c < 3; ++c) r[c] += s_f[(2 * 6 + co + c) * N2 + i + N1 * k];
-        }
-        if (j == N1 - 1) {
-#pragma unroll
-            for (int c = 0; c < 3; ++c) r[c] += s_f[(3 * 6 + co + c) * N2 + i + N1 * k];
+            for (int c = 0; c < 3; ++c) r[c] += s_f[ybase + c * N2 + N1 * k];
         }
         const int64_t node = eo + k * N2 + q;
         if (FEAT == 2 && po >= 0) {
