@@ -241,7 +241,9 @@ int32_t launch_stage(const DevMesh &dm, const StageArgs &a, int mode, hipStream_
     b.per_xcd = per_xcd;
     int grid = 8 * per_xcd;
     b.xcd_bal = 0;
-    if (dm.xcd_lo.size() == 9 && layout_epw(pk) == 1) {
+    // the equal-work ranges: instances with in-stage physics only (a Drude
+    // region sets them), one element per workgroup
+    if (dm.xcd_lo.size() == 9 && layout_epw(pk) == 1 && (a.src_off || a.drude_off || a.lor_off || a.pml_off || a.inc_mask || a.gr_off)) {
         int len = 0;
         for (int x = 0; x < 8; ++x) {
             b.xcd_lo[x] = dm.xcd_lo[x];

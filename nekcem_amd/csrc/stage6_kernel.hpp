@@ -99,7 +99,7 @@ stage6_kernel(StageArgs a) {
     __shared__ double s_f[36 * N2];     // [face][comp 0..5][a + n1 b]
     __shared__ double s_D[N2];
 
-    const int e = stage_element(a, blockIdx.x, a.nelt);
+    const int e = stage_element<(FEAT >= 1)>(a, blockIdx.x, a.nelt);
     if (e >= a.nelt) return;
     const int t = threadIdx.x;
     const int role = t / N2, q = t % N2;

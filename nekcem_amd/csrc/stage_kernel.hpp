@@ -300,9 +300,12 @@ __device__ __forceinline__ int remap_element(int b, int nelt, int per_xcd) {
 }
 
 // The element of workgroup b of a one-element-per-workgroup stage launch
-// (nwg when the workgroup has none).
+// (nwg when the workgroup has none).  BAL: the instance reads the
+// equal-work ranges (those with in-stage physics, whose elements differ in
+// cost; the plain instances keep the registers the table would take).
+template <bool BAL>
 __device__ __forceinline__ int stage_element(const StageArgs &a, int b, int nwg) {
-    if (a.xcd_bal) {
+    if (BAL && a.xcd_bal) {
         const int x = b & 7, e = a.xcd_lo[x] + (b >> 3);
         return e < a.xcd_lo[x + 1] ? e : nwg;
     }
@@ -373,7 +376,7 @@ stage_kernel(StageArgs a) {
     __shared__ double s_D[N2];          // D(i,l) at [i + n1 l]
 
     const int sub = (EPW > 1) ? (int)threadIdx.x / BS : 0;
-    const int e = (EPW == 1 ? stage_element(a, blockIdx.x, a.nelt)
+    const int e = (EPW == 1 ? stage_element<(FEAT >= 1)>(a, blockIdx.x, a.nelt)
                             : remap_element(blockIdx.x, (a.nelt + EPW - 1) / EPW, a.per_xcd) * EPW) + sub;
     if (e - sub >= a.nelt) return;   // whole workgroup leaves together
     // with EPW > 1 the last workgroup's spare blocks stay for the barriers
