@@ -11,7 +11,14 @@
  *   - sizes are int64 (the reference's CUDA prototype used 32-bit sizes,
  *     src/cem_mxm_gpu.cu:34-83);
  *   - every entry point returns an int32 status, NKC_OK = 0; the message of
- *     the last failure on the calling thread is nkc_last_error().
+ *     the last failure on the calling thread is nkc_last_error();
+ *   - the per-node (and per-Drude-node) arrays of nkc_set_state /
+ *     nkc_get_state / nkc_residual / nkc_curl / nkc_set_materials /
+ *     nkc_set_drude / nkc_cem_error_fields / nkc_adjoint_backward /
+ *     nkc_adjoint_get_drude / nkc_adjoint_apply / nkc_batch_set_state /
+ *     nkc_batch_get_state may be host or device memory of the handle's GPU
+ *     (copies use hipMemcpyDefault; tables built on the host stage device
+ *     arrays through host memory first); they are synchronous.
  * Fortran binds these through iso_c_binding (nekcem_amd/fortran/
  * nekcem_hip_mod.f90); Python through ctypes (nekcem_amd/_lib.py).
  */

@@ -17,15 +17,21 @@ per-element Drude damping and plasma frequency as the parameters.
 
 The reference has no adjoint; the forward it differentiates is the step of
 cem_maxwell_op_rk (src/cem_maxwell.F:327-345) with a usersrc-style source.
-PyTorch is only the autograd plumbing: fields move between torch and the
-library through host arrays at the two ends of the run.
+PyTorch is only the autograd plumbing.  The tensors stay where they are:
+with GPU tensors the parameters, the initial state, the probe fields, the
+adjoint seeds and the gradients go between torch and the library as device
+pointers (device-to-device copies); only the material and Drude tables,
+which the library builds on the host, stage their per-node inputs through
+host memory.  CPU tensors go through the same calls as host pointers.
 """
 from __future__ import annotations
 
-from dataclasses import dataclass
+import ctypes as C
+from dataclasses import dataclass, field
 
 import numpy as np
 
+from . import _lib
 from .maxwell import MaxwellSolver
 
 try:
@@ -45,9 +51,11 @@ class ProbeProblem:
     every: int                # checkpoint interval (steps)
     probes: np.ndarray        # node indices (0-based)
     t0: float = 0.0
+    _dev: dict = field(default_factory=dict, repr=False)   # the above as tensors, per device
 
 
 def _run_forward(problem: ProbeProblem, eps: np.ndarray, mu: np.ndarray):
+    """The probe fields for host arrays eps, mu (numpy; finite differences)."""
     s = problem.solver
     s.adjoint_set_grads(True)
     s.set_materials(eps, mu)
@@ -73,9 +81,11 @@ class DrudeProbeProblem:
     dindex: np.ndarray        # listed volume nodes (0-based)
     owner: np.ndarray         # per listed node: its Drude element (0..ne-1)
     t0: float = 0.0
+    _dev: dict = field(default_factory=dict, repr=False)
 
 
 def _run_forward_drude(problem: DrudeProbeProblem, gamma: np.ndarray, wp: np.ndarray):
+    """The probe fields for host arrays gamma, wp (numpy; finite differences)."""
     s = problem.solver
     s.adjoint_set_grads(False)      # only the Drude gradients are asked for
     s.set_drude(problem.dindex, gamma[problem.owner], (wp * wp)[problem.owner])
@@ -86,34 +96,95 @@ def _run_forward_drude(problem: DrudeProbeProblem, gamma: np.ndarray, wp: np.nda
     return np.concatenate([H[:, problem.probes], E[:, problem.probes]])
 
 
-def _seed(probes, n, g):
-    lamH, lamE = np.zeros((3, n)), np.zeros((3, n))
-    for c in range(3):
-        np.add.at(lamH[c], probes, g[c])
-        np.add.at(lamE[c], probes, g[3 + c])
-    return lamH, lamE
-
-
 if torch is not None:
+
+    _dp = C.POINTER(C.c_double)
+
+    def _ptr(t):
+        """double* of a contiguous float64 tensor (host or device memory)."""
+        return None if t is None else C.cast(C.c_void_p(t.data_ptr()), _dp)
+
+    def _ready(*ts):
+        """The library copies synchronously, outside torch's streams: let
+        torch's pending work on these tensors finish first."""
+        for t in ts:
+            if t is not None and t.is_cuda:
+                torch.cuda.synchronize(t.device)
+                return
+
+    def _problem_tensors(problem, dev):
+        """H0, E0 (3, npts), the probe indices (and for a Drude problem the
+        owners and the 1-based node list) on dev, made once per device."""
+        key = str(dev)
+        if key not in problem._dev:
+            n = problem.solver.npts
+            H0 = torch.as_tensor(np.asarray(problem.H0), dtype=torch.float64, device=dev).reshape(3, n).contiguous()
+            E0 = torch.as_tensor(np.asarray(problem.E0), dtype=torch.float64, device=dev).reshape(3, n).contiguous()
+            pr = torch.as_tensor(np.asarray(problem.probes), dtype=torch.long, device=dev)
+            extra = {}
+            if isinstance(problem, DrudeProbeProblem):
+                extra["owner"] = torch.as_tensor(np.asarray(problem.owner), dtype=torch.long, device=dev)
+                extra["dindex"] = np.ascontiguousarray(np.asarray(problem.dindex) + 1, dtype=np.int32)
+            problem._dev[key] = (H0, E0, pr, extra)
+        return problem._dev[key]
+
+    def _start(problem, dev):
+        """Time and state to the problem's start (device to device)."""
+        s = problem.solver
+        H0, E0, _, _ = _problem_tensors(problem, dev)
+        s.time = problem.t0
+        _ready(H0)
+        _lib.check(s._L.nkc_set_state(s._h, _ptr(H0), _ptr(E0), None, None, s.npts))
+
+    def _probe_fields(problem, dev):
+        """The six fields at the probes after the forward, a (6, nprobes)
+        tensor on dev: the state copied device to device, then gathered."""
+        s = problem.solver
+        n = s.npts
+        _, _, pr, _ = _problem_tensors(problem, dev)
+        H = torch.empty((3, n), dtype=torch.float64, device=dev)
+        E = torch.empty((3, n), dtype=torch.float64, device=dev)
+        _ready(H)
+        _lib.check(s._L.nkc_get_state(s._h, _ptr(H), _ptr(E), None, None, n))
+        return torch.cat([H[:, pr], E[:, pr]])
+
+    def _adjoint(problem, grad_out, want_material):
+        """The backward sweep seeded by grad_out at the probes (index_add: a
+        probe listed twice gets both seeds); dL/deps, dL/dmu when asked."""
+        s = problem.solver
+        n = s.npts
+        dev = grad_out.device
+        _, _, pr, _ = _problem_tensors(problem, dev)
+        g = grad_out.detach().to(torch.float64)
+        lamH = torch.zeros((3, n), dtype=torch.float64, device=dev).index_add_(1, pr, g[:3])
+        lamE = torch.zeros((3, n), dtype=torch.float64, device=dev).index_add_(1, pr, g[3:])
+        ge = torch.empty(n, dtype=torch.float64, device=dev) if want_material else None
+        gm = torch.empty(n, dtype=torch.float64, device=dev) if want_material else None
+        _ready(lamH)
+        _lib.check(s._L.nkc_adjoint_backward(s._h, _ptr(lamH), _ptr(lamE), n, _ptr(ge), _ptr(gm)))
+        return ge, gm
 
     class ProbeFields(torch.autograd.Function):
         """(eps, mu) -> fields (Hx, Hy, Hz, Ex, Ey, Ez) at the probes."""
 
         @staticmethod
         def forward(ctx, eps, mu, problem: ProbeProblem):
-            e = eps.detach().cpu().numpy().astype(np.float64)
-            m = mu.detach().cpu().numpy().astype(np.float64)
-            out = _run_forward(problem, e, m)
+            s = problem.solver
+            dev = eps.device
+            e = eps.detach().to(torch.float64).contiguous()
+            m = mu.detach().to(device=dev, dtype=torch.float64).contiguous()
+            s.adjoint_set_grads(True)
+            _ready(e, m)
+            _lib.check(s._L.nkc_set_materials(s._h, _ptr(e), _ptr(m)))
+            _start(problem, dev)
+            s.adjoint_forward(problem.dt, problem.nsteps, problem.every)
             ctx.problem = problem
-            return torch.from_numpy(out).to(eps.device)
+            return _probe_fields(problem, dev)
 
         @staticmethod
         def backward(ctx, grad_out):
-            p = ctx.problem
-            g = grad_out.detach().cpu().numpy().astype(np.float64)
-            ge, gm = p.solver.adjoint_backward(*_seed(p.probes, p.solver.npts, g))
-            dev = grad_out.device
-            return torch.from_numpy(ge).to(dev), torch.from_numpy(gm).to(dev), None
+            ge, gm = _adjoint(ctx.problem, grad_out, True)
+            return ge, gm, None
 
     class DrudeProbeFields(torch.autograd.Function):
         """(gamma, omega_p) per Drude element -> fields at the probes; the
@@ -122,21 +193,38 @@ if torch is not None:
 
         @staticmethod
         def forward(ctx, gamma, wp, problem: DrudeProbeProblem):
-            g = gamma.detach().cpu().numpy().astype(np.float64)
-            w = wp.detach().cpu().numpy().astype(np.float64)
-            out = _run_forward_drude(problem, g, w)
+            s = problem.solver
+            dev = gamma.device
+            _, _, _, extra = _problem_tensors(problem, dev)
+            owner, di = extra["owner"], extra["dindex"]
+            g = gamma.detach().to(torch.float64)
+            w = wp.detach().to(device=dev, dtype=torch.float64)
+            par = torch.stack([g[owner], (w * w)[owner]], dim=1).contiguous()   # (a, b) per listed node
+            s.adjoint_set_grads(False)      # only the Drude gradients are asked for
+            _ready(par)
+            _lib.check(s._L.nkc_set_drude(s._h, len(di), di.ctypes.data_as(C.POINTER(C.c_int32)), _ptr(par),
+                                          None, None))
+            s.ndrude = len(di)
+            _start(problem, dev)
+            s.adjoint_forward(problem.dt, problem.nsteps, problem.every)
             ctx.problem = problem
             ctx.wp = w
-            return torch.from_numpy(out).to(gamma.device)
+            ctx.ne = g.shape[0]
+            return _probe_fields(problem, dev)
 
         @staticmethod
         def backward(ctx, grad_out):
             p = ctx.problem
-            g = grad_out.detach().cpu().numpy().astype(np.float64)
-            p.solver.adjoint_backward(*_seed(p.probes, p.solver.npts, g))
-            ga, gb = p.solver.adjoint_drude_grad()
-            ne = len(ctx.wp)
-            g_gamma = np.bincount(p.owner, weights=ga, minlength=ne)
-            g_wp = 2.0 * ctx.wp * np.bincount(p.owner, weights=gb, minlength=ne)
+            s = p.solver
             dev = grad_out.device
-            return torch.from_numpy(g_gamma).to(dev), torch.from_numpy(g_wp).to(dev), None
+            _adjoint(p, grad_out, False)
+            _, _, _, extra = _problem_tensors(p, dev)
+            owner = extra["owner"]
+            nd = len(extra["dindex"])
+            ga = torch.empty(nd, dtype=torch.float64, device=dev)
+            gb = torch.empty(nd, dtype=torch.float64, device=dev)
+            _ready(ga)
+            _lib.check(s._L.nkc_adjoint_get_drude(s._h, _ptr(ga), _ptr(gb)))
+            g_gamma = torch.zeros(ctx.ne, dtype=torch.float64, device=dev).index_add_(0, owner, ga)
+            g_wp = 2.0 * ctx.wp * torch.zeros(ctx.ne, dtype=torch.float64, device=dev).index_add_(0, owner, gb)
+            return g_gamma, g_wp, None

@@ -782,13 +782,15 @@ int32_t make_handle(const HostMesh &hm, double C0, nkc_handle **out) {
     return NKC_OK;
 }
 
-// copy 6 fields between host (3 + 3 components at stride ld) and device
+// copy 6 fields between the caller's arrays (3 + 3 components at stride
+// ld, in host or device memory: hipMemcpyDefault tells them apart) and the
+// library's device blocks
 int32_t copy_fields_h2d(double *dst, int64_t fs, int64_t dst_off, const double *hn, const double *en,
                         int64_t n, int64_t ld) {
     const size_t nb = sizeof(double) * n;
     for (int c = 0; c < 3; ++c) {
-        HIPCHK(hipMemcpy(dst + c * fs + dst_off, hn + c * ld, nb, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(dst + (3 + c) * fs + dst_off, en + c * ld, nb, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(dst + c * fs + dst_off, hn + c * ld, nb, hipMemcpyDefault));
+        HIPCHK(hipMemcpy(dst + (3 + c) * fs + dst_off, en + c * ld, nb, hipMemcpyDefault));
     }
     return NKC_OK;
 }
@@ -797,9 +799,18 @@ int32_t copy_fields_d2h(const double *src, int64_t fs, int64_t src_off, double *
                         int64_t n, int64_t ld) {
     const size_t nb = sizeof(double) * n;
     for (int c = 0; c < 3; ++c) {
-        HIPCHK(hipMemcpy(hn + c * ld, src + c * fs + src_off, nb, hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(en + c * ld, src + (3 + c) * fs + src_off, nb, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(hn + c * ld, src + c * fs + src_off, nb, hipMemcpyDefault));
+        HIPCHK(hipMemcpy(en + c * ld, src + (3 + c) * fs + src_off, nb, hipMemcpyDefault));
     }
+    return NKC_OK;
+}
+
+// A caller's array (host or device memory) as a host vector, for the entry
+// points that build their tables on the host.
+template <class T>
+int32_t stage_host(std::vector<T> &out, const T *src, int64_t n) {
+    out.resize((size_t)n);
+    if (n > 0) HIPCHK(hipMemcpy(out.data(), src, sizeof(T) * (size_t)n, hipMemcpyDefault));
     return NKC_OK;
 }
 
@@ -1036,9 +1047,9 @@ int32_t nkc_set_state(nkc_handle *h, const double *hn, const double *en, const d
     if (rc) return rc;
     const size_t nb = sizeof(double) * m.npts;
     for (int c = 0; c < 3; ++c) {
-        if (khn) HIPCHK(hipMemcpy(m.k + c * m.fs, khn + c * ld, nb, hipMemcpyHostToDevice));
+        if (khn) HIPCHK(hipMemcpy(m.k + c * m.fs, khn + c * ld, nb, hipMemcpyDefault));
         else HIPCHK(hipMemset(m.k + c * m.fs, 0, nb));
-        if (ken) HIPCHK(hipMemcpy(m.k + (3 + c) * m.fs, ken + c * ld, nb, hipMemcpyHostToDevice));
+        if (ken) HIPCHK(hipMemcpy(m.k + (3 + c) * m.fs, ken + c * ld, nb, hipMemcpyDefault));
         else HIPCHK(hipMemset(m.k + (3 + c) * m.fs, 0, nb));
     }
     return NKC_OK;
@@ -1220,9 +1231,23 @@ std::vector<int32_t> xcd_ranges(const std::vector<int32_t> &off, double w) {
     return lo;
 }
 
-int32_t nkc_set_drude(nkc_handle *h, int64_t ndrude, const int32_t *dindex, const double *params,
-                      const double *J, const double *kJ) {
+int32_t nkc_set_drude(nkc_handle *h, int64_t ndrude, const int32_t *dindex_in, const double *params_in,
+                      const double *J_in, const double *kJ_in) {
     if (!h) return fail(NKC_ERR_ARG, "null handle");
+    // the blocks are built on the host from the caller's arrays (host or device memory)
+    std::vector<int32_t> dindex_h;
+    std::vector<double> params_h, J_h, kJ_h;
+    if (ndrude > 0) {
+        int32_t rs;
+        if ((dindex_in && (rs = stage_host(dindex_h, dindex_in, ndrude))) ||
+            (params_in && (rs = stage_host(params_h, params_in, 2 * ndrude))) ||
+            (J_in && (rs = stage_host(J_h, J_in, 3 * ndrude))) || (kJ_in && (rs = stage_host(kJ_h, kJ_in, 3 * ndrude))))
+            return rs;
+    }
+    const int32_t *dindex = dindex_in ? dindex_h.data() : nullptr;
+    const double *params = params_in ? params_h.data() : nullptr;
+    const double *J = J_in ? J_h.data() : nullptr;
+    const double *kJ = kJ_in ? kJ_h.data() : nullptr;
     if (h->m.general && (ndrude > 0)) return general_refuses(h, "a Drude region");
     HIPCHK(hipStreamSynchronize(h->stream));
     (void)hipFree(h->drude_off); h->drude_off = nullptr;
@@ -1947,10 +1972,15 @@ int32_t nkc_tune(nkc_handle *h, double *out) {
     return NKC_OK;
 }
 
-int32_t nkc_set_materials(nkc_handle *h, const double *eps, const double *mu) {
-    if (!h || !eps || !mu) return fail(NKC_ERR_ARG, "null handle or material array");
+int32_t nkc_set_materials(nkc_handle *h, const double *eps_in, const double *mu_in) {
+    if (!h || !eps_in || !mu_in) return fail(NKC_ERR_ARG, "null handle or material array");
     if (h->m.general && (true)) return general_refuses(h, "replacing the materials");
     const int64_t npts = h->m.npts;
+    // the tables are built on the host from the caller's arrays (host or device memory)
+    std::vector<double> eps_h, mu_h;
+    int32_t rc0;
+    if ((rc0 = stage_host(eps_h, eps_in, npts)) || (rc0 = stage_host(mu_h, mu_in, npts))) return rc0;
+    const double *eps = eps_h.data(), *mu = mu_h.data();
     for (int64_t p = 0; p < npts; ++p)
         if (!(eps[p] > 0.0) || !(mu[p] > 0.0)) return fail(NKC_ERR_ARG, "material not positive at node %lld", (long long)p);
     HIPCHK(hipStreamSynchronize(h->stream));
@@ -2324,8 +2354,8 @@ int32_t nkc_adjoint_backward(nkc_handle *h, const double *lamH, const double *la
     }
     if ((rc = set_time(h, t_end))) return rc;
     A.have_grad = true;
-    if (geps) HIPCHK(hipMemcpy(geps, A.geps, sizeof(double) * m.npts, hipMemcpyDeviceToHost));
-    if (gmu) HIPCHK(hipMemcpy(gmu, A.gmu, sizeof(double) * m.npts, hipMemcpyDeviceToHost));
+    if (geps) HIPCHK(hipMemcpy(geps, A.geps, sizeof(double) * m.npts, hipMemcpyDefault));
+    if (gmu) HIPCHK(hipMemcpy(gmu, A.gmu, sizeof(double) * m.npts, hipMemcpyDefault));
     return NKC_OK;
 }
 
@@ -2335,7 +2365,7 @@ int32_t nkc_adjoint_get_drude(nkc_handle *h, double *ga, double *gb) {
     if (!h->adj.gA || !h->adj.have_grad) return fail(NKC_ERR_ARG, "no nkc_adjoint_backward run with this Drude region");
     HIPCHK(hipStreamSynchronize(h->stream));
     const int64_t n3 = (int64_t)h->m.n1 * h->m.n1 * h->m.n1;
-    std::vector<double> buf(drude_size(h) / 3);
+    std::vector<double> buf(drude_size(h) / 3), out((size_t)h->ndrude);
     double *dst[2] = {ga, gb};
     const double *srcp[2] = {h->adj.gA, h->adj.gB};
     for (int w = 0; w < 2; ++w) {
@@ -2343,8 +2373,9 @@ int32_t nkc_adjoint_get_drude(nkc_handle *h, double *ga, double *gb) {
         HIPCHK(hipMemcpy(buf.data(), srcp[w], sizeof(double) * buf.size(), hipMemcpyDeviceToHost));
         for (int64_t l = 0; l < h->ndrude; ++l) {
             const int64_t b = h->drude_slot[l];          // dd 3 n3 + vloc
-            dst[w][l] = buf[(b / (3 * n3)) * n3 + b % (3 * n3)];
+            out[l] = buf[(b / (3 * n3)) * n3 + b % (3 * n3)];
         }
+        HIPCHK(hipMemcpy(dst[w], out.data(), sizeof(double) * out.size(), hipMemcpyDefault));   // host or device
     }
     return NKC_OK;
 }
