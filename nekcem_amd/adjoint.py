@@ -23,6 +23,9 @@ adjoint seeds and the gradients go between torch and the library as device
 pointers (device-to-device copies); only the material and Drude tables,
 which the library builds on the host, stage their per-node inputs through
 host memory.  CPU tensors go through the same calls as host pointers.
+torch ships its own HIP runtime; the two share one HSA runtime, and torch
+sees the GPU, only if torch's is loaded first: import torch (and call
+torch.cuda.init()) before the first MaxwellSolver of the process.
 """
 from __future__ import annotations
 
