@@ -303,7 +303,7 @@ adjoint_kernel(AdjArgs a) {
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
             vo[c] = s_v[c * N3 + S::idx(vi, vj, vk)];
-            uo[c] = a.us[c * fs + own];
+            uo[c] = GM ? a.us[c * fs + own] : 0.0;    // the jumps feed the eps/mu gradient only
         }
         double Yn = 1.0, Zn = 1.0, Ln = 0.0;
         const NodeMat mo = node_mat<N1, MAT>(a, e, own);
@@ -316,7 +316,7 @@ adjoint_kernel(AdjArgs a) {
             for (int c = 0; c < 6; ++c) {
                 const double pm = (c < 3) ? mn.im : mn.ie;
                 vn[c] = pm * (a.kap_in[c * fs + pn] + a.rkb * a.lam_in[c * fs + pn]);
-                un[c] = a.us[c * fs + pn];
+                un[c] = GM ? a.us[c * fs + pn] : 0.0;
             }
             Ln = a.geo[(int64_t)nb * 9 + 3 + (r ^ 1)];
         } else {
@@ -358,7 +358,7 @@ adjoint_kernel(AdjArgs a) {
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
             vo[c] = s_v[c * N3 + S::idx(i, j, k)];
-            uo[c] = a.us[c * fs + node];
+            uo[c] = GM ? a.us[c * fs + node] : 0.0;
         }
         double Yn = 1.0, Zn = 1.0, Ln = 0.0;
         const NodeMat mo = node_mat<N1, MAT>(a, e, node);
@@ -371,7 +371,7 @@ adjoint_kernel(AdjArgs a) {
             for (int c = 0; c < 6; ++c) {
                 const double pm = (c < 3) ? mn.im : mn.ie;
                 vn[c] = pm * (a.kap_in[c * fs + pn] + a.rkb * a.lam_in[c * fs + pn]);
-                un[c] = a.us[c * fs + pn];
+                un[c] = GM ? a.us[c * fs + pn] : 0.0;
             }
             Ln = a.geo[(int64_t)nbz * 9 + 3 + (zface ^ 1)];
         } else {
