@@ -512,6 +512,10 @@ struct nkc_handle {
         // (5 S stages) and reversed.  S = every when the store fits.
         int64_t S = 0, nsub_max = 0;
         int64_t sub_seg = -1;             // segment whose window starts `sub` holds
+        // sub_all: `sub` holds the window starts of every segment, kept by
+        // the forward (S < every and room for them), so the backward needs
+        // no pass to rebuild them
+        bool sub_all = false;
         // The store has K window slots; slot_id[k] = seg * nsub_max + q of
         // the window slot k holds (-1: none).  K > 1 only when windows are
         // whole segments: the forward then keeps its last K segments.
@@ -522,8 +526,8 @@ struct nkc_handle {
         bool have_grad = false;           // a backward sweep has filled the gradients
         double dt = 0.0, t0 = 0.0;
         double *ckpt = nullptr;           // nseg x 6 fs: the state at the start of each segment
-        double *sub = nullptr;            // nsub_max x 6 fs: window start states
-        double *subJ = nullptr;           // nsub_max x jsz
+        double *sub = nullptr;            // (sub_all ? nseg : 1) x nsub_max x 6 fs: window start states
+        double *subJ = nullptr;           // the same x jsz
         double *st_u = nullptr;           // K x (5 S + 1) x 6 fs: every stage input of a window
         double *st_r = nullptr;           // K x 5 S x 6 fs: every stage residual
         double *lam[2] = {nullptr, nullptr}, *kap[2] = {nullptr, nullptr};
@@ -2078,10 +2082,21 @@ int32_t alloc_adjoint(nkc_handle *h) {
     A.cur_slot = 0;
     if ((rc = dev_alloc_counted(h, &A.st_u, blk * (5 * S + 1) * K))) return rc;
     if (gm && (rc = dev_alloc_counted(h, &A.st_r, blk * 5 * S * K))) return rc;
-    if ((rc = dev_alloc_counted(h, &A.sub, blk * A.nsub_max))) return rc;
+    // every segment's window starts, when the windows split segments and
+    // they fit in a quarter of the free memory: the backward then recomputes
+    // each window once, with no pass to rebuild its segment's window starts
+    A.sub_all = false;
+    if (A.nsub_max > 1) {
+        size_t fr = 0, tot = 0;
+        HIPCHK(hipMemGetInfo(&fr, &tot));
+        const size_t need = sizeof(double) * (blk + jsz) * (size_t)(A.nseg * A.nsub_max);
+        A.sub_all = need <= fr / 4;
+    }
+    const size_t nsub = (size_t)((A.sub_all ? A.nseg : 1) * A.nsub_max);
+    if ((rc = dev_alloc_counted(h, &A.sub, blk * nsub))) return rc;
     if (h->ndrude) {
         if ((rc = dev_alloc_counted(h, &A.stJ, jsz * 5 * S * K))) return rc;
-        if ((rc = dev_alloc_counted(h, &A.subJ, jsz * A.nsub_max))) return rc;
+        if ((rc = dev_alloc_counted(h, &A.subJ, jsz * nsub))) return rc;
     }
     return NKC_OK;
 }
@@ -2139,6 +2154,11 @@ int find_slot(const nkc_handle *h, int64_t seg, int64_t q) {
     return -1;
 }
 
+// Where window q of segment seg keeps its start state (and current).
+int64_t sub_index(const nkc_handle *h, int64_t seg, int64_t q) {
+    return (h->adj.sub_all ? seg * h->adj.nsub_max : 0) + q;
+}
+
 // Segment seg from the state in u[cur] (and dJ) at its start time: keep
 // every window's start state, run all windows but the last with the plain
 // step, and the last through the stage store.
@@ -2148,8 +2168,8 @@ int32_t run_segment_windows(nkc_handle *h, int64_t seg) {
     const int64_t nq = nwin(h, seg);
     int32_t rc;
     for (int64_t q = 0; q < nq; ++q) {
-        HIPCHK(d2d(h, A.sub + q * blk, h->m.u[h->cur], blk));
-        if (h->ndrude) HIPCHK(d2d(h, A.subJ + q * jsz, h->dJ, jsz));
+        HIPCHK(d2d(h, A.sub + sub_index(h, seg, q) * blk, h->m.u[h->cur], blk));
+        if (h->ndrude) HIPCHK(d2d(h, A.subJ + sub_index(h, seg, q) * jsz, h->dJ, jsz));
         if (q + 1 < nq) {
             if ((rc = nkc_step(h, nullptr, A.dt, win_steps(h, seg, q)))) return rc;
         } else {
@@ -2263,7 +2283,14 @@ int32_t nkc_adjoint_forward(nkc_handle *h, double *time, double dt, int64_t nste
         if (fin) HIPCHK(d2d(h, h->m.u[h->cur], fin, blk));   // state after the previous kept segment
         HIPCHK(d2d(h, A.ckpt + seg * blk, h->m.u[h->cur], blk));
         if (h->ndrude) HIPCHK(d2d(h, A.ckJ + seg * jsz, h->dJ, jsz));
-        if (seg < first_kept) {
+        if (seg < first_kept && A.sub_all && nwin(h, seg) > 1) {
+            // keep the segment's window starts for the backward sweep
+            for (int64_t q = 0; q < nwin(h, seg); ++q) {
+                HIPCHK(d2d(h, A.sub + sub_index(h, seg, q) * blk, h->m.u[h->cur], blk));
+                if (h->ndrude) HIPCHK(d2d(h, A.subJ + sub_index(h, seg, q) * jsz, h->dJ, jsz));
+                if ((rc = nkc_step(h, nullptr, dt, win_steps(h, seg, q)))) return rc;
+            }
+        } else if (seg < first_kept) {
             if ((rc = nkc_step(h, nullptr, dt, A.every))) return rc;
         } else if (A.K > 1) {
             A.cur_slot = (int)(seg - first_kept);
@@ -2323,7 +2350,7 @@ int32_t nkc_adjoint_backward(nkc_handle *h, const double *lamH, const double *la
         const int64_t nq = nwin(h, seg);
         for (int64_t q = nq - 1; q >= 0; --q) {
             int k = find_slot(h, seg, q);
-            if (k < 0 && nq > 1 && A.sub_seg != seg) {
+            if (k < 0 && nq > 1 && !A.sub_all && A.sub_seg != seg) {
                 // window starts of this segment (and its last window's stages)
                 HIPCHK(d2d(h, m.u[h->cur], A.ckpt + seg * blk, blk));
                 if ((rc = start_from(A.ckJ + seg * jsz, seg * A.every))) return rc;
@@ -2334,8 +2361,8 @@ int32_t nkc_adjoint_backward(nkc_handle *h, const double *lamH, const double *la
                 // recompute the window into slot 0 (the kept segments, which
                 // come last in time, are reversed before it is reused)
                 A.cur_slot = 0;
-                const double *u0 = nq > 1 ? A.sub + q * blk : A.ckpt + seg * blk;
-                const double *J0 = nq > 1 ? A.subJ + q * jsz : A.ckJ + seg * jsz;
+                const double *u0 = nq > 1 ? A.sub + sub_index(h, seg, q) * blk : A.ckpt + seg * blk;
+                const double *J0 = nq > 1 ? A.subJ + sub_index(h, seg, q) * jsz : A.ckJ + seg * jsz;
                 HIPCHK(d2d(h, slot_u(h, 0), u0, blk));
                 if ((rc = start_from(J0, seg * A.every + q * A.S))) return rc;
                 if ((rc = store_window(h, seg, q))) return rc;
