@@ -72,6 +72,7 @@ struct StageArgs {
     double *__restrict__ kbuf;        // MODE_RK: RK register, in place
     double *__restrict__ res;         // MODE_RESIDUAL: residual out
     const double *__restrict__ D;     // n1*n1, D(i,l) at D[i + n1*l]
+    const double *__restrict__ Dt;    // its transpose, D(i,l) at Dt[l + n1*i] (a row per level)
     const double *__restrict__ geo;   // per element: gx gy gz L0..L5 (9)
     const int32_t *__restrict__ nbr;  // per element: 6 neighbour ids / FaceType
     const double *__restrict__ fco;   // ELEM: e*6*4 ; NODE: (e*6+f)*n2*4
@@ -384,7 +385,12 @@ stage_kernel(StageArgs a) {
     double *const s_u = s_u_all[sub];
     double *const s_f = s_f_all[sub];
     const int t = (EPW > 1) ? (int)threadIdx.x % BS : (int)threadIdx.x;
-    const int role = t / LANES, q = t % LANES;
+    // with whole waves per role (PACK = 0) the role, and with it the
+    // update group, the level half and the face a lane works on, is the
+    // same across a wave: the compiler is told so and keeps them (and what
+    // derives from them) in scalar registers
+    const int role = PACK ? t / LANES : __builtin_amdgcn_readfirstlane(t / LANES);
+    const int q = t % LANES;
     const bool act = valid && q < N2 && role < S::ROLES;
     const int i = q % N1, j = q / N1;
     const int grp = role & 1;
