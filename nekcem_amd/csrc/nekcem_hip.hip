@@ -102,7 +102,7 @@ struct DevMesh {
     int64_t nelt = 0, npts = 0, fs = 0;
     double *u[2] = {nullptr, nullptr};
     double *k = nullptr;
-    double *D = nullptr, *Dt = nullptr, *geo = nullptr, *fco = nullptr, *ieps = nullptr, *imu = nullptr;
+    double *D = nullptr, *geo = nullptr, *fco = nullptr, *ieps = nullptr, *imu = nullptr;
     double *dt_e = nullptr;       // per-element dt (batches), or null
     int32_t *nbr = nullptr;
     int layout = -1;              // stage-kernel layout (PACK | OCC << 1); -1: not tuned yet
@@ -132,12 +132,6 @@ int32_t dev_mesh_upload(const HostMesh &m, DevMesh &dm) {
     HIPCHK(hipMemset(dm.k, 0, ub));
     int32_t rc;
     if ((rc = upload(&dm.D, m.D))) return rc;
-    {
-        std::vector<double> dt(m.D.size());
-        for (int i = 0; i < m.n1; ++i)
-            for (int l = 0; l < m.n1; ++l) dt[l + m.n1 * i] = m.D[i + m.n1 * l];
-        if ((rc = upload(&dm.Dt, dt))) return rc;
-    }
     if ((rc = upload(&dm.geo, m.geo))) return rc;
     if ((rc = upload(&dm.nbr, m.nbr))) return rc;
     if ((rc = upload(&dm.fco, m.fco))) return rc;
@@ -155,7 +149,7 @@ int32_t dev_mesh_upload(const HostMesh &m, DevMesh &dm) {
 }
 
 void dev_mesh_free(DevMesh &dm) {
-    double *p[] = {dm.u[0], dm.u[1], dm.k, dm.D, dm.Dt, dm.geo, dm.fco, dm.ieps, dm.imu, dm.dt_e, dm.gmet, dm.fgeo, dm.bmw};
+    double *p[] = {dm.u[0], dm.u[1], dm.k, dm.D, dm.geo, dm.fco, dm.ieps, dm.imu, dm.dt_e, dm.gmet, dm.fgeo, dm.bmw};
     for (double *q : p) (void)hipFree(q);
     (void)hipFree(dm.nbr);
     (void)hipFree(dm.fnb);
@@ -276,7 +270,6 @@ int32_t launch_stage(const DevMesh &dm, const StageArgs &a, int mode, hipStream_
 StageArgs mesh_args(const DevMesh &dm) {
     StageArgs a{};
     a.D = dm.D;
-    a.Dt = dm.Dt;
     a.geo = dm.geo;
     a.nbr = dm.nbr;
     a.fco = dm.fco;

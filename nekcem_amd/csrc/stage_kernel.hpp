@@ -72,7 +72,6 @@ struct StageArgs {
     double *__restrict__ kbuf;        // MODE_RK: RK register, in place
     double *__restrict__ res;         // MODE_RESIDUAL: residual out
     const double *__restrict__ D;     // n1*n1, D(i,l) at D[i + n1*l]
-    const double *__restrict__ Dt;    // its transpose, D(i,l) at Dt[l + n1*i] (a row per level)
     const double *__restrict__ geo;   // per element: gx gy gz L0..L5 (9)
     const int32_t *__restrict__ nbr;  // per element: 6 neighbour ids / FaceType
     const double *__restrict__ fco;   // ELEM: e*6*4 ; NODE: (e*6+f)*n2*4
