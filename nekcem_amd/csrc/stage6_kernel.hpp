@@ -215,6 +215,15 @@ stage6_kernel(StageArgs a) {
     const int yface = (j == 0) ? 2 : (j == N1 - 1 ? 3 : -1);
     const int xbase = ((xface < 0 ? 0 : xface) * 6 + 3 * grp) * N2 + j;
     const int ybase = ((yface < 0 ? 2 : yface) * 6 + 3 * grp) * N2 + i;
+    // the plain N = 8 instance holds this lane's r row of D in registers
+    // (fits its 128-VGPR budget): N+1 fewer LDS reads per level, which
+    // bound the level loop at this order
+    constexpr bool DREGR = N1 == 9 && FEAT == 0;
+    double Dr[DREGR ? N1 : 1];
+    if (DREGR) {
+#pragma unroll
+        for (int l = 0; l < N1; ++l) Dr[l] = s_D[i + N1 * l];
+    }
 #pragma unroll
     for (int kk = 0; kk < KP; ++kk) {
         if (kk >= kn) break;
@@ -238,7 +247,7 @@ stage6_kernel(StageArgs a) {
         for (int l = 0; l < N1; ++l) {
             const int pr = S::idx(l, j, k), ps = S::idx(i, l, k), pt = S::idx(i, j, l);
             const double dkl = s_D[k + N1 * l];
-            const double drl = s_D[i + N1 * l];
+            const double drl = DREGR ? Dr[DREGR ? l : 0] : s_D[i + N1 * l];
             const double dsl = s_D[j + N1 * l];
             S1_r += drl * S1[pr];
             S2_r += drl * S2[pr];

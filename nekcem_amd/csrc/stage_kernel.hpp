@@ -312,6 +312,15 @@ __device__ __forceinline__ int stage_element(const StageArgs &a, int b, int nwg)
     return remap_element(b, nwg, a.per_xcd);
 }
 
+// Value of v in lane `lane` of the wave (lane a compile-time constant or
+// wave-uniform), for every lane: two v_readlane_b32 into a scalar pair.
+__device__ __forceinline__ double bcast_lane(double v, int lane) {
+    const long long b = __builtin_bit_cast(long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, lane);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __builtin_bit_cast(double, (long long)(((unsigned long long)hi << 32) | lo));
+}
+
 template <int N1, int MAT>
 __device__ __forceinline__ void load_coef(const StageArgs &a, int e, int f, int q, double c4[4]) {
     const double *p = (MAT == MAT_ELEM) ? a.fco + ((int64_t)e * 6 + f) * 4
@@ -537,14 +546,31 @@ stage_kernel(StageArgs a) {
     const double *S0 = s_u + (src + 0) * N3;
     const double *S1 = s_u + (src + 1) * N3;
     const double *S2 = s_u + (src + 2) * N3;
-    // D rows of this lane's column cached in registers, except at high
-    // orders where 2 (N+1) doubles would cost occupancy: read from LDS
-    constexpr bool DREG = N1 <= 8 && !OCC;
-    double Dr[DREG ? N1 : 1], Ds[DREG ? N1 : 1];
+    // D rows of this lane's column cached in registers, except where
+    // 2 (N+1) doubles would cost occupancy: read from LDS.  At N = 8, 9 with
+    // whole waves per role the 4-waves layouts, which read the RK register
+    // level by level, hold both rows within their 128 VGPRs (not the PML
+    // instance, nor the source / Drude one at N = 9, which would spill), the
+    // free-budget ones the r row only: the level loop's LDS reads, which
+    // bound it at these orders, drop from 9 (N+1) per level to 6 (N+1)
+    constexpr bool DREG = (N1 <= 8 && !OCC) || (N1 >= 9 && !PACK && OCC && (FEAT == 0 || (FEAT == 1 && N1 == 9)));
+    constexpr bool DREGR = DREG || (N1 >= 9 && !PACK && FEAT < 2);
+    double Dr[DREGR ? N1 : 1], Ds[DREG ? N1 : 1];
+    if (DREGR) {
+#pragma unroll
+        for (int l = 0; l < N1; ++l) Dr[l] = s_D[i + N1 * l];
+    }
     if (DREG) {
 #pragma unroll
-        for (int l = 0; l < N1; ++l) { Dr[l] = s_D[i + N1 * l]; Ds[l] = s_D[j + N1 * l]; }
+        for (int l = 0; l < N1; ++l) Ds[l] = s_D[j + N1 * l];
     }
+    // t derivative at N = 8, 9 with whole waves per role: the level k is
+    // the same across the wave, so lane l reads D(k, l) once per level and
+    // the wave takes the N+1 entries from lanes 0..N (always active) by
+    // readlane, in place of N+1 LDS reads (not in the free-budget source /
+    // Drude instance, whose registers it would push past 128)
+    constexpr bool DTL = N1 >= 9 && !PACK && (FEAT == 0 || (FEAT == 1 && OCC));
+    const int wl = (int)threadIdx.x & 63;
     double minv = 0.0;
     if (MAT == MAT_ELEM) minv = grp ? a.ieps[e] : a.imu[e];
     const double dt = a.dt_e ? a.dt_e[e] : a.dt;
@@ -590,11 +616,12 @@ stage_kernel(StageArgs a) {
             for (int c = 0; c < 3; ++c) kold[c] = ld_stream<NT>(a.kbuf + (3 * grp + c) * fs + eo + vloc);
         }
         double S1_r = 0, S2_r = 0, S0_s = 0, S2_s = 0, S0_t = 0, S1_t = 0;
+        const double dkrow = DTL ? s_D[k + N1 * (wl < N1 ? wl : 0)] : 0.0;
 #pragma unroll
         for (int l = 0; l < N1; ++l) {
             const int pr = S::idx(l, j, k), ps = S::idx(i, l, k), pt = S::idx(i, j, l);
-            const double dkl = s_D[k + N1 * l];
-            const double drl = DREG ? Dr[DREG ? l : 0] : s_D[i + N1 * l];
+            const double dkl = DTL ? bcast_lane(dkrow, l) : s_D[k + N1 * l];
+            const double drl = DREGR ? Dr[DREGR ? l : 0] : s_D[i + N1 * l];
             const double dsl = DREG ? Ds[DREG ? l : 0] : s_D[j + N1 * l];
             S1_r += drl * S1[pr];
             S2_r += drl * S2[pr];
