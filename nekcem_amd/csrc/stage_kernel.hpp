@@ -568,8 +568,8 @@ stage_kernel(StageArgs a) {
     // the same across the wave, so lane l reads D(k, l) once per level and
     // the wave takes the N+1 entries from lanes 0..N (always active) by
     // readlane, in place of N+1 LDS reads (not in the free-budget source /
-    // Drude instance, whose registers it would push past 128)
-    constexpr bool DTL = N1 >= 9 && !PACK && (FEAT == 0 || (FEAT == 1 && OCC));
+    // Drude instance at N = 8, whose registers it would push past 128)
+    constexpr bool DTL = N1 >= 9 && !PACK && (FEAT == 0 || (FEAT == 1 && (OCC || N1 == 10)));
     const int wl = (int)threadIdx.x & 63;
     double minv = 0.0;
     if (MAT == MAT_ELEM) minv = grp ? a.ieps[e] : a.imu[e];
