@@ -219,10 +219,15 @@ stage6_kernel(StageArgs a) {
     // (fits its 128-VGPR budget): N+1 fewer LDS reads per level, which
     // bound the level loop at this order
     constexpr bool DREGR = N1 == 9 && FEAT == 0;
-    double Dr[DREGR ? N1 : 1];
+    constexpr bool DREGS = DREGR && OCC;
+    double Dr[DREGR ? N1 : 1], Ds[DREGS ? N1 : 1];
     if (DREGR) {
 #pragma unroll
         for (int l = 0; l < N1; ++l) Dr[l] = s_D[i + N1 * l];
+    }
+    if (DREGS) {
+#pragma unroll
+        for (int l = 0; l < N1; ++l) Ds[l] = s_D[j + N1 * l];
     }
 #pragma unroll
     for (int kk = 0; kk < KP; ++kk) {
@@ -248,7 +253,7 @@ stage6_kernel(StageArgs a) {
             const int pr = S::idx(l, j, k), ps = S::idx(i, l, k), pt = S::idx(i, j, l);
             const double dkl = s_D[k + N1 * l];
             const double drl = DREGR ? Dr[DREGR ? l : 0] : s_D[i + N1 * l];
-            const double dsl = s_D[j + N1 * l];
+            const double dsl = DREGS ? Ds[DREGS ? l : 0] : s_D[j + N1 * l];
             S1_r += drl * S1[pr];
             S2_r += drl * S2[pr];
             S0_s += dsl * S0[ps];
