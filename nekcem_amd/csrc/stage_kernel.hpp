@@ -564,12 +564,16 @@ stage_kernel(StageArgs a) {
 #pragma unroll
         for (int l = 0; l < N1; ++l) Ds[l] = s_D[j + N1 * l];
     }
-    // t derivative at N = 8, 9 with whole waves per role: the level k is
-    // the same across the wave, so lane l reads D(k, l) once per level and
-    // the wave takes the N+1 entries from lanes 0..N (always active) by
-    // readlane, in place of N+1 LDS reads (not in the free-budget source /
-    // Drude instance at N = 8, whose registers it would push past 128)
-    constexpr bool DTL = N1 >= 9 && !PACK && (FEAT == 0 || (FEAT == 1 && (OCC || N1 == 10)));
+    // t derivative with whole waves per role: the level k is the same
+    // across the wave, so lane l reads D(k, l) once per level and the wave
+    // takes the N+1 entries from lanes 0..N (always active) by readlane, in
+    // place of N+1 LDS reads.  At N = 8, 9 the plain and source / Drude
+    // instances (not the free-budget source / Drude one at N = 8, whose
+    // registers it would push past 128); at N = 5, 6 the plain ones, at
+    // N = 7 the plain 4-waves one (the free-budget one would pass 128 and
+    // lose its fourth workgroup per CU)
+    constexpr bool DTL = (N1 >= 9 && !PACK && (FEAT == 0 || (FEAT == 1 && (OCC || N1 == 10)))) ||
+                         (N1 >= 6 && N1 <= 8 && !PACK && FEAT == 0 && (OCC || N1 <= 7));
     const int wl = (int)threadIdx.x & 63;
     double minv = 0.0;
     if (MAT == MAT_ELEM) minv = grp ? a.ieps[e] : a.imu[e];
