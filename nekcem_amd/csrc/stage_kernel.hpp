@@ -414,7 +414,7 @@ stage_kernel(StageArgs a) {
     // ---- phase 0: every global load of the element, issued up front ----
     // At N = 7 the neighbours' traces go first: they then land with the
     // element's own streams (one in-order counter) instead of holding up
-    // the face phase (578 against 582 us per stage at 32^3); at the other
+    // the face phase (580.7 against 584.7 us per stage at 32^3); at the other
     // orders they follow the own streams (first measured 0.4-0.9 % slower)
     constexpr bool NBR_FIRST = N1 == 8 && FEAT < 2;
     double pre[NBR_FIRST ? 12 : 1];
